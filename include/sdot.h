@@ -1,0 +1,184 @@
+/*
+ * sdot.h -- C ABI of the MI355X (gfx950) regularized semi-discrete OT dual
+ * evaluation.  Plain pointers and sizes only; no exceptions cross this
+ * boundary (every call returns an sdot_status, details via sdot_last_error).
+ *
+ * Each entry point names the member of the reference's C++ class
+ * SotSolver<dim,spacedim> (include/SemiDiscreteOT/solvers/SotSolver.h) that
+ * it replaces.  Host arrays are copied at the call; the library owns every
+ * device buffer and frees it in sdot_ctx_destroy.  One context per host
+ * thread; calls on a context are ordered on the context's HIP stream.
+ */
+#ifndef SDOT_H
+#define SDOT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sdot_ctx sdot_ctx;
+
+typedef enum {
+  SDOT_OK = 0,
+  SDOT_ERR_INVALID = -1,   /* bad argument / shape (reference: AssertThrow, runtime_error) */
+  SDOT_ERR_STATE = -2,     /* source/target not set (SotSolver.cc:110-112) */
+  SDOT_ERR_HIP = -3,       /* HIP runtime failure */
+  SDOT_ERR_NOMEM = -4,
+  SDOT_ERR_PARSE = -5,     /* .prm parse error */
+  SDOT_ERR_UNSUPPORTED = -6
+} sdot_status;
+
+/* Outcome of a solve (the reference keeps non-convergence out of the
+ * exception path: SotSolver.cc:207-211). */
+typedef enum {
+  SDOT_SOLVE_CONVERGED = 0,
+  SDOT_SOLVE_MAX_ITER = 1,
+  SDOT_SOLVE_LINESEARCH_FAIL = 2,
+  SDOT_SOLVE_NONFINITE = 3
+} sdot_solve_status;
+
+typedef enum { SDOT_THRESH_POINTWISE = 0, SDOT_THRESH_INTEGRAL = 1, SDOT_THRESH_GEOMETRIC = 2 } sdot_threshold_type;
+typedef enum { SDOT_CTRL_L1NORM = 0, SDOT_CTRL_COMPONENTWISE = 1 } sdot_control_type;
+
+/* Arithmetic of the pair kernels.
+ *  FP64:     everything in double (parity mode, matches the FP64 CPU path).
+ *  FP32_LSE: exponent argument and max in double, exp2 in FP32, sums and
+ *            every reduction in double.  Plain (non-LSE) mode always runs FP64. */
+typedef enum { SDOT_PREC_FP64 = 0, SDOT_PREC_FP32_LSE = 1 } sdot_precision;
+
+/* SotParameterManager::SolverParameters (ParameterManager.h:56-72), the keys
+ * of subsection rsot_solver (ParameterManager.cc:64-95), plus `precision`. */
+typedef struct {
+  uint32_t max_iterations;        /* 1000 */
+  double tolerance;               /* 1 */
+  int32_t solver_control_type;    /* sdot_control_type, "l1norm" */
+  double epsilon;                 /* 1e-3 */
+  double tau;                     /* 1e-8 */
+  int32_t distance_threshold_type;/* sdot_threshold_type, "pointwise" */
+  int32_t verbose_output;         /* 1 */
+  int32_t solver_type;            /* 0 = BFGS (the only type) */
+  uint32_t quadrature_order;      /* 3 */
+  uint32_t nb_points;             /* 1000 */
+  uint32_t number_of_threads;     /* 0 */
+  int32_t use_epsilon_scaling;    /* 0 */
+  double epsilon_scaling_factor;  /* 2.0 */
+  uint32_t epsilon_scaling_steps; /* 5 */
+  int32_t use_log_sum_exp_trick;  /* 0 */
+  int32_t precision;              /* sdot_precision, FP64 */
+} sdot_params;
+
+typedef struct {
+  int32_t status;          /* sdot_solve_status */
+  int32_t iterations;      /* SotSolver::get_last_iteration_count */
+  int32_t evaluations;     /* functional evaluations incl. line search */
+  double functional;       /* SotSolver::get_last_functional_value */
+  double check_value;      /* last stopping-test value */
+  double seconds;
+} sdot_solve_info;
+
+typedef struct {
+  double last_functional;        /* get_last_functional_value (SotSolver.h:240) */
+  double C_global;               /* get_C_global (SotSolver.h:256) */
+  double last_distance_threshold;/* get_last_distance_threshold (SotSolver.h:252) */
+  double covering_radius;        /* R0 x 1.1 after a solve (SotSolver.cc:132) */
+  double min_target_density;     /* SotSolver.cc:133 */
+  int32_t last_iteration_count;  /* get_last_iteration_count (SotSolver.cc:572-576) */
+  int32_t converged;             /* get_convergence_status (SotSolver.cc:578-582) */
+  int64_t last_pair_count;       /* (q, candidate) pairs of the last evaluation, when counted */
+} sdot_state;
+
+/* Kernel timing gathered with HIP events on the context stream. */
+typedef struct {
+  int64_t evaluations;
+  double ms_lse_max, ms_lse_sum, ms_qfin, ms_grad, ms_total;
+} sdot_profile;
+
+/* ---- library / context ---------------------------------------------- */
+const char *sdot_version(void);
+/* Thread-local message of the last failing call on this thread. */
+const char *sdot_last_error(void);
+/* SotSolver::SotSolver(comm) (SotSolver.cc:3-20).  device = HIP ordinal. */
+int sdot_ctx_create(int device, sdot_ctx **out);
+int sdot_ctx_destroy(sdot_ctx *ctx);
+
+/* ---- parameters --------------------------------------------------------- */
+/* Defaults of SolverParameters (ParameterManager.h:56-72). */
+int sdot_params_default(sdot_params *p);
+/* Parse `subsection rsot_solver ... end` of a deal.II .prm text
+ * (ParameterManager.cc:64-95); keys outside it are ignored. */
+int sdot_params_parse_prm(const char *text, sdot_params *p);
+int sdot_params_load_prm(const char *path, sdot_params *p);
+/* SotSolver::configure (SotSolver.cc:63-70). */
+int sdot_set_params(sdot_ctx *ctx, const sdot_params *p);
+
+/* ---- measures ----------------------------------------------------------- */
+/* SotSolver::setup_source (SotSolver.h:186-190, SotSolver.cc:44-53): the
+ * values FEValues produces per locally owned cell, pulled once.
+ *   x       [n_cells*nq*dim]  quadrature points, cell-major, q fastest
+ *   w       [n_cells*nq]      density(x_q) * JxW_q
+ *   centers [n_cells*dim]     cell centers (candidate query points, SotSolver.cc:801) */
+int sdot_set_source(sdot_ctx *ctx, int dim, int64_t n_cells, int nq,
+                    const double *x, const double *w, const double *centers);
+/* SotSolver::setup_target (SotSolver.h:197-198, SotSolver.cc:55-61). */
+int sdot_set_target(sdot_ctx *ctx, int64_t n, const double *y, const double *nu);
+
+/* ---- the hot path --------------------------------------------------------- */
+/* SotSolver::evaluate_functional (SotSolver.h:335-336, SotSolver.cc:217-377):
+ * threshold from the stored state, assembly, linear term.  grad may be NULL. */
+int sdot_eval(sdot_ctx *ctx, const double *psi, double *grad, double *F, double *C_global);
+/* Same with an explicit distance threshold R (no state update). */
+int sdot_eval_R(sdot_ctx *ctx, double R, const double *psi, double *grad, double *F,
+                double *C_global);
+/* Device-pointer variant for callers whose psi/grad already live in HBM:
+ * d_psi, d_grad: N doubles in original target order; d_FC: 2 doubles (F, C).
+ * Asynchronous on the context stream (sdot_synchronize to wait). */
+int sdot_eval_device(sdot_ctx *ctx, double R, const double *d_psi, double *d_grad, double *d_FC);
+int sdot_synchronize(sdot_ctx *ctx);
+
+/* SotSolver::compute_hessian (declared at SotSolver.h:338-344, no reference
+ * definition): dense H (row-major, leading dimension ld >= N) of the
+ * functional at psi with the current threshold rule. */
+int sdot_hessian(sdot_ctx *ctx, const double *psi, double *H, int64_t ld);
+
+/* SotSolver::solve (SotSolver.h:211-212, SotSolver.cc:105-215): L-BFGS with
+ * the reference's line search and stopping test; psi in/out (size N). */
+int sdot_solve(sdot_ctx *ctx, double *psi, sdot_solve_info *info);
+
+/* ---- thresholds and audit ------------------------------------------------ */
+/* compute_covering_radius (SotSolver.cc:431-473), raw (without the 1.1). */
+int sdot_covering_radius(sdot_ctx *ctx, double *r0);
+/* compute_distance_threshold (SotSolver.cc:379-429) for psi with the state. */
+int sdot_distance_threshold(sdot_ctx *ctx, const double *psi, double *R);
+/* set_distance_threshold (SotSolver.h:262). */
+int sdot_set_distance_threshold(sdot_ctx *ctx, double R);
+/* Candidate set of every cell under threshold R (find_nearest_target_points,
+ * SotSolver.cc:553-570), as count and order-free hash per cell (original
+ * cell order): hash = sum over members j of mix(j+1), mix(v) = v*0x9E3779B97F4A7C15,
+ * v ^= v>>29.  Returns the total pair count in *pairs (may be NULL). */
+int sdot_candidate_stats(sdot_ctx *ctx, double R, int64_t *count, uint64_t *hash,
+                         int64_t *pairs);
+int sdot_get_state(sdot_ctx *ctx, sdot_state *st);
+int sdot_set_state(sdot_ctx *ctx, const sdot_state *st);
+int sdot_profile_enable(sdot_ctx *ctx, int on);
+int sdot_profile_get(sdot_ctx *ctx, sdot_profile *out);
+
+/* ---- source-measure extraction (the FEValues stand-in) --------------------- */
+/* Structured Q1 box mesh (GridGenerator::subdivided_hyper_rectangle), QGauss
+ * of `qorder` points per direction, Q1-interpolated nodal density (vertices
+ * lexicographic, x fastest; NULL = 1).  Outputs sized n_cells*nq*dim etc.,
+ * with w = rho*JxW.  If normalize, the nodal density is divided by
+ * sum |rho_q| JxW_q first (normalize_density, SemiDiscreteOT.cc:422-449). */
+int sdot_mesh_structured(int dim, const int *n_sub, const double *lo, const double *hi,
+                         int qorder, const double *nodal_rho, int normalize,
+                         double *x, double *w, double *centers);
+/* P1 simplices (triangles/tets) with QGaussSimplex(qorder), qorder 1 or 2. */
+int sdot_mesh_simplex(int dim, int64_t n_vertices, const double *vertices, int64_t n_cells,
+                      const int64_t *cells, int qorder, const double *nodal_rho,
+                      int normalize, double *x, double *w, double *centers);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,13 @@
+"""semidiscreteot_amd -- MI355X (gfx950) regularized semi-discrete optimal
+transport dual evaluation, a drop-in for the functional / gradient / Hessian
+assembly behind SemiDiscreteOT's SotSolver and its BFGS driver.
+
+The compute path is libsdot.so (HIP kernels + C ABI, include/sdot.h); this
+package is the host-side mirror of the reference's SotSolver interface.
+"""
+from ._lib import LIB_PATH, SdotError, load  # noqa: F401
+from .mesh import SourceArrays, simplex_source, structured_source  # noqa: F401
+from .solver import SolverParameters, SotSolver  # noqa: F401
+
+__all__ = ["SotSolver", "SolverParameters", "SourceArrays", "structured_source",
+           "simplex_source", "SdotError", "load", "LIB_PATH"]

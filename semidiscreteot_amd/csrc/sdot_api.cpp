@@ -1,0 +1,835 @@
+// sdot_api.cpp -- the C ABI (include/sdot.h): solver context, device layout
+// of the measures, distance thresholds, the BFGS driver and the stopping test.
+// Everything numerical runs in the HIP kernels of sdot_kernels.hip; this file
+// only prepares layouts, computes the scalar truncation radius on the host in
+// FP64 (so that candidate sets are bit-identical to the FP64 CPU evaluation)
+// and drives L-BFGS.
+#include <algorithm>
+#include <array>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "sdot_bfgs.hpp"
+#include "sdot_internal.hpp"
+
+using namespace sdot;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t _e = (expr);                                                            \
+    if (_e != hipSuccess)                                                              \
+      return fail(SDOT_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));    \
+  } while (0)
+
+template <class T>
+hipError_t dalloc(T *&p, size_t n) {
+  if (p) {
+    (void)hipFree(p);
+    p = nullptr;
+  }
+  if (n == 0) n = 1;
+  return hipMalloc((void **)&p, n * sizeof(T));
+}
+
+template <class T>
+hipError_t upload(T *&p, const std::vector<T> &h) {
+  hipError_t e = dalloc(p, h.size());
+  if (e != hipSuccess) return e;
+  if (h.empty()) return hipSuccess;
+  return hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+bool all_finite(const double *a, int64_t n) {
+  for (int64_t i = 0; i < n; ++i)
+    if (!std::isfinite(a[i])) return false;
+  return true;
+}
+
+// Morton (Z-order) key of a point inside [lo, hi].
+uint64_t morton_key(const double *p, const double *lo, const double *hi, int dim) {
+  const int bits = dim == 3 ? 21 : 31;
+  const double scale = (double)((1u << (bits - 1)) - 1) * 2.0;
+  uint64_t key = 0;
+  uint64_t q[3] = {0, 0, 0};
+  for (int d = 0; d < dim; ++d) {
+    const double ext = hi[d] - lo[d];
+    double u = ext > 0 ? (p[d] - lo[d]) / ext : 0.0;
+    u = std::min(1.0, std::max(0.0, u));
+    q[d] = (uint64_t)(u * scale);
+  }
+  for (int b = bits - 1; b >= 0; --b)
+    for (int d = dim - 1; d >= 0; --d) key = (key << 1) | ((q[d] >> b) & 1u);
+  return key;
+}
+
+// Uniform grid over [lo, hi] with about `per_bin` points per bin.
+Grid make_grid(const double *lo, const double *hi, int dim, int64_t n, double per_bin) {
+  Grid g{};
+  double ext[3] = {1, 1, 1}, maxext = 0;
+  for (int d = 0; d < dim; ++d) maxext = std::max(maxext, hi[d] - lo[d]);
+  if (!(maxext > 0)) maxext = 1.0;
+  double vol = 1.0;
+  for (int d = 0; d < dim; ++d) {
+    ext[d] = std::max(hi[d] - lo[d], maxext * 1e-6);
+    vol *= ext[d];
+  }
+  double h = std::pow(vol * per_bin / (double)std::max<int64_t>(n, 1), 1.0 / dim);
+  auto nbins = [&](double hh) {
+    double t = 1;
+    for (int d = 0; d < dim; ++d) t *= std::max(1.0, std::ceil(ext[d] / hh));
+    return t;
+  };
+  while (nbins(h) > (double)(1 << 22)) h *= 1.25;
+  g.h = h;
+  g.inv_h = 1.0 / h;
+  for (int d = 0; d < 3; ++d) {
+    g.o[d] = d < dim ? lo[d] : 0.0;
+    g.n[d] = d < dim ? (int)std::max(1.0, std::ceil(ext[d] / h)) : 1;
+  }
+  return g;
+}
+
+int bin_linear(const Grid &g, const double *p, int dim) {
+  int b[3] = {0, 0, 0};
+  for (int d = 0; d < dim; ++d) b[d] = bin_of(p[d], g.o[d], g.inv_h, g.n[d]);
+  return (b[2] * g.n[1] + b[1]) * g.n[0] + b[0];
+}
+
+// Largest double s with sqrt(s) <= R (IEEE sqrt is correctly rounded and
+// monotone), so that `s <= smax` is the reference's `sqrt(s) <= R`.
+double smax_for(double R) {
+  if (std::isnan(R) || R < 0) return -1.0;
+  if (std::isinf(R)) return INFINITY;
+  double s = R * R;
+  if (std::isinf(s)) s = DBL_MAX;
+  while (s > 0 && std::sqrt(s) > R) s = std::nextafter(s, 0.0);
+  for (;;) {
+    const double n = std::nextafter(s, INFINITY);
+    if (std::isinf(n) || std::sqrt(n) > R) break;
+    s = n;
+  }
+  return s;
+}
+
+constexpr int kMaxSeg = 128;
+constexpr int kProfSets = 128;
+
+}  // namespace
+
+struct sdot_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  sdot_params params{};
+  // members SotSolver keeps between evaluations (SotSolver.h:505-514)
+  double C_global = 0.0, global_functional = 0.0, current_threshold = 0.1;
+  double covering_radius = 0.0, min_target_density = 0.0;
+  int last_iterations = 0, converged = 0;
+  int64_t last_pairs = -1;
+  bool have_source = false, have_target = false;
+  Plan plan;
+  Dev dev;
+  std::vector<double> h_nu;
+  double src_lo[3] = {0, 0, 0}, src_hi[3] = {0, 0, 0};
+  double tgt_lo[3] = {0, 0, 0}, tgt_hi[3] = {0, 0, 0};
+  double coord_scale = 1.0;
+  size_t cap_trec = 0, cap_qrec = 0, cap_M = 0, cap_G = 0, cap_F = 0, cap_P = 0, cap_N = 0;
+  // profiling: a pool of event sets resolved at synchronisation points
+  bool prof_on = false;
+  std::vector<std::array<hipEvent_t, 6>> ev_pool;
+  int ev_pending = 0;
+  sdot_profile prof_acc{};
+  std::vector<double> h_tmp, h_tmp2;
+};
+
+namespace {
+
+int ctx_check(sdot_ctx *c) {
+  if (!c) return fail(SDOT_ERR_INVALID, "null context");
+  return SDOT_OK;
+}
+
+int resolve_profile(sdot_ctx *c) {
+  if (c->ev_pending == 0) return SDOT_OK;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < c->ev_pending; ++i) {
+    auto &ev = c->ev_pool[i];
+    float t[5] = {0, 0, 0, 0, 0};
+    for (int k = 0; k < 4; ++k) HIP_TRY(hipEventElapsedTime(&t[k], ev[k], ev[k + 1]));
+    HIP_TRY(hipEventElapsedTime(&t[4], ev[0], ev[5]));
+    c->prof_acc.ms_lse_max += t[0];
+    c->prof_acc.ms_lse_sum += t[1];
+    c->prof_acc.ms_qfin += t[2];
+    c->prof_acc.ms_grad += t[3];
+    c->prof_acc.ms_total += t[4];
+    c->prof_acc.evaluations += 1;
+  }
+  c->ev_pending = 0;
+  return SDOT_OK;
+}
+
+// Choose segment counts and make sure every per-evaluation buffer fits.
+int prepare(sdot_ctx *c, double R) {
+  if (!c->have_source || !c->have_target)
+    return fail(SDOT_ERR_STATE, "Invalid measures configuration: source and target must be set");
+  Plan &p = c->plan;
+  p.eps = c->params.epsilon;
+  if (!(p.eps > 0) || !std::isfinite(p.eps)) return fail(SDOT_ERR_INVALID, "epsilon must be > 0");
+  const bool lse = c->params.use_log_sum_exp_trick != 0;
+  p.mode = !lse ? kF64Plain : (c->params.precision == SDOT_PREC_FP32_LSE ? kF32Lse : kF64Lse);
+  // dense when every center-target distance is surely <= R
+  double diag2 = 0.0;
+  for (int d = 0; d < p.dim; ++d) {
+    const double lo = std::min(c->src_lo[d], c->tgt_lo[d]);
+    const double hi = std::max(c->src_hi[d], c->tgt_hi[d]);
+    diag2 += (hi - lo) * (hi - lo);
+  }
+  const double diag = std::sqrt(diag2);
+  p.smax = smax_for(R);
+  p.dense = (R >= diag * (1.0 + 1e-6) + 1e-300) ? 1 : 0;
+  p.Rq = R * (1.0 + 1e-9) + 1e-9 * (p.tg.h + p.cg.h + c->coord_scale);
+  if (!std::isfinite(p.Rq)) p.dense = std::isnan(R) ? 0 : 1;
+  if (std::isnan(R)) p.Rq = 0.0;
+  const int64_t target_waves = 8192;
+  int64_t K1 = std::max<int64_t>(1, target_waves / std::max<int64_t>(1, p.n_blocks));
+  int64_t K2 = std::max<int64_t>(1, target_waves / std::max<int64_t>(1, p.n_twaves));
+  if (p.dense) {
+    K1 = std::min<int64_t>(K1, std::max<int64_t>(1, p.N / 32));
+    K2 = std::min<int64_t>(K2, std::max<int64_t>(1, p.n_cells / 8));
+  }
+  p.K1 = (int)std::min<int64_t>(K1, kMaxSeg);
+  p.K2 = (int)std::min<int64_t>(K2, kMaxSeg);
+  p.nFpart = (p.nQ + 255) / 256;
+  p.nPpart = (p.N + 255) / 256;
+  Dev &d = c->dev;
+  const size_t needM = (size_t)p.K1 * p.nQ, needG = (size_t)p.K2 * p.N;
+  if (c->cap_M < needM) {
+    HIP_TRY(dalloc(d.Mpart, needM));
+    HIP_TRY(dalloc(d.Spart, needM));
+    c->cap_M = needM;
+  }
+  if (c->cap_G < needG) {
+    HIP_TRY(dalloc(d.Gpart, needG));
+    c->cap_G = needG;
+  }
+  if (c->cap_trec < (size_t)p.N * kRec) {
+    HIP_TRY(dalloc(d.trec_r, (size_t)p.N * kRec));
+    HIP_TRY(dalloc(d.trec_m, (size_t)p.N * kRec));
+    c->cap_trec = (size_t)p.N * kRec;
+  }
+  if (c->cap_qrec < (size_t)p.nQ * kRec) {
+    HIP_TRY(dalloc(d.qrec_r, (size_t)p.nQ * kRec));
+    c->cap_qrec = (size_t)p.nQ * kRec;
+  }
+  if (c->cap_F < (size_t)p.nFpart) {
+    HIP_TRY(dalloc(d.Fpart, p.nFpart));
+    HIP_TRY(dalloc(d.Cpart, p.nFpart));
+    c->cap_F = p.nFpart;
+  }
+  if (c->cap_P < (size_t)p.nPpart) {
+    HIP_TRY(dalloc(d.PNpart, p.nPpart));
+    c->cap_P = p.nPpart;
+  }
+  if (c->cap_N < (size_t)p.N) {
+    HIP_TRY(dalloc(d.psi, p.N));
+    HIP_TRY(dalloc(d.grad, p.N));
+    c->cap_N = p.N;
+  }
+  if (!d.out) HIP_TRY(dalloc(d.out, 4));
+  return SDOT_OK;
+}
+
+int run_eval(sdot_ctx *c, double R, const double *d_psi, double *d_grad, double *d_FC) {
+  int rc = prepare(c, R);
+  if (rc) return rc;
+  Profile pr;
+  Profile *pp = nullptr;
+  if (c->prof_on) {
+    if (c->ev_pending == kProfSets) {
+      rc = resolve_profile(c);
+      if (rc) return rc;
+    }
+    pr.on = true;
+    for (int k = 0; k < 6; ++k) pr.ev[k] = c->ev_pool[c->ev_pending][k];
+    c->ev_pending++;
+    pp = &pr;
+  }
+  HIP_TRY(launch_eval(c->plan, c->dev, d_psi, d_grad, d_FC, c->stream, pp));
+  return SDOT_OK;
+}
+
+int host_eval(sdot_ctx *c, double R, const double *psi, double *grad, double *F, double *C) {
+  if (!psi) return fail(SDOT_ERR_INVALID, "psi is null");
+  const int64_t N = c->plan.N;
+  if (!c->have_source || !c->have_target)
+    return fail(SDOT_ERR_STATE, "Invalid measures configuration: source and target must be set");
+  int rc = prepare(c, R);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(c->dev.psi, psi, sizeof(double) * N, hipMemcpyHostToDevice, c->stream));
+  rc = run_eval(c, R, c->dev.psi, c->dev.grad, c->dev.out);
+  if (rc) return rc;
+  double fc[2];
+  if (grad)
+    HIP_TRY(hipMemcpyAsync(grad, c->dev.grad, sizeof(double) * N, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(fc, c->dev.out, sizeof(double) * 2, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (c->prof_on) {
+    rc = resolve_profile(c);
+    if (rc) return rc;
+  }
+  if (F) *F = fc[0];
+  if (C) *C = fc[1];
+  return SDOT_OK;
+}
+
+int raw_covering_radius(sdot_ctx *c, double *r0) {
+  const Plan &p = c->plan;
+  double *d_min = nullptr;
+  HIP_TRY(hipMalloc((void **)&d_min, sizeof(double) * std::max<int64_t>(1, p.n_cells)));
+  hipError_t e = launch_covering(p, c->dev, d_min, c->stream);
+  std::vector<double> h(p.n_cells);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(h.data(), d_min, sizeof(double) * p.n_cells, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d_min);
+  if (e != hipSuccess) return fail(SDOT_ERR_HIP, std::string("covering radius: ") + hipGetErrorString(e));
+  double m = 0.0;
+  for (double v : h) m = std::max(m, v);
+  *r0 = std::sqrt(m);
+  return SDOT_OK;
+}
+
+double vmax(const double *v, int64_t n) {
+  double m = v[0];
+  for (int64_t i = 1; i < n; ++i) m = v[i] > m ? v[i] : m;
+  return m;
+}
+double vmin(const double *v, int64_t n) {
+  double m = v[0];
+  for (int64_t i = 1; i < n; ++i) m = v[i] < m ? v[i] : m;
+  return m;
+}
+
+// compute_distance_threshold (SotSolver.cc:379-429) with its three bounds:
+// integral (SotSolver.cc:475-507), geometric (SotSolver.cc:509-551) and
+// pointwise (SotSolver.cc:406-419).
+int threshold(sdot_ctx *c, const double *psi, double *R) {
+  const int64_t N = c->plan.N;
+  const double eps = c->params.epsilon, tau = c->params.tau;
+  const double mx = vmax(psi, N);
+  const int type = c->params.distance_threshold_type;
+  if (type == SDOT_THRESH_INTEGRAL) {
+    double absF = std::fabs(c->global_functional);
+    if (absF < 1e-10) absF = 1e-10;
+    const double r2 = 2.0 * mx + 2.0 * eps * std::log((eps * c->C_global) / (tau * absF));
+    double r = std::sqrt(0.0 < r2 ? r2 : 0.0);
+    *R = r < c->covering_radius ? c->covering_radius : r;
+  } else if (type == SDOT_THRESH_GEOMETRIC) {
+    double r0 = c->covering_radius;
+    if (!(r0 > 0.0)) {
+      int rc = raw_covering_radius(c, &r0);
+      if (rc) return rc;
+    }
+    const double range = mx - vmin(psi, N);
+    const double mind = c->min_target_density > 0.0 ? c->min_target_density : vmin(c->h_nu.data(), N);
+    double fv = std::fabs(c->global_functional);
+    if (fv < 1e-10) fv = 1e-10;
+    const double lt = std::log(eps / (mind * tau * fv));
+    double r2 = r0 * r0 + 2.0 * range + 2.0 * eps * lt;
+    if (r2 < r0 * r0) r2 = r0 * r0;
+    *R = std::sqrt(r2);
+  } else {
+    const double mind = c->min_target_density > 0.0 ? c->min_target_density : vmin(c->h_nu.data(), N);
+    if (mind <= 0 || tau <= 0) {
+      *R = DBL_MAX;
+    } else {
+      const double r2 = -2.0 * eps * std::log(tau / mind) + 2.0 * mx;
+      *R = std::sqrt(0.0 < r2 ? r2 : 0.0);
+    }
+  }
+  return SDOT_OK;
+}
+
+// evaluate_functional: threshold from the stored state, assembly, state update.
+int stateful_eval(sdot_ctx *c, const double *psi, double *grad, double *F, double *C) {
+  double R;
+  int rc = threshold(c, psi, &R);
+  if (rc) return rc;
+  c->current_threshold = R;
+  double f = 0, cg = 0;
+  rc = host_eval(c, R, psi, grad, &f, &cg);
+  if (rc) return rc;
+  c->global_functional = f;
+  c->C_global = cg;
+  if (F) *F = f;
+  if (C) *C = cg;
+  return SDOT_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+
+extern "C" const char *sdot_version(void) { return "sdot-mi355x 0.1 (gfx950)"; }
+
+extern "C" const char *sdot_last_error(void) { return g_err.c_str(); }
+
+extern "C" int sdot_ctx_create(int device, sdot_ctx **out) {
+  if (!out) return fail(SDOT_ERR_INVALID, "out is null");
+  *out = nullptr;
+  int n = 0;
+  HIP_TRY(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(SDOT_ERR_INVALID, "no such HIP device");
+  HIP_TRY(hipSetDevice(device));
+  auto *c = new sdot_ctx();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(SDOT_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  sdot_params_default(&c->params);
+  *out = c;
+  return SDOT_OK;
+}
+
+extern "C" int sdot_ctx_destroy(sdot_ctx *c) {
+  if (!c) return SDOT_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  Dev &d = c->dev;
+  void *bufs[] = {d.qx_m, d.qw_m, d.qx2_m, d.cc_m, d.blkbox, d.cc_r, d.cstart, d.cell_r2m,
+                  d.cell_m2o, d.ty_o, d.tnu_o, d.tpos_r, d.tpos_m, d.tr2o, d.tm2o, d.tstart,
+                  d.twbox, d.ty_m, d.trec_r, d.trec_m, d.qrec_r, d.Mpart, d.Spart, d.Gpart,
+                  d.Fpart, d.Cpart, d.PNpart, d.psi, d.grad, d.out, d.scratch, d.i64scratch};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  for (auto &set : c->ev_pool)
+    for (auto e : set) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return SDOT_OK;
+}
+
+extern "C" int sdot_set_params(sdot_ctx *c, const sdot_params *p) {
+  if (ctx_check(c) || !p) return fail(SDOT_ERR_INVALID, "null argument");
+  if (!(p->epsilon > 0) || !std::isfinite(p->epsilon)) return fail(SDOT_ERR_INVALID, "epsilon must be finite and > 0");
+  if (p->precision != SDOT_PREC_FP64 && p->precision != SDOT_PREC_FP32_LSE)
+    return fail(SDOT_ERR_INVALID, "unknown precision");
+  c->params = *p;
+  return SDOT_OK;
+}
+
+extern "C" int sdot_set_source(sdot_ctx *c, int dim, int64_t n_cells, int nq, const double *x,
+                               const double *w, const double *centers) {
+  if (ctx_check(c)) return SDOT_ERR_INVALID;
+  if (dim != 2 && dim != 3) return fail(SDOT_ERR_INVALID, "dim must be 2 or 3");
+  if (n_cells < 1 || n_cells >= (1ll << 31) - 1) return fail(SDOT_ERR_INVALID, "n_cells out of range");
+  if (nq < 1 || nq > 4096) return fail(SDOT_ERR_INVALID, "nq out of range");
+  if (!x || !w || !centers) return fail(SDOT_ERR_INVALID, "null source array");
+  if (c->have_target && dim != c->plan.dim)
+    return fail(SDOT_ERR_INVALID, "source dim differs from the target dim");
+  const int64_t nQ = n_cells * nq;
+  if (!all_finite(x, nQ * dim) || !all_finite(w, nQ) || !all_finite(centers, n_cells * dim))
+    return fail(SDOT_ERR_INVALID, "non-finite source values");
+  HIP_TRY(hipSetDevice(c->device));
+  Plan &p = c->plan;
+  p.dim = dim;
+  p.n_cells = n_cells;
+  p.nq = nq;
+  p.nQ = nQ;
+  int G = 1;
+  for (int g = 9; g >= 1; --g)
+    if (nq % g == 0) {
+      G = g;
+      break;
+    }
+  p.G = G;
+  p.gpc = nq / G;
+  p.n_groups = n_cells * p.gpc;
+  p.n_blocks = (p.n_groups + kWave - 1) / kWave;
+
+  // bounding boxes; the shift origin is the middle of the quadrature-point box
+  double qlo[3], qhi[3], clo[3], chi[3];
+  for (int d = 0; d < dim; ++d) {
+    qlo[d] = clo[d] = INFINITY;
+    qhi[d] = chi[d] = -INFINITY;
+  }
+  for (int64_t i = 0; i < nQ; ++i)
+    for (int d = 0; d < dim; ++d) {
+      qlo[d] = std::min(qlo[d], x[i * dim + d]);
+      qhi[d] = std::max(qhi[d], x[i * dim + d]);
+    }
+  for (int64_t i = 0; i < n_cells; ++i)
+    for (int d = 0; d < dim; ++d) {
+      clo[d] = std::min(clo[d], centers[i * dim + d]);
+      chi[d] = std::max(chi[d], centers[i * dim + d]);
+    }
+  double scale = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    p.origin[d] = d < dim ? 0.5 * (qlo[d] + qhi[d]) : 0.0;
+    c->src_lo[d] = d < dim ? clo[d] : 0.0;
+    c->src_hi[d] = d < dim ? chi[d] : 0.0;
+    if (d < dim) scale = std::max({scale, std::fabs(clo[d]), std::fabs(chi[d])});
+  }
+  c->coord_scale = std::max(c->coord_scale, scale);
+
+  // Morton order of the cells (q-major passes)
+  std::vector<uint64_t> key(n_cells);
+  for (int64_t i = 0; i < n_cells; ++i) key[i] = morton_key(centers + i * dim, clo, chi, dim);
+  std::vector<int64_t> m2o(n_cells);
+  std::iota(m2o.begin(), m2o.end(), 0);
+  std::stable_sort(m2o.begin(), m2o.end(), [&](int64_t a, int64_t b) { return key[a] < key[b]; });
+  std::vector<double> qx((size_t)nQ * dim), qw(nQ), qx2(nQ), ccm((size_t)n_cells * dim);
+  for (int64_t m = 0; m < n_cells; ++m) {
+    const int64_t o = m2o[m];
+    for (int d = 0; d < dim; ++d) ccm[m * dim + d] = centers[o * dim + d];
+    for (int k = 0; k < nq; ++k) {
+      double s2 = 0.0;
+      for (int d = 0; d < dim; ++d) {
+        const double v = x[(o * nq + k) * dim + d] - p.origin[d];
+        qx[(m * nq + k) * dim + d] = v;
+        s2 += v * v;
+      }
+      qx2[m * nq + k] = s2;
+      qw[m * nq + k] = w[o * nq + k];
+    }
+  }
+  // AABB of the cell centers of each 64-group block
+  std::vector<double> box((size_t)p.n_blocks * 2 * dim);
+  for (int64_t b = 0; b < p.n_blocks; ++b) {
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    const int64_t g0 = b * kWave, g1 = std::min<int64_t>(p.n_groups, g0 + kWave);
+    for (int64_t g = g0; g < g1; ++g) {
+      const int64_t cm = g / p.gpc;
+      for (int d = 0; d < dim; ++d) {
+        lo[d] = std::min(lo[d], ccm[cm * dim + d]);
+        hi[d] = std::max(hi[d], ccm[cm * dim + d]);
+      }
+    }
+    for (int d = 0; d < dim; ++d) {
+      box[b * 2 * dim + d] = lo[d];
+      box[b * 2 * dim + dim + d] = hi[d];
+    }
+  }
+  // row (bin) order of the cells (target-major pass)
+  p.cg = make_grid(clo, chi, dim, n_cells, 8.0);
+  const int64_t ncb = (int64_t)p.cg.n[0] * p.cg.n[1] * p.cg.n[2];
+  std::vector<int> cbin(n_cells);
+  for (int64_t m = 0; m < n_cells; ++m) cbin[m] = bin_linear(p.cg, &ccm[m * dim], dim);
+  std::vector<int64_t> r2m(n_cells);
+  std::iota(r2m.begin(), r2m.end(), 0);
+  std::stable_sort(r2m.begin(), r2m.end(), [&](int64_t a, int64_t b) { return cbin[a] < cbin[b]; });
+  std::vector<int32_t> cstart(ncb + 1, 0);
+  for (int64_t m = 0; m < n_cells; ++m) cstart[cbin[m] + 1]++;
+  for (int64_t b = 0; b < ncb; ++b) cstart[b + 1] += cstart[b];
+  std::vector<double> ccr((size_t)n_cells * 4, 0.0);
+  for (int64_t r = 0; r < n_cells; ++r)
+    for (int d = 0; d < dim; ++d) ccr[r * 4 + d] = ccm[r2m[r] * dim + d];
+
+  Dev &dv = c->dev;
+  HIP_TRY(upload(dv.qx_m, qx));
+  HIP_TRY(upload(dv.qw_m, qw));
+  HIP_TRY(upload(dv.qx2_m, qx2));
+  HIP_TRY(upload(dv.cc_m, ccm));
+  HIP_TRY(upload(dv.blkbox, box));
+  HIP_TRY(upload(dv.cc_r, ccr));
+  HIP_TRY(upload(dv.cstart, cstart));
+  HIP_TRY(upload(dv.cell_r2m, r2m));
+  HIP_TRY(upload(dv.cell_m2o, m2o));
+  c->cap_M = c->cap_qrec = c->cap_F = 0;
+  c->have_source = true;
+  return SDOT_OK;
+}
+
+extern "C" int sdot_set_target(sdot_ctx *c, int64_t n, const double *y, const double *nu) {
+  if (ctx_check(c)) return SDOT_ERR_INVALID;
+  if (n < 1 || n >= (1ll << 31) - 1) return fail(SDOT_ERR_INVALID, "Target measure not set up (n must be >= 1)");
+  if (!y || !nu) return fail(SDOT_ERR_INVALID, "null target array");
+  if (!c->have_source) return fail(SDOT_ERR_STATE, "call sdot_set_source before sdot_set_target");
+  const int dim = c->plan.dim;
+  if (!all_finite(y, n * dim) || !all_finite(nu, n)) return fail(SDOT_ERR_INVALID, "non-finite target values");
+  HIP_TRY(hipSetDevice(c->device));
+  Plan &p = c->plan;
+  p.N = n;
+  p.n_twaves = (n + kWave - 1) / kWave;
+  double lo[3], hi[3];
+  for (int d = 0; d < dim; ++d) {
+    lo[d] = INFINITY;
+    hi[d] = -INFINITY;
+  }
+  for (int64_t j = 0; j < n; ++j)
+    for (int d = 0; d < dim; ++d) {
+      lo[d] = std::min(lo[d], y[j * dim + d]);
+      hi[d] = std::max(hi[d], y[j * dim + d]);
+    }
+  double scale = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    c->tgt_lo[d] = d < dim ? lo[d] : 0.0;
+    c->tgt_hi[d] = d < dim ? hi[d] : 0.0;
+    if (d < dim) scale = std::max({scale, std::fabs(lo[d]), std::fabs(hi[d])});
+  }
+  c->coord_scale = std::max(c->coord_scale, scale);
+  // Morton order (gradient waves)
+  std::vector<uint64_t> key(n);
+  for (int64_t j = 0; j < n; ++j) key[j] = morton_key(y + j * dim, lo, hi, dim);
+  std::vector<int32_t> m2o(n), pos_m(n);
+  std::iota(m2o.begin(), m2o.end(), 0);
+  std::stable_sort(m2o.begin(), m2o.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+  for (int64_t m = 0; m < n; ++m) pos_m[m2o[m]] = (int32_t)m;
+  std::vector<double> box((size_t)p.n_twaves * 2 * dim), tym((size_t)n * 4, 0.0);
+  for (int64_t w = 0; w < p.n_twaves; ++w) {
+    double bl[3] = {INFINITY, INFINITY, INFINITY}, bh[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t m = w * kWave; m < std::min<int64_t>(n, (w + 1) * kWave); ++m)
+      for (int d = 0; d < dim; ++d) {
+        bl[d] = std::min(bl[d], y[(int64_t)m2o[m] * dim + d]);
+        bh[d] = std::max(bh[d], y[(int64_t)m2o[m] * dim + d]);
+      }
+    for (int d = 0; d < dim; ++d) {
+      box[w * 2 * dim + d] = bl[d];
+      box[w * 2 * dim + dim + d] = bh[d];
+    }
+  }
+  for (int64_t m = 0; m < n; ++m)
+    for (int d = 0; d < dim; ++d) tym[m * 4 + d] = y[(int64_t)m2o[m] * dim + d];
+  // row (bin) order (q-major passes stream contiguous bin runs)
+  p.tg = make_grid(lo, hi, dim, n, 8.0);
+  const int64_t ntb = (int64_t)p.tg.n[0] * p.tg.n[1] * p.tg.n[2];
+  std::vector<int> tb(n);
+  for (int64_t j = 0; j < n; ++j) tb[j] = bin_linear(p.tg, y + j * dim, dim);
+  std::vector<int32_t> r2o(n), pos_r(n);
+  std::iota(r2o.begin(), r2o.end(), 0);
+  std::stable_sort(r2o.begin(), r2o.end(), [&](int32_t a, int32_t b) { return tb[a] < tb[b]; });
+  for (int64_t r = 0; r < n; ++r) pos_r[r2o[r]] = (int32_t)r;
+  std::vector<int32_t> tstart(ntb + 1, 0);
+  for (int64_t j = 0; j < n; ++j) tstart[tb[j] + 1]++;
+  for (int64_t b = 0; b < ntb; ++b) tstart[b + 1] += tstart[b];
+
+  Dev &dv = c->dev;
+  std::vector<double> yv(y, y + n * dim), nv(nu, nu + n);
+  HIP_TRY(upload(dv.ty_o, yv));
+  HIP_TRY(upload(dv.tnu_o, nv));
+  HIP_TRY(upload(dv.tpos_r, pos_r));
+  HIP_TRY(upload(dv.tpos_m, pos_m));
+  HIP_TRY(upload(dv.tr2o, r2o));
+  HIP_TRY(upload(dv.tm2o, m2o));
+  HIP_TRY(upload(dv.tstart, tstart));
+  HIP_TRY(upload(dv.twbox, box));
+  HIP_TRY(upload(dv.ty_m, tym));
+  c->h_nu = nv;
+  c->cap_G = c->cap_trec = c->cap_P = c->cap_N = 0;
+  c->have_target = true;
+  return SDOT_OK;
+}
+
+extern "C" int sdot_eval_R(sdot_ctx *c, double R, const double *psi, double *grad, double *F,
+                           double *C) {
+  if (ctx_check(c)) return SDOT_ERR_INVALID;
+  HIP_TRY(hipSetDevice(c->device));
+  return host_eval(c, R, psi, grad, F, C);
+}
+
+extern "C" int sdot_eval(sdot_ctx *c, const double *psi, double *grad, double *F, double *C) {
+  if (ctx_check(c)) return SDOT_ERR_INVALID;
+  if (!c->have_source || !c->have_target)
+    return fail(SDOT_ERR_STATE, "Invalid measures configuration: source and target must be set");
+  if (!psi) return fail(SDOT_ERR_INVALID, "psi is null");
+  HIP_TRY(hipSetDevice(c->device));
+  return stateful_eval(c, psi, grad, F, C);
+}
+
+extern "C" int sdot_eval_device(sdot_ctx *c, double R, const double *d_psi, double *d_grad,
+                                double *d_FC) {
+  if (ctx_check(c)) return SDOT_ERR_INVALID;
+  if (!d_psi || !d_grad || !d_FC) return fail(SDOT_ERR_INVALID, "null device pointer");
+  HIP_TRY(hipSetDevice(c->device));
+  return run_eval(c, R, d_psi, d_grad, d_FC);
+}
+
+extern "C" int sdot_synchronize(sdot_ctx *c) {
+  if (ctx_check(c)) return SDOT_ERR_INVALID;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return resolve_profile(c);
+}
+
+extern "C" int sdot_hessian(sdot_ctx *c, const double *psi, double *H, int64_t ld) {
+  if (ctx_check(c)) return SDOT_ERR_INVALID;
+  (void)psi;
+  (void)H;
+  (void)ld;
+  return fail(SDOT_ERR_UNSUPPORTED, "sdot_hessian: not built yet");
+}
+
+extern "C" int sdot_solve(sdot_ctx *c, double *psi, sdot_solve_info *info) {
+  if (ctx_check(c)) return SDOT_ERR_INVALID;
+  if (!c->have_source || !c->have_target)
+    return fail(SDOT_ERR_STATE, "Invalid measures configuration");
+  if (!psi) return fail(SDOT_ERR_INVALID, "psi is null");
+  HIP_TRY(hipSetDevice(c->device));
+  const auto t0 = std::chrono::steady_clock::now();
+  const int64_t N = c->plan.N;
+  double r0 = 0.0;
+  int rc = raw_covering_radius(c, &r0);
+  if (rc) return rc;
+  c->covering_radius = r0 * 1.1;
+  c->min_target_density = vmin(c->h_nu.data(), N);
+  int eval_rc = SDOT_OK;
+  auto eval = [&](const double *x, double *g) -> double {
+    double F = NAN;
+    if (eval_rc == SDOT_OK) {
+      eval_rc = stateful_eval(c, x, g, &F, nullptr);
+      if (eval_rc) {
+        F = NAN;
+        for (int64_t j = 0; j < N; ++j) g[j] = NAN;
+      }
+    }
+    return F;
+  };
+  const bool comp = c->params.solver_control_type == SDOT_CTRL_COMPONENTWISE;
+  const double tol = comp ? DBL_MIN : c->params.tolerance;
+  const double utol = c->params.tolerance;
+  const unsigned max_it = c->params.max_iterations;
+  auto check = [&](int step, const double *g, double &cv) -> int {
+    if (comp) {
+      cv = -INFINITY;
+      for (int64_t j = 0; j < N; ++j) cv = std::max(cv, std::fabs(g[j]) - c->h_nu[j] * utol);
+    } else {
+      cv = 0.0;
+      for (int64_t j = 0; j < N; ++j) cv += std::fabs(g[j]);
+    }
+    if (cv <= tol) return kSuccess;
+    if ((unsigned)step >= max_it || std::isnan(cv)) return kFailure;
+    return kIterate;
+  };
+  BfgsResult r = lbfgs_solve(N, psi, eval, check);
+  if (eval_rc) return eval_rc;
+  c->last_iterations = r.iterations;
+  c->converged = r.status == kConverged;
+  if (info) {
+    info->status = r.status;
+    info->iterations = r.iterations;
+    info->evaluations = r.evaluations;
+    info->functional = r.f;
+    info->check_value = r.check;
+    info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return SDOT_OK;
+}
+
+extern "C" int sdot_covering_radius(sdot_ctx *c, double *r0) {
+  if (ctx_check(c) || !r0) return fail(SDOT_ERR_INVALID, "null argument");
+  if (!c->have_source || !c->have_target) return fail(SDOT_ERR_STATE, "measures not set");
+  HIP_TRY(hipSetDevice(c->device));
+  return raw_covering_radius(c, r0);
+}
+
+extern "C" int sdot_distance_threshold(sdot_ctx *c, const double *psi, double *R) {
+  if (ctx_check(c) || !psi || !R) return fail(SDOT_ERR_INVALID, "null argument");
+  if (!c->have_source || !c->have_target) return fail(SDOT_ERR_STATE, "measures not set");
+  HIP_TRY(hipSetDevice(c->device));
+  return threshold(c, psi, R);
+}
+
+extern "C" int sdot_set_distance_threshold(sdot_ctx *c, double R) {
+  if (ctx_check(c)) return SDOT_ERR_INVALID;
+  c->current_threshold = R;
+  return SDOT_OK;
+}
+
+extern "C" int sdot_candidate_stats(sdot_ctx *c, double R, int64_t *count, uint64_t *hash,
+                                    int64_t *pairs) {
+  if (ctx_check(c)) return SDOT_ERR_INVALID;
+  HIP_TRY(hipSetDevice(c->device));
+  int rc = prepare(c, R);
+  if (rc) return rc;
+  const Plan &p = c->plan;
+  // target records carry the raw coordinates the candidate rule reads
+  std::vector<double> zero(p.N, 0.0);
+  HIP_TRY(hipMemcpy(c->dev.psi, zero.data(), sizeof(double) * p.N, hipMemcpyHostToDevice));
+  rc = run_eval(c, R, c->dev.psi, c->dev.grad, c->dev.out);
+  if (rc) return rc;
+  int64_t *d_cnt = nullptr;
+  uint64_t *d_hash = nullptr;
+  HIP_TRY(hipMalloc((void **)&d_cnt, sizeof(int64_t) * p.n_cells));
+  HIP_TRY(hipMalloc((void **)&d_hash, sizeof(uint64_t) * p.n_cells));
+  hipError_t e = launch_candidate_stats(p, c->dev, d_cnt, d_hash, c->stream);
+  std::vector<int64_t> hc(p.n_cells);
+  std::vector<uint64_t> hh(p.n_cells);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(hc.data(), d_cnt, sizeof(int64_t) * p.n_cells, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(hh.data(), d_hash, sizeof(uint64_t) * p.n_cells, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d_cnt);
+  (void)hipFree(d_hash);
+  if (e != hipSuccess) return fail(SDOT_ERR_HIP, std::string("candidate stats: ") + hipGetErrorString(e));
+  int64_t tot = 0;
+  for (int64_t i = 0; i < p.n_cells; ++i) tot += hc[i];
+  if (count) std::memcpy(count, hc.data(), sizeof(int64_t) * p.n_cells);
+  if (hash) std::memcpy(hash, hh.data(), sizeof(uint64_t) * p.n_cells);
+  if (pairs) *pairs = tot * p.nq;
+  c->last_pairs = tot * p.nq;
+  return SDOT_OK;
+}
+
+extern "C" int sdot_get_state(sdot_ctx *c, sdot_state *st) {
+  if (ctx_check(c) || !st) return fail(SDOT_ERR_INVALID, "null argument");
+  st->last_functional = c->global_functional;
+  st->C_global = c->C_global;
+  st->last_distance_threshold = c->current_threshold;
+  st->covering_radius = c->covering_radius;
+  st->min_target_density = c->min_target_density;
+  st->last_iteration_count = c->last_iterations;
+  st->converged = c->converged;
+  st->last_pair_count = c->last_pairs;
+  return SDOT_OK;
+}
+
+extern "C" int sdot_set_state(sdot_ctx *c, const sdot_state *st) {
+  if (ctx_check(c) || !st) return fail(SDOT_ERR_INVALID, "null argument");
+  c->global_functional = st->last_functional;
+  c->C_global = st->C_global;
+  c->current_threshold = st->last_distance_threshold;
+  c->covering_radius = st->covering_radius;
+  c->min_target_density = st->min_target_density;
+  return SDOT_OK;
+}
+
+extern "C" int sdot_profile_enable(sdot_ctx *c, int on) {
+  if (ctx_check(c)) return SDOT_ERR_INVALID;
+  HIP_TRY(hipSetDevice(c->device));
+  if (on && c->ev_pool.empty()) {
+    c->ev_pool.resize(kProfSets);
+    for (auto &set : c->ev_pool)
+      for (auto &e : set) HIP_TRY(hipEventCreate(&e));
+  }
+  int rc = resolve_profile(c);
+  if (rc) return rc;
+  c->prof_on = on != 0;
+  c->prof_acc = sdot_profile{};
+  return SDOT_OK;
+}
+
+extern "C" int sdot_profile_get(sdot_ctx *c, sdot_profile *out) {
+  if (ctx_check(c) || !out) return fail(SDOT_ERR_INVALID, "null argument");
+  int rc = resolve_profile(c);
+  if (rc) return rc;
+  *out = c->prof_acc;
+  return SDOT_OK;
+}

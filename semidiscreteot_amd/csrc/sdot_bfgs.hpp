@@ -1,0 +1,212 @@
+// sdot_bfgs.hpp -- FP64 limited-memory BFGS with the line search that
+// SotSolver::solve uses through deal.II (SotSolver.cc:190-198):
+//   * SolverBFGS: history of 5 (s, y) pairs, newest first, no preconditioner,
+//     pair stored only when s.y > 0; the in-tree text of the algorithm is
+//     core/wasserstein_solver_bfgs.h:402-491.
+//   * default line minimisation: first trial step 1 on the first iteration,
+//     else min(1, -2.02 * max(f_prev - f, 100 eps_mach) / g0)
+//     (wasserstein_solver_bfgs.h:322-365); Wolfe parameters eta = 0.9,
+//     mu = 0.001, at most 20 evaluations.
+//   * LineMinimization::line_search (line_minimization.h:498-749) with
+//     poly_fit (cubic, then quadratic, then bisection of the safeguarded
+//     interval, line_minimization.h:423-455): bracketing growth 9, sectioning
+//     safeguards 0.1 / 0.5.  Running out of evaluations is reported as
+//     kLineSearchFail (stock deal.II throws there).
+#pragma once
+
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace sdot {
+
+enum BfgsStatus { kConverged = 0, kMaxIter = 1, kLineSearchFail = 2, kNonFinite = 3 };
+enum CheckState { kIterate = 0, kSuccess = 1, kFailure = 2 };
+
+struct BfgsResult {
+  int status = kMaxIter;
+  int iterations = 0;
+  int evaluations = 0;
+  double f = 0.0;
+  double check = 0.0;
+};
+
+inline double bfgs_dot(const double *a, const double *b, int64_t n) {
+  double s = 0.0;
+  for (int64_t i = 0; i < n; ++i) s += a[i] * b[i];
+  return s;
+}
+
+// One-dimensional Wolfe search along p from x0.  phi(a, f, g) evaluates the
+// objective at x0 + a p and returns f and the slope g = grad . p.
+template <class Phi>
+bool wolfe_line_search(Phi &&phi, double f0, double g0, double a1, double &alpha) {
+  constexpr double eta = 0.9, mu = 0.001, a_max = DBL_MAX;
+  constexpr unsigned max_evals = 20;
+  constexpr double grow = 9.0, shrink_lo = 0.1, shrink_hi = 0.5;
+  const double slope_tol = eta * std::fabs(g0);
+  const double f_floor = f0 + a_max * mu * g0;
+  auto armijo = [&](double a, double f) { return f <= f0 + a * mu * g0; };
+  auto curvature = [&](double g) { return std::fabs(g) <= slope_tol; };
+
+  double lo_a = NAN, lo_f = NAN, lo_g = NAN, hi_a = NAN, hi_f = NAN, hi_g = NAN;
+  unsigned used = 0;
+  double prev_a = 0.0, prev_f = f0, prev_g = g0;
+  double a = a1, f = 0.0, g = 0.0;
+  bool bracketed = false;
+  while (used < max_evals) {
+    phi(a, f, g);
+    ++used;
+    if (f <= f_floor || a == a_max) { alpha = a; return true; }
+    if (!armijo(a, f) || (f >= prev_f && used > 1)) {
+      lo_a = prev_a; lo_f = prev_f; lo_g = prev_g;
+      hi_a = a; hi_f = f; hi_g = g;
+      bracketed = true;
+      break;
+    }
+    if (curvature(g)) { alpha = a; return true; }
+    if (g >= 0) {
+      lo_a = a; lo_f = f; lo_g = g;
+      hi_a = prev_a; hi_f = prev_f; hi_g = prev_g;
+      bracketed = true;
+      break;
+    }
+    const double next = std::fmin(a_max, a + grow * (a - prev_a));
+    prev_a = a; prev_f = f; prev_g = g;
+    a = next;
+  }
+  if (!bracketed || used >= max_evals) return false;
+
+  while (used < max_evals) {
+    const double s1 = lo_a + shrink_lo * (hi_a - lo_a);
+    const double s2 = hi_a - shrink_hi * (hi_a - lo_a);
+    const double bl = std::fmin(s1, s2), bh = std::fmax(s1, s2);
+    // cubic through both ends with slopes
+    double trial = 0.0;
+    bool ok = false;
+    {
+      const double b1 = lo_g + hi_g - 3.0 * (lo_f - hi_f) / (lo_a - hi_a);
+      const double disc = b1 * b1 - lo_g * hi_g;
+      if (disc >= 0) {
+        const double b2 = std::sqrt(disc);
+        const double den = lo_a < hi_a ? hi_g - lo_g + 2.0 * b2 : lo_g - hi_g + 2.0 * b2;
+        if (den != 0.0) {
+          trial = lo_a < hi_a ? hi_a - (hi_a - lo_a) * (hi_g + b2 - b1) / den
+                              : lo_a - (lo_a - hi_a) * (lo_g + b2 - b1) / den;
+          ok = trial >= bl && trial <= bh;
+        }
+      }
+    }
+    if (!ok) {  // quadratic through lo (value, slope) and hi (value)
+      const double den = 2.0 * lo_g * hi_a - 2.0 * lo_g * lo_a - 2.0 * hi_f + 2.0 * lo_f;
+      if (den != 0) {
+        trial = (lo_g * (hi_a * hi_a - lo_a * lo_a) + 2.0 * (lo_f - hi_f) * lo_a) / den;
+        ok = trial >= bl && trial <= bh;
+      }
+    }
+    if (!ok) trial = 0.5 * (bl + bh);
+    a = trial;
+    phi(a, f, g);
+    ++used;
+    if (!armijo(a, f) || f >= lo_f) {
+      hi_a = a; hi_f = f; hi_g = g;
+    } else {
+      if (curvature(g)) { alpha = a; return true; }
+      if (g * (hi_a - lo_a) >= 0) {
+        hi_a = lo_a; hi_f = lo_f; hi_g = lo_g;
+      }
+      lo_a = a; lo_f = f; lo_g = g;
+    }
+  }
+  return false;
+}
+
+// eval(x, g) -> f ; check(step, g, &value) -> CheckState
+template <class Eval, class Check>
+BfgsResult lbfgs_solve(int64_t n, double *x, Eval &&eval, Check &&check, int history = 5) {
+  BfgsResult res;
+  std::vector<double> g(n), p(n), sk(n), yk(n), x0(n);
+  std::vector<std::vector<double>> S(history, std::vector<double>(n)),
+      Y(history, std::vector<double>(n));
+  std::vector<double> rho(history), coef(history);
+  int count = 0, newest = -1;  // ring buffer; slot of pair i (0 = newest) is (newest - i) mod h
+  auto slot = [&](int i) { return ((newest - i) % history + history) % history; };
+
+  double f = eval(x, g.data());
+  res.evaluations = 1;
+  int k = 0;
+  double cv = 0.0;
+  int st = check(k, g.data(), cv);
+  bool first = true;
+  double f_prev = 0.0;
+  while (st == kIterate) {
+    // two-loop recursion, H0 = identity
+    std::memcpy(p.data(), g.data(), sizeof(double) * n);
+    for (int i = 0; i < count; ++i) {
+      const int s = slot(i);
+      coef[i] = rho[s] * bfgs_dot(S[s].data(), p.data(), n);
+      const double c = -coef[i];
+      const double *ys = Y[s].data();
+      for (int64_t t = 0; t < n; ++t) p[t] += c * ys[t];
+    }
+    for (int i = count - 1; i >= 0; --i) {
+      const int s = slot(i);
+      const double c2 = rho[s] * bfgs_dot(Y[s].data(), p.data(), n);
+      const double c = coef[i] - c2;
+      const double *ss = S[s].data();
+      for (int64_t t = 0; t < n; ++t) p[t] += c * ss[t];
+    }
+    for (int64_t t = 0; t < n; ++t) p[t] = -p[t];
+    std::memcpy(sk.data(), x, sizeof(double) * n);
+    std::memcpy(yk.data(), g.data(), sizeof(double) * n);
+
+    const double f0 = f, g0 = bfgs_dot(g.data(), p.data(), n);
+    std::memcpy(x0.data(), x, sizeof(double) * n);
+    const double df = std::fmax(f_prev - f, 100.0 * DBL_EPSILON);
+    const double a1 = first ? 1.0 : std::fmin(1.0, -1.01 * 2.0 * df / g0);
+    f_prev = f;
+    auto phi = [&](double a, double &fo, double &go) {
+      for (int64_t t = 0; t < n; ++t) x[t] = x0[t] + a * p[t];
+      f = eval(x, g.data());
+      ++res.evaluations;
+      fo = f;
+      go = bfgs_dot(g.data(), p.data(), n);
+    };
+    double alpha = 0.0;
+    if (!wolfe_line_search(phi, f0, g0, a1, alpha)) {
+      ++k;
+      res.status = kLineSearchFail;
+      res.iterations = k;
+      res.f = f;
+      res.check = cv;
+      return res;
+    }
+    first = false;
+    for (int64_t t = 0; t < n; ++t) {
+      sk[t] = x[t] - sk[t];
+      yk[t] = g[t] - yk[t];
+    }
+    ++k;
+    st = check(k, g.data(), cv);
+    if (st != kIterate) break;
+    const double curv = bfgs_dot(sk.data(), yk.data(), n);
+    if (curv > 0.0 && history > 0) {
+      newest = (newest + 1) % history;
+      S[newest].swap(sk);
+      Y[newest].swap(yk);
+      rho[newest] = 1.0 / curv;
+      if (count < history) ++count;
+      sk.resize(n);
+      yk.resize(n);
+    }
+  }
+  res.status = st == kSuccess ? kConverged : (std::isnan(cv) ? kNonFinite : kMaxIter);
+  res.iterations = k;
+  res.f = f;
+  res.check = cv;
+  return res;
+}
+
+}  // namespace sdot

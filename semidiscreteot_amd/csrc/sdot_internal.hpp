@@ -1,0 +1,118 @@
+// sdot_internal.hpp -- context layout and kernel-launcher interface shared by
+// the host-side C++ (sdot_api.cpp) and the HIP kernels (sdot_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cmath>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/sdot.h"
+
+namespace sdot {
+
+// Doubles per target record and per quadrature record (64 bytes: one scalar
+// s_load_dwordx16 per record).
+constexpr int kRec = 8;
+constexpr int kWave = 64;
+
+enum Mode : int { kF64Lse = 0, kF64Plain = 1, kF32Lse = 2 };
+
+// Uniform cubic bin grid. bin_of() is the single binning rule used by the
+// host (to sort points into bins) and by the device (to turn query bounds into
+// bin ranges). It is monotone non-decreasing in v, which is what makes the
+// device's conservative range query never miss a binned point.
+struct Grid {
+  double o[3];
+  double h;
+  double inv_h;
+  int n[3];
+};
+
+#if defined(__HIPCC__)
+#define SDOT_HD __host__ __device__
+#else
+#define SDOT_HD
+#endif
+
+SDOT_HD inline int bin_of(double v, double o, double inv_h, int n) {
+  double f = floor((v - o) * inv_h);
+  if (!(f >= 0.0)) return 0;
+  if (f >= (double)n) return n - 1;
+  return (int)f;
+}
+
+// Device buffers and host metadata of one solver context.
+struct Dev {
+  // ---- source (static after sdot_set_source) ----
+  double *qx_m = nullptr;      // [nQ*dim] shifted quadrature points, Morton cell order
+  double *qw_m = nullptr;      // [nQ] rho*JxW
+  double *qx2_m = nullptr;     // [nQ] |x'|^2 (plain-mode shift)
+  double *cc_m = nullptr;      // [n_cells*dim] raw centers, Morton order
+  double *blkbox = nullptr;    // [n_blocks*2*dim] AABB of the block's cell centers
+  double *cc_r = nullptr;      // [n_cells*4] raw centers, row (bin) order
+  int32_t *cstart = nullptr;   // [n_cbins+1] row-order cell offsets per bin
+  int64_t *cell_r2m = nullptr; // [n_cells] row-order cell -> Morton cell
+  int64_t *cell_m2o = nullptr; // [n_cells] Morton cell -> caller's cell index
+  // ---- target (static after sdot_set_target) ----
+  double *ty_o = nullptr;      // [N*dim] raw target points, caller order
+  double *tnu_o = nullptr;     // [N]
+  int32_t *tpos_r = nullptr;   // [N] caller index -> row-order position
+  int32_t *tpos_m = nullptr;   // [N] caller index -> Morton position
+  int32_t *tr2o = nullptr;     // [N] row-order position -> caller index
+  int32_t *tm2o = nullptr;     // [N] Morton position -> caller index
+  int32_t *tstart = nullptr;   // [n_tbins+1]
+  double *twbox = nullptr;     // [n_twaves*2*dim]
+  double *ty_m = nullptr;      // [N*4] raw points, Morton order (covering radius)
+  // ---- per-evaluation ----
+  double *trec_r = nullptr;    // [N*kRec] row order
+  double *trec_m = nullptr;    // [N*kRec] Morton order
+  double *qrec_r = nullptr;    // [nQ*kRec] row cell order
+  double *Mpart = nullptr;     // [K1*nQ]
+  double *Spart = nullptr;     // [K1*nQ]
+  double *Gpart = nullptr;     // [K2*N]
+  double *Fpart = nullptr;     // [nFpart]
+  double *Cpart = nullptr;     // [nFpart]
+  double *PNpart = nullptr;    // [nPpart]
+  double *psi = nullptr;       // [N] staging
+  double *grad = nullptr;      // [N] staging
+  double *out = nullptr;       // [4] F, C, ...
+  double *scratch = nullptr;   // misc
+  size_t scratch_bytes = 0;
+  int64_t *i64scratch = nullptr;
+  size_t i64scratch_n = 0;
+};
+
+// Everything a kernel launch needs, by value.
+struct Plan {
+  int dim = 0, nq = 0, G = 1, gpc = 1, mode = kF64Lse;
+  int64_t n_cells = 0, nQ = 0, n_groups = 0, n_blocks = 0;
+  int64_t N = 0, n_twaves = 0;
+  int K1 = 1, K2 = 1;
+  int64_t nFpart = 0, nPpart = 0;
+  Grid tg{}, cg{};
+  double origin[3] = {0, 0, 0};
+  double eps = 1.0;
+  double smax = 0.0;   // candidate rule: s <= smax  <=>  sqrt(s) <= R
+  double Rq = 0.0;     // conservative enumeration radius
+  int dense = 0;       // every pair is a candidate (no enumeration, no test)
+};
+
+struct Profile {
+  bool on = false;
+  hipEvent_t ev[6] = {};
+  sdot_profile acc{};
+};
+
+// Kernel launchers (sdot_kernels.hip).
+hipError_t launch_eval(const Plan &p, Dev &d, const double *d_psi, double *d_grad,
+                       double *d_FC, hipStream_t s, Profile *prof);
+hipError_t launch_candidate_stats(const Plan &p, Dev &d, int64_t *d_count,
+                                  uint64_t *d_hash, hipStream_t s);
+hipError_t launch_covering(const Plan &p, Dev &d, double *d_cellmin, hipStream_t s);
+hipError_t launch_hessian(const Plan &p, Dev &d, const double *d_psi, double *d_H,
+                          int64_t ld, hipStream_t s);
+
+}  // namespace sdot

@@ -1,0 +1,157 @@
+"""Write tests/golden/tiny_cases.json: 50-digit values of the regularized
+semi-discrete dual (functional F, gradient g, C_global, per-cell candidate
+sets) on tiny seeded inputs.
+
+The definition evaluated here is the one SotSolver::evaluate_functional
+assembles (SotSolver.cc:217-377, local_assemble SotSolver.cc:769-869):
+    e_qj = (psi_j - |x_q - y_j|^2 / 2) / eps,  j in C(cell(q))
+    C(K) = { j : |y_j - center(K)| <= R }
+    S_q  = sum_j nu_j exp(e_qj)                 (any shift cancels)
+    F    = sum_q rho_q JxW_q eps log S_q - sum_j psi_j nu_j
+    g_j  = sum_q rho_q JxW_q nu_j exp(e_qj) / S_q - nu_j
+    C    = sum_q rho_q JxW_q / S_q^shifted,   S^shifted = S_q exp(-max_j e_qj)
+with q-points whose double-precision S_q underflows to 0 skipped (plain mode,
+SotSolver.cc:854).  Inputs are stored as the exact doubles used, so the
+values are the mathematical ones for those inputs; quadrature points come
+from textbook Gauss-Legendre nodes, not from the oracle.
+
+Run: python tests/golden/make_golden.py   (needs mpmath and numpy only)
+"""
+import json
+import os
+
+import mpmath as mp
+import numpy as np
+
+mp.mp.dps = 50
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+# double underflow: exp(e) rounds to 0 below log(2^-1075)
+UNDERFLOW = mp.log(mp.mpf(2) ** -1075)
+
+
+def gauss2():
+    a = 0.5 - 0.5 / np.sqrt(3.0)
+    return [a, 1.0 - a], [0.5, 0.5]
+
+
+def box_source(dim, n, seed):
+    """n^dim cells on [0,1]^dim, 2-point Gauss per direction, bilinear /
+    trilinear density from seeded nodal values."""
+    rng = np.random.default_rng(seed)
+    gp, gw = gauss2()
+    h = 1.0 / n
+    nodal = 0.5 + rng.random(tuple([n + 1] * dim))
+    xs, rho, jxw, cen = [], [], [], []
+    for cell in np.ndindex(*([n] * dim)):
+        c = [(cell[d] + 0.5) * h for d in range(dim)]
+        cen.append(c)
+        for q in np.ndindex(*([2] * dim)):
+            xi = [gp[q[d]] for d in range(dim)]
+            x = [(cell[d] + xi[d]) * h for d in range(dim)]
+            r = 0.0
+            for v in np.ndindex(*([2] * dim)):
+                phi = 1.0
+                for d in range(dim):
+                    phi *= xi[d] if v[d] else 1.0 - xi[d]
+                r += nodal[tuple(cell[d] + v[d] for d in range(dim))] * phi
+            w = 1.0
+            for d in range(dim):
+                w *= gw[q[d]]
+            xs.append(x)
+            rho.append(r)
+            jxw.append(w * h ** dim)
+    return (np.array(xs), np.array(rho), np.array(jxw), np.array(cen), 2 ** dim)
+
+
+def evaluate(dim, nq, x, rho, jxw, cen, y, nu, psi, eps, R, lse):
+    n_cells = len(cen)
+    N = len(y)
+    eps = mp.mpf(eps)
+    F = mp.mpf(0)
+    Cg = mp.mpf(0)
+    g = [mp.mpf(0)] * N
+    counts, members = [], []
+    for c in range(n_cells):
+        cand = []
+        for j in range(N):
+            d2 = sum((mp.mpf(y[j][k]) - mp.mpf(cen[c][k])) ** 2 for k in range(dim))
+            if mp.sqrt(d2) <= mp.mpf(R):
+                cand.append(j)
+        counts.append(len(cand))
+        members.append(cand)
+        for q in range(nq):
+            i = c * nq + q
+            es = []
+            for j in cand:
+                d2 = sum((mp.mpf(x[i][k]) - mp.mpf(y[j][k])) ** 2 for k in range(dim))
+                es.append((mp.mpf(psi[j]) - d2 / 2) / eps)
+            if not es:
+                continue
+            terms = []
+            for e in es:
+                if not lse and e < UNDERFLOW + 5:
+                    terms.append(mp.mpf(0))
+                else:
+                    terms.append(mp.exp(e))
+            S = sum(mp.mpf(nu[j]) * t for j, t in zip(cand, terms))
+            if S == 0:
+                continue
+            w = mp.mpf(rho[i]) * mp.mpf(jxw[i])
+            F += w * eps * mp.log(S)
+            M = max(es) if lse else mp.mpf(0)
+            Cg += w / (S * mp.exp(-M))
+            for j, t in zip(cand, terms):
+                g[j] += w * mp.mpf(nu[j]) * t / S
+    for j in range(N):
+        F -= mp.mpf(psi[j]) * mp.mpf(nu[j])
+        g[j] -= mp.mpf(nu[j])
+    return F, g, Cg, counts, members
+
+
+def case(name, dim, n, N, seed, eps, R, lse, cluster=None):
+    x, rho, jxw, cen, nq = box_source(dim, n, seed)
+    rng = np.random.default_rng(seed + 100)
+    if cluster is None:
+        y = rng.random((N, dim))
+    else:
+        y = cluster + 0.05 * rng.random((N, dim))
+    nu = rng.random(N) + 0.2
+    nu = nu / nu.sum()
+    psi = 0.05 * rng.standard_normal(N)
+    F, g, Cg, counts, members = evaluate(dim, nq, x, rho, jxw, cen, y, nu, psi, eps, R, lse)
+    hashes = []
+    for m in members:
+        h = 0
+        for j in m:
+            v = ((j + 1) * 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+            v ^= v >> 29
+            h = (h + v) & 0xFFFFFFFFFFFFFFFF
+        hashes.append(str(h))
+    return {
+        "name": name, "dim": dim, "nq": nq, "eps": eps, "R": R, "use_lse": lse,
+        "x": x.ravel().tolist(), "rho": rho.tolist(), "jxw": jxw.tolist(),
+        "center": cen.ravel().tolist(), "y": y.ravel().tolist(), "nu": nu.tolist(),
+        "psi": psi.tolist(),
+        "F": mp.nstr(F, 40), "g": [mp.nstr(v, 40) for v in g], "C": mp.nstr(Cg, 40),
+        "count": counts, "hash": hashes,
+    }
+
+
+def main():
+    cases = [
+        case("2d_untruncated_lse", 2, 2, 5, 1, 0.05, 1.7976931348623157e308, True),
+        case("2d_untruncated_plain", 2, 2, 5, 2, 0.05, 1.7976931348623157e308, False),
+        case("3d_truncated_lse", 3, 2, 7, 3, 0.1, 0.6, True),
+        case("2d_truncated_lse", 2, 3, 9, 4, 0.02, 0.45, True),
+        case("2d_plain_underflow", 2, 2, 3, 5, 2e-4, 1.7976931348623157e308, False,
+             cluster=np.array([0.02, 0.02])),
+    ]
+    out = os.path.join(HERE, "tiny_cases.json")
+    with open(out, "w") as f:
+        json.dump(cases, f)
+    print("wrote", out)
+
+
+if __name__ == "__main__":
+    main()

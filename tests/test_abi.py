@@ -1,0 +1,132 @@
+"""CPU checks of the product library: it loads, exports every symbol that
+include/sdot.h declares, and its host-only parts (parameter defaults, .prm
+reader, FEValues stand-in) agree with the reference's definitions and the
+oracle.  No GPU compute is called here."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import semidiscreteot_amd as sd
+from semidiscreteot_amd import _lib as L
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "sdot.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(sdot_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = L.load()
+    syms = declared_symbols()
+    assert len(syms) >= 25
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(syms) == set(L.SIGNATURES), set(syms) ^ set(L.SIGNATURES)
+
+
+def test_version_and_error_channel():
+    lib = L.load()
+    assert b"gfx950" in lib.sdot_version()
+    rc = lib.sdot_set_params(None, None)
+    assert rc == -1 and lib.sdot_last_error()
+
+
+def test_parameter_defaults_match_reference():
+    p = sd.SolverParameters.from_prm("")
+    # ParameterManager.h:56-72
+    assert p.max_iterations == 1000 and p.tolerance == 1.0 and p.epsilon == 1e-3
+    assert p.tau == 1e-8 and p.solver_control_type == "l1norm"
+    assert p.distance_threshold_type == "pointwise" and p.quadrature_order == 3
+    assert p.use_log_sum_exp_trick is False and p.n_threads == 0
+    assert p.epsilon_scaling_factor == 2.0 and p.epsilon_scaling_steps == 5
+
+
+PRM = """
+subsection SotParameterManager
+  set selected_task =  sot    # Options: mesh_generation, sot
+  subsection rsot_solver
+    set max_iterations = 10000
+    set epsilon = 1e-2
+    set tau = 1e-8  # Truncation error tolerance for integral radius bound
+    set tolerance = 1e-2
+    set distance_threshold_type = integral # Options: pointwise/integral
+    set solver_control_type = componentwise
+    set verbose_output = false
+    set solver_type = BFGS
+    set quadrature_order = 2
+    set number_of_threads = 0
+    set use_log_sum_exp_trick = true
+    set use_epsilon_scaling = false
+    set epsilon_scaling_factor = 10
+    set epsilon_scaling_steps = 3
+    set precision = fp32_lse
+  end
+  subsection power_diagram_parameters
+    set implementation = geogram
+  end
+end
+"""
+
+
+def test_prm_reader():
+    p = sd.SolverParameters.from_prm(PRM)
+    assert p.max_iterations == 10000 and p.epsilon == 1e-2 and p.tolerance == 1e-2
+    assert p.distance_threshold_type == "integral" and p.solver_control_type == "componentwise"
+    assert p.verbose_output is False and p.quadrature_order == 2
+    assert p.use_log_sum_exp_trick is True and p.epsilon_scaling_factor == 10.0
+    assert p.epsilon_scaling_steps == 3 and p.precision == "fp32_lse"
+
+
+def test_prm_reader_errors():
+    with pytest.raises(sd.SdotError):
+        sd.SolverParameters.from_prm("subsection rsot_solver\n set bogus_key = 1\nend\n")
+    with pytest.raises(sd.SdotError):
+        sd.SolverParameters.from_prm("subsection rsot_solver\n set epsilon = abc\nend\n")
+    with pytest.raises(sd.SdotError):
+        sd.SolverParameters.from_prm("subsection rsot_solver\n set epsilon = 1\n")
+    # unknown threshold names fall back to pointwise as in SotSolver.cc:406
+    p = sd.SolverParameters.from_prm("subsection rsot_solver\n set distance_threshold_type = foo\nend\n")
+    assert p.distance_threshold_type == "pointwise"
+
+
+def test_reference_prm_file_parses():
+    path = os.path.join(ROOT, "tests", "golden", "parameters_rsot.prm")
+    p = sd.SolverParameters.from_prm(path=path)
+    assert p.epsilon == 1e-2 and p.max_iterations == 10000
+
+
+@pytest.mark.parametrize("dim,n,q", [(2, [7, 5], 3), (3, [4, 3, 5], 2), (3, [3, 3, 3], 3)])
+def test_structured_mesh_matches_oracle(dim, n, q):
+    rng = np.random.default_rng(dim * 10 + q)
+    nv = int(np.prod([k + 1 for k in n]))
+    rho = 0.5 + rng.random(nv)
+    lo, hi = [-0.5] * dim, [1.5] * dim
+    a = sd.structured_source(dim, n, lo, hi, q, rho)
+    o = O.structured_source(dim, n, lo, hi, q, rho)
+    assert np.array_equal(a.centers, o.center)
+    assert np.max(np.abs(a.x - o.x)) <= 4e-16 * 2
+    assert np.max(np.abs(a.w - o.w) / o.w) < 1e-14
+    assert abs(a.w.sum() - 1.0) < 1e-13
+
+
+def test_simplex_mesh_matches_oracle():
+    from tests.cases import tet_case
+    _, _, (p, cells, rho) = tet_case(200, 5, seed=1)
+    a = sd.simplex_source(3, p.ravel(), cells.ravel(), 2, rho)
+    o = O.simplex_source(3, p.ravel(), cells.ravel(), 2, rho)
+    assert np.array_equal(a.centers, o.center)
+    assert np.max(np.abs(a.x - o.x)) < 1e-15
+    assert np.max(np.abs(a.w - o.w) / np.abs(o.w)) < 1e-13
+
+
+def test_mesh_argument_errors():
+    with pytest.raises(sd.SdotError):
+        sd.structured_source(4, [2, 2, 2], [0] * 3, [1] * 3, 2)
+    with pytest.raises(sd.SdotError):
+        sd.simplex_source(3, np.zeros(12), np.array([0, 1, 2, 3]), 3)

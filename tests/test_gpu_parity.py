@@ -1,0 +1,231 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 oracle and
+the 50-digit golden vectors.
+
+Tolerances (FP64 mode): |F - F_ref| <= 1e-10 |F_ref| and
+max_j |g_j - g_ref_j| <= 1e-10 * max_j (g_ref_j + nu_j), i.e. relative to the
+assembled marginal; candidate sets per cell identical (count and hash).
+FP32_LSE mode: relative marginal error max_j |g_j - g_ref_j| / nu_j <= 1e-6
+and |F - F_ref| <= 1e-8 |F_ref|."""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import semidiscreteot_amd as sd
+from oracle import oracle as O
+from tests.cases import structured_case, tet_case
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = json.load(open(os.path.join(HERE, "golden", "tiny_cases.json")))
+DMAX = np.finfo(float).max
+
+
+def say(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def make(src, tgt, **kw):
+    s = sd.SotSolver(src.dim)
+    s.setup_source(src.x, src.w, src.center, src.nq)
+    s.setup_target(tgt.y, tgt.nu)
+    s.configure(sd.SolverParameters(**kw))
+    return s
+
+
+def assert_fp64(F, g, Fo, go, nu, tol=1e-10):
+    assert abs(F - Fo) <= tol * abs(Fo), (F, Fo)
+    scale = np.max(np.abs(go + nu))
+    err = np.max(np.abs(g - go))
+    assert err <= tol * scale, (err, scale)
+
+
+@pytest.mark.parametrize("case", GOLD, ids=[c["name"] for c in GOLD])
+def test_golden_fp64(case):
+    say(f"golden {case['name']}")
+    src = O.Source(case["dim"], case["nq"], case["x"], case["rho"], case["jxw"], case["center"])
+    tgt = O.Target(case["y"], case["nu"])
+    s = make(src, tgt, epsilon=case["eps"], use_log_sum_exp_trick=case["use_lse"])
+    F, g, C = s.evaluate_functional_R(case["R"], case["psi"])
+    gg = np.array([float(v) for v in case["g"]])
+    assert_fp64(F, g, float(case["F"]), gg, np.array(case["nu"]), tol=1e-12)
+    assert abs(C - float(case["C"])) <= 1e-12 * float(case["C"])
+    cnt, h, _ = s.candidate_stats(case["R"])
+    assert cnt.tolist() == case["count"]
+    assert [str(v) for v in h.tolist()] == case["hash"]
+
+
+@pytest.mark.parametrize("lse", [True, False])
+def test_untruncated_2d(lse):
+    say(f"untruncated 2d lse={lse}")
+    src, tgt = structured_case(2, 32, 3, 400, seed=0)
+    psi = 0.01 * np.random.default_rng(0).standard_normal(tgt.n)
+    s = make(src, tgt, epsilon=1e-2, use_log_sum_exp_trick=lse)
+    F, g, C = s.evaluate_functional_R(DMAX, psi)
+    Fo, go, Co = O.evaluate_R(src, tgt, O.params(epsilon=1e-2, use_lse=lse), DMAX, psi)
+    assert_fp64(F, g, Fo, go, tgt.nu)
+    assert abs(C - Co) <= 1e-10 * Co
+
+
+def test_plain_mode_underflow_skips_q():
+    say("plain underflow")
+    src, tgt = structured_case(2, 12, 2, 5, seed=9)
+    y = np.full((5, 2), 0.03) + 0.01 * np.arange(5)[:, None]
+    tgt = O.Target(y.ravel(), np.full(5, 0.2))
+    s = make(src, tgt, epsilon=2e-4, use_log_sum_exp_trick=False)
+    psi = np.zeros(5)
+    F, g, C = s.evaluate_functional_R(DMAX, psi)
+    Fo, go, Co = O.evaluate_R(src, tgt, O.params(epsilon=2e-4, use_lse=0), DMAX, psi)
+    assert Co > 0
+    assert_fp64(F, g, Fo, go, tgt.nu)
+    assert abs(C - Co) <= 1e-10 * Co
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_lse"])
+def test_truncated_3d_sets_and_values(precision):
+    say(f"truncated 3d {precision}")
+    src, tgt = structured_case(3, 10, 2, 2000, seed=1, gmm_source=True, gmm_targets=True,
+                               dirichlet=5)
+    rng = np.random.default_rng(2)
+    psi = 0.01 * rng.standard_normal(tgt.n)
+    eps, tau = 1e-2, 1e-10
+    st = O.new_state()
+    R = O.distance_threshold(src, tgt, O.params(epsilon=eps, tau=tau), st, psi)
+    assert R < 1.0  # truncated, not dense
+    s = make(src, tgt, epsilon=eps, tau=tau, use_log_sum_exp_trick=True, precision=precision)
+    assert s.distance_threshold(psi) == R
+    cnt, h, pairs = s.candidate_stats(R)
+    cnt_o, h_o = O.candidate_stats(src, tgt, R)
+    assert np.array_equal(cnt, cnt_o) and np.array_equal(h, h_o)
+    assert 0 < pairs < src.n_cells * src.nq * tgt.n
+    F, g, C = s.evaluate_functional_R(R, psi)
+    Fo, go, Co = O.evaluate_R(src, tgt, O.params(epsilon=eps, use_lse=1), R, psi)
+    if precision == "fp64":
+        assert_fp64(F, g, Fo, go, tgt.nu)
+    else:
+        assert abs(F - Fo) <= 1e-8 * abs(Fo)
+        assert np.max(np.abs(g - go) / tgt.nu) <= 1e-6
+
+
+def test_fp32_small_eps_error():
+    say("fp32 eps=1e-3")
+    src, tgt = structured_case(3, 12, 2, 3000, seed=5, gmm_source=True, gmm_targets=True,
+                               dirichlet=5)
+    psi = 0.02 * np.random.default_rng(3).standard_normal(tgt.n)
+    s = make(src, tgt, epsilon=1e-3, use_log_sum_exp_trick=True, precision="fp32_lse")
+    F, g, C = s.evaluate_functional_R(DMAX, psi)
+    Fo, go, Co = O.evaluate_R(src, tgt, O.params(epsilon=1e-3, use_lse=1), DMAX, psi)
+    rel = np.max(np.abs(g - go) / tgt.nu)
+    say(f"fp32 rel marginal err {rel:.3e}  F rel {abs(F - Fo) / abs(Fo):.3e}")
+    assert rel <= 1e-6
+    assert abs(F - Fo) <= 1e-8 * abs(Fo)
+
+
+def test_stateful_threshold_sequence():
+    """evaluate_functional reads C_global / the previous functional for the
+    integral and geometric bounds (SotSolver.cc:392-404)."""
+    say("stateful thresholds")
+    src, tgt = structured_case(2, 16, 2, 300, seed=7, dirichlet=5)
+    rng = np.random.default_rng(7)
+    for ttype in ("pointwise", "integral", "geometric"):
+        s = make(src, tgt, epsilon=1e-2, tau=1e-9, use_log_sum_exp_trick=True,
+                 distance_threshold_type=ttype)
+        prm = O.params(epsilon=1e-2, tau=1e-9, use_lse=1, threshold=ttype)
+        st = O.new_state()
+        for k in range(3):
+            psi = 0.02 * rng.standard_normal(tgt.n)
+            F, g = s.evaluate_functional(psi)
+            Fo, go = O.evaluate(src, tgt, prm, st, psi)
+            assert s.get_last_distance_threshold() == st.last_threshold, ttype
+            assert_fp64(F, g, Fo, go, tgt.nu)
+            assert abs(s.get_C_global() - st.C_global) <= 1e-10 * st.C_global
+
+
+def test_covering_radius_exact():
+    say("covering radius")
+    src, tgt = structured_case(3, 8, 2, 500, seed=4, gmm_targets=True)
+    s = make(src, tgt, epsilon=1e-2)
+    assert s.compute_covering_radius() == O.covering_radius(src, tgt)
+
+
+def test_determinism_bitwise():
+    say("determinism")
+    src, tgt = structured_case(3, 10, 2, 1500, seed=6, gmm_targets=True)
+    s = make(src, tgt, epsilon=5e-3, tau=1e-10, use_log_sum_exp_trick=True, precision="fp32_lse")
+    psi = 0.01 * np.random.default_rng(6).standard_normal(tgt.n)
+    R = s.distance_threshold(psi)
+    a = s.evaluate_functional_R(R, psi)
+    b = s.evaluate_functional_R(R, psi)
+    assert a[0] == b[0] and np.array_equal(a[1], b[1]) and a[2] == b[2]
+
+
+def test_tet_source_parity():
+    say("tets")
+    src, tgt, _ = tet_case(2000, 400, seed=3)
+    psi = 0.01 * np.random.default_rng(8).standard_normal(tgt.n)
+    s = make(src, tgt, epsilon=5e-3, tau=1e-10, use_log_sum_exp_trick=True)
+    R = s.distance_threshold(psi)
+    F, g, C = s.evaluate_functional_R(R, psi)
+    Fo, go, Co = O.evaluate_R(src, tgt, O.params(epsilon=5e-3, use_lse=1), R, psi)
+    assert_fp64(F, g, Fo, go, tgt.nu)
+    cnt, h, _ = s.candidate_stats(R)
+    cnt_o, h_o = O.candidate_stats(src, tgt, R)
+    assert np.array_equal(cnt, cnt_o) and np.array_equal(h, h_o)
+
+
+def test_edge_cases():
+    say("edge cases")
+    src, _ = structured_case(2, 8, 3, 1, seed=0)
+    # single target: closed form
+    tgt = O.Target(np.array([0.3, 0.6]), np.array([1.0]))
+    s = make(src, tgt, epsilon=1e-2, use_log_sum_exp_trick=True)
+    F, g, C = s.evaluate_functional_R(DMAX, np.zeros(1))
+    assert abs(F + 0.5 * (0.2 ** 2 + 0.1 ** 2 + 1 / 6)) < 1e-13 and abs(g[0]) < 1e-13
+    # empty candidate sets everywhere: F = -psi.nu, g = -nu
+    rng = np.random.default_rng(1)
+    tgt = O.Target(rng.random(40) + 5.0, np.full(20, 0.05))
+    s = make(src, tgt, epsilon=1e-2, use_log_sum_exp_trick=True)
+    psi = rng.standard_normal(20)
+    F, g, C = s.evaluate_functional_R(0.5, psi)
+    assert abs(F + psi @ tgt.nu) < 1e-15 and np.allclose(g, -tgt.nu) and C == 0.0
+    Fo, go, _ = O.evaluate_R(src, tgt, O.params(epsilon=1e-2, use_lse=1), 0.5, psi)
+    assert abs(F - Fo) < 1e-15
+    # NaN threshold: no candidates (sqrt(s) <= NaN is false)
+    F, g, C = s.evaluate_functional_R(float("nan"), psi)
+    assert np.allclose(g, -tgt.nu)
+
+
+def test_errors_raise():
+    say("errors")
+    s = sd.SotSolver(2)
+    with pytest.raises(sd.SdotError):
+        s.evaluate_functional(np.zeros(3))
+    src, tgt = structured_case(2, 4, 2, 5, seed=0)
+    s.setup_source(src.x, src.w, src.center, src.nq)
+    with pytest.raises(ValueError):
+        s.setup_target(np.zeros(7), np.ones(5))
+    with pytest.raises(sd.SdotError):
+        s.configure(sd.SolverParameters(epsilon=0.0))
+
+
+def test_solve_matches_oracle():
+    say("solve")
+    src, tgt = structured_case(2, 16, 3, 60, seed=12, gmm_source=True, dirichlet=5)
+    kw = dict(epsilon=1e-2, tau=1e-12, tolerance=1e-7, solver_control_type="componentwise",
+              max_iterations=300, use_log_sum_exp_trick=True)
+    s = make(src, tgt, **kw)
+    psi = s.solve()
+    info = s.get_last_solve_info()
+    prm = O.params(epsilon=1e-2, tau=1e-12, tolerance=1e-7, control="componentwise",
+                   max_iterations=300, use_lse=1)
+    psi_o, info_o = O.solve(src, tgt, prm)
+    say(f"gpu {info}  oracle {info_o['status']} it={info_o['iterations']}")
+    assert info["status"] == info_o["status"] == "converged"
+    assert s.get_convergence_status()
+    assert abs(info["iterations"] - info_o["iterations"]) <= 2
+    assert np.max(np.abs((psi - psi.mean()) - (psi_o - psi_o.mean()))) < 1e-6
+    Fo, go, _ = O.evaluate_R(src, tgt, prm, DMAX, psi)
+    assert np.max(np.abs(go) / tgt.nu) <= 1e-7 * 1.001

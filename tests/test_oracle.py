@@ -1,0 +1,192 @@
+"""Pins of the FP64 CPU oracle: golden 50-digit vectors, closed forms,
+invariants and textbook problems, each chosen so that a plausible slip in the
+restatement (dropped linear term, wrong sign or index, transposed operands,
+missing max shift) fails at least one of them."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.cases import structured_case, tet_case
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = json.load(open(os.path.join(HERE, "golden", "tiny_cases.json")))
+
+
+def golden_source(c):
+    return (O.Source(c["dim"], c["nq"], c["x"], c["rho"], c["jxw"], c["center"]),
+            O.Target(c["y"], c["nu"]))
+
+
+@pytest.mark.parametrize("case", GOLD, ids=[c["name"] for c in GOLD])
+def test_oracle_matches_50_digit_golden(case):
+    src, tgt = golden_source(case)
+    prm = O.params(epsilon=case["eps"], use_lse=case["use_lse"], n_threads=2)
+    F, g, C = O.evaluate_R(src, tgt, prm, case["R"], np.array(case["psi"]))
+    Fg = float(case["F"])
+    gg = np.array([float(v) for v in case["g"]])
+    assert abs(F - Fg) <= 1e-13 * max(1.0, abs(Fg))
+    scale = np.max(np.abs(gg + np.array(case["nu"])))
+    assert np.max(np.abs(g - gg)) <= 1e-13 * scale
+    assert abs(C - float(case["C"])) <= 1e-12 * abs(float(case["C"]))
+    cnt, h = O.candidate_stats(src, tgt, case["R"])
+    assert cnt.tolist() == case["count"]
+    assert [str(v) for v in h.tolist()] == case["hash"]
+
+
+def test_gauss_legendre_textbook():
+    p, w = O.gauss_legendre01(2)
+    assert np.allclose(p, [0.5 - 0.5 / np.sqrt(3), 0.5 + 0.5 / np.sqrt(3)], atol=1e-15)
+    assert np.allclose(w, [0.5, 0.5], atol=1e-15)
+    p, w = O.gauss_legendre01(3)
+    assert np.allclose(p, [0.5 - 0.5 * np.sqrt(0.6), 0.5, 0.5 + 0.5 * np.sqrt(0.6)], atol=1e-15)
+    assert np.allclose(w, [5 / 18, 8 / 18, 5 / 18], atol=1e-15)
+    for n in range(1, 8):  # exact for degree 2n-1
+        p, w = O.gauss_legendre01(n)
+        assert abs(np.sum(w * p ** (2 * n - 1)) - 1.0 / (2 * n)) < 1e-14
+
+
+def test_structured_source_integrates_polynomials():
+    src = O.structured_source(3, [3, 4, 5], [0.0, -1.0, 2.0], [1.0, 1.0, 3.0], 2, normalize=False)
+    x = src.x.reshape(-1, 3)
+    assert abs(src.jxw.sum() - 2.0) < 1e-13
+    # int x0 y1^2 z2 over the box = (1/2) * (2/3) * (5/2)
+    assert abs(np.sum(src.jxw * x[:, 0] * x[:, 1] ** 2 * x[:, 2]) - 0.5 * (2 / 3) * 2.5) < 1e-13
+    # bilinear nodal density is reproduced exactly at quadrature points
+    g = [np.linspace(0, 1, 4), np.linspace(-1, 1, 5), np.linspace(2, 3, 6)]
+    Z, Y, X = np.meshgrid(g[2], g[1], g[0], indexing="ij")
+    f = lambda a, b, c: 1 + a + 2 * b + 3 * c + a * b * c
+    s2 = O.structured_source(3, [3, 4, 5], [0.0, -1.0, 2.0], [1.0, 1.0, 3.0], 2,
+                             f(X, Y, Z).ravel(), normalize=False)
+    assert np.allclose(s2.rho, f(x[:, 0], x[:, 1], x[:, 2]), atol=1e-13)
+    c = src.center.reshape(-1, 3)
+    assert np.allclose(c[0], [1 / 6, -0.75, 2.1])
+
+
+def test_simplex_source_volume_and_rule():
+    verts = np.array([[0, 0, 0], [2, 0, 0], [0, 3, 0], [0, 0, 1.5]], dtype=float)
+    src = O.simplex_source(3, verts.ravel(), np.array([0, 1, 2, 3]), 2, normalize=False)
+    assert abs(src.jxw.sum() - 2 * 3 * 1.5 / 6) < 1e-14
+    x = src.x.reshape(-1, 3)
+    # the 4-point rule is exact for quadratics:
+    # int_T x0^2 = V (sum_i v_i^2 + sum_{i<j} v_i v_j) / 10 with v = x0 of the vertices
+    vol = 1.5
+    exact_xx = vol * (2.0 ** 2) / 10.0
+    assert abs(np.sum(src.jxw * x[:, 0] ** 2) - exact_xx) < 1e-13
+
+
+def test_single_target_closed_form():
+    """N=1, nu=1: g = mass - 1 = 0 and F = -1/2 sum_q w_q |x_q - y|^2, which for a
+    uniform unit square is -(|c-y|^2 + 1/6)/2 (QGauss(3) integrates it exactly)."""
+    src = O.structured_source(2, [8, 8], [0, 0], [1, 1], 3)
+    for y, psi in [((0.3, 0.6), 0.0), ((1.4, -0.2), 0.7)]:
+        tgt = O.Target(np.array(y), [1.0])
+        for lse in (0, 1):
+            F, g, _ = O.evaluate_R(src, tgt, O.params(epsilon=1e-2, use_lse=lse), 1e300, [psi])
+            cf = -0.5 * ((0.5 - y[0]) ** 2 + (0.5 - y[1]) ** 2 + 1 / 6)
+            assert abs(F - cf) < 1e-13
+            assert abs(g[0]) < 1e-13
+
+
+def test_quadrant_symmetry_zero_gradient():
+    src = O.structured_source(2, [8, 8], [0, 0], [1, 1], 3)
+    y = np.array([[0.25, 0.25], [0.75, 0.25], [0.25, 0.75], [0.75, 0.75]])
+    tgt = O.Target(y.ravel(), np.full(4, 0.25))
+    for eps in (1e-1, 1e-2, 1e-3):
+        F, g, _ = O.evaluate_R(src, tgt, O.params(epsilon=eps, use_lse=1), 1e300, np.zeros(4))
+        assert np.max(np.abs(g)) < 1e-13
+
+
+def test_invariants_and_finite_differences():
+    src, tgt = structured_case(2, 6, 3, 15, seed=11, gmm_source=True)
+    rng = np.random.default_rng(5)
+    psi = 0.02 * rng.standard_normal(tgt.n)
+    prm = O.params(epsilon=2e-2, use_lse=1)
+    F, g, _ = O.evaluate_R(src, tgt, prm, 1e300, psi)
+    assert abs(g.sum()) < 1e-13                     # sum g = mass - sum nu = 0
+    F2, g2, _ = O.evaluate_R(src, tgt, prm, 1e300, psi + 0.3)
+    assert abs(F2 - F) < 1e-13 and np.max(np.abs(g2 - g)) < 1e-13  # shift invariance
+    h = 1e-6
+    fd = [(O.evaluate_R(src, tgt, prm, 1e300, psi + h * e)[0] -
+           O.evaluate_R(src, tgt, prm, 1e300, psi - h * e)[0]) / (2 * h) for e in np.eye(tgt.n)]
+    assert np.max(np.abs(np.array(fd) - g)) < 1e-8
+    # plain and LSE modes agree where nothing underflows
+    Fp, gp, Cp = O.evaluate_R(src, tgt, O.params(epsilon=2e-2, use_lse=0), 1e300, psi)
+    assert abs(Fp - F) < 1e-13 and np.max(np.abs(gp - g)) < 1e-14
+
+
+def test_hessian_fd_symmetry_kernel():
+    src, tgt = structured_case(2, 5, 2, 12, seed=3)
+    psi = 0.01 * np.random.default_rng(1).standard_normal(tgt.n)
+    prm = O.params(epsilon=3e-2, use_lse=1)
+    H = O.hessian(src, tgt, prm, 1e300, psi)
+    h = 1e-6
+    fd = np.array([(O.evaluate_R(src, tgt, prm, 1e300, psi + h * e)[1] -
+                    O.evaluate_R(src, tgt, prm, 1e300, psi - h * e)[1]) / (2 * h)
+                   for e in np.eye(tgt.n)])
+    assert np.max(np.abs(fd - H)) < 1e-7
+    assert np.max(np.abs(H - H.T)) < 1e-15
+    assert np.max(np.abs(H.sum(1))) < 1e-12
+    assert np.min(np.linalg.eigvalsh(H)) > -1e-12
+
+
+def test_thresholds_closed_form():
+    src, tgt = structured_case(2, 4, 2, 10, seed=2, dirichlet=5)
+    psi = np.linspace(-0.1, 0.2, tgt.n)
+    st = O.new_state()
+    eps, tau = 1e-2, 1e-8
+    R = O.distance_threshold(src, tgt, O.params(epsilon=eps, tau=tau), st, psi)
+    assert abs(R - np.sqrt(-2 * eps * np.log(tau / tgt.nu.min()) + 2 * 0.2)) < 1e-15
+    st.C_global, st.global_functional, st.covering_radius = 3.0, -0.05, 0.01
+    R = O.distance_threshold(src, tgt, O.params(epsilon=eps, tau=tau, threshold="integral"), st, psi)
+    assert abs(R - np.sqrt(2 * 0.2 + 2 * eps * np.log(eps * 3.0 / (tau * 0.05)))) < 1e-15
+    st.covering_radius, st.min_target_density = 0.3, 0.0
+    R = O.distance_threshold(src, tgt, O.params(epsilon=eps, tau=tau, threshold="geometric"), st, psi)
+    r2 = 0.09 + 2 * 0.3 + 2 * eps * np.log(eps / (tgt.nu.min() * tau * 0.05))
+    assert abs(R - np.sqrt(r2)) < 1e-15
+    # tau <= 0 disables truncation
+    assert O.distance_threshold(src, tgt, O.params(tau=0.0), O.new_state(), psi) == np.finfo(float).max
+
+
+def test_candidate_sets_brute_force():
+    src, tgt = structured_case(3, 4, 2, 60, seed=8, gmm_targets=True)
+    R = 0.31
+    cnt, h = O.candidate_stats(src, tgt, R)
+    c = src.center.reshape(-1, 3)
+    y = tgt.y.reshape(-1, 3)
+    d = np.sqrt(((y[None, :, :] - c[:, None, :]) ** 2).sum(-1))
+    assert cnt.tolist() == (d <= R).sum(1).tolist()
+
+
+def test_bfgs_textbook_problems():
+    A = np.array([[4.0, 1.0, 0.5], [1.0, 3.0, 0.2], [0.5, 0.2, 2.0]])
+    b = np.array([1.0, -2.0, 0.5])
+    x, info = O.bfgs_generic(lambda x: (0.5 * x @ A @ x - b @ x, A @ x - b), np.zeros(3), tol=1e-12)
+    assert info["status"] == "converged"
+    assert np.allclose(x, np.linalg.solve(A, b), atol=1e-11)
+    rb = lambda x: ((1 - x[0]) ** 2 + 100 * (x[1] - x[0] ** 2) ** 2,
+                    np.array([-2 * (1 - x[0]) - 400 * x[0] * (x[1] - x[0] ** 2),
+                              200 * (x[1] - x[0] ** 2)]))
+    x, info = O.bfgs_generic(rb, np.array([-1.2, 1.0]), max_iterations=2000, tol=1e-10)
+    assert info["status"] == "converged" and np.allclose(x, [1, 1], atol=1e-8)
+
+
+def test_solve_reaches_componentwise_tolerance():
+    src, tgt = structured_case(2, 8, 2, 30, seed=4, gmm_source=True, dirichlet=5)
+    prm = O.params(epsilon=2e-2, tau=1e-12, use_lse=1, control="componentwise",
+                   tolerance=1e-6, max_iterations=500)
+    psi, info = O.solve(src, tgt, prm)
+    assert info["status"] == "converged"
+    F, g, _ = O.evaluate_R(src, tgt, prm, 1e300, psi)
+    assert np.max(np.abs(g) / tgt.nu) <= 1e-6 * 1.0001
+    prm1 = O.params(epsilon=2e-2, use_lse=1, max_iterations=1, tolerance=1e-12)
+    _, info1 = O.solve(src, tgt, prm1)
+    assert info1["status"] == "max_iterations" and info1["iterations"] == 1
+
+
+def test_tet_source_mass():
+    src, tgt, _ = tet_case(300, 20, seed=3)
+    assert abs(src.w.sum() - 1.0) < 1e-12
+    assert src.nq == 4
