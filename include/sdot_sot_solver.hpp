@@ -1,0 +1,126 @@
+// sdot_sot_solver.hpp -- header-only C++ facade with the method set of the
+// reference's SotSolver<dim, spacedim> (include/SemiDiscreteOT/solvers/SotSolver.h:
+// 176-348) over the C ABI of sdot.h.  No deal.II types: setup_source takes the
+// flat arrays a deal.II caller pulls once from FEValues (see INTEGRATION.md for
+// the loop that fills them).  Errors throw std::runtime_error as the reference
+// does for invalid measures (SotSolver.cc:110-112); non-convergence does not
+// throw (SotSolver.cc:207-211) and is reported by get_convergence_status().
+#pragma once
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "sdot.h"
+
+namespace sdot {
+
+class Error : public std::runtime_error {
+ public:
+  Error(int code, const std::string &msg) : std::runtime_error(msg), code(code) {}
+  int code;
+};
+
+inline void check(int rc) {
+  if (rc != SDOT_OK) throw Error(rc, sdot_last_error());
+}
+
+template <int dim, int spacedim = dim>
+class SotSolverGPU {
+  static_assert(spacedim == 2 || spacedim == 3, "spacedim must be 2 or 3");
+
+ public:
+  using Params = sdot_params;
+
+  explicit SotSolverGPU(int device = 0) {
+    check(sdot_ctx_create(device, &ctx_));
+    check(sdot_params_default(&params_));
+  }
+  ~SotSolverGPU() { sdot_ctx_destroy(ctx_); }
+  SotSolverGPU(const SotSolverGPU &) = delete;
+  SotSolverGPU &operator=(const SotSolverGPU &) = delete;
+
+  static Params default_parameters() {
+    Params p;
+    check(sdot_params_default(&p));
+    return p;
+  }
+  static Params parameters_from_prm(const std::string &path) {
+    Params p = default_parameters();
+    check(sdot_params_load_prm(path.c_str(), &p));
+    return p;
+  }
+
+  // SotSolver::setup_source: x [n_cells*nq*spacedim], w = rho*JxW [n_cells*nq],
+  // centers [n_cells*spacedim].
+  void setup_source(const std::vector<double> &x, const std::vector<double> &w,
+                    const std::vector<double> &centers, int nq) {
+    const int64_t n_cells = (int64_t)(centers.size() / spacedim);
+    if ((int64_t)w.size() != n_cells * nq || (int64_t)x.size() != n_cells * nq * spacedim)
+      throw Error(SDOT_ERR_INVALID, "setup_source: inconsistent array sizes");
+    check(sdot_set_source(ctx_, spacedim, n_cells, nq, x.data(), w.data(), centers.data()));
+  }
+
+  // SotSolver::setup_target: points [N*spacedim], weights [N].
+  void setup_target(const std::vector<double> &points, const std::vector<double> &weights) {
+    if (points.size() != weights.size() * spacedim)
+      throw Error(SDOT_ERR_INVALID, "setup_target: dimension mismatch");
+    n_ = (int64_t)weights.size();
+    check(sdot_set_target(ctx_, n_, points.data(), weights.data()));
+  }
+
+  void configure(const Params &p) {
+    params_ = p;
+    check(sdot_set_params(ctx_, &params_));
+  }
+
+  // SotSolver::solve(potential, params): potential resized to N if needed
+  // (SotSolver.cc:147-150).
+  void solve(std::vector<double> &potential, const Params &p) {
+    configure(p);
+    if (n_ == 0) throw Error(SDOT_ERR_STATE, "Invalid measures configuration");
+    if ((int64_t)potential.size() != n_) potential.assign(n_, 0.0);
+    check(sdot_solve(ctx_, potential.data(), &info_));
+  }
+
+  // SotSolver::evaluate_functional
+  double evaluate_functional(const std::vector<double> &potential, std::vector<double> &gradient_out) {
+    gradient_out.resize(n_);
+    double F = 0, C = 0;
+    check(sdot_eval(ctx_, potential.data(), gradient_out.data(), &F, &C));
+    return F;
+  }
+
+  // SotSolver::compute_hessian (row-major N x N)
+  void compute_hessian(const std::vector<double> &potential, std::vector<double> &hessian_out) {
+    hessian_out.resize((size_t)n_ * n_);
+    check(sdot_hessian(ctx_, potential.data(), hessian_out.data(), n_));
+  }
+
+  double get_last_functional_value() const { return state().last_functional; }
+  unsigned int get_last_iteration_count() const { return (unsigned)state().last_iteration_count; }
+  bool get_convergence_status() const { return state().converged != 0; }
+  double get_last_distance_threshold() const { return state().last_distance_threshold; }
+  double get_C_global() const { return state().C_global; }
+  void set_distance_threshold(double t) { check(sdot_set_distance_threshold(ctx_, t)); }
+  double compute_covering_radius() const {
+    double r = 0;
+    check(sdot_covering_radius(ctx_, &r));
+    return r;
+  }
+  const sdot_solve_info &last_solve_info() const { return info_; }
+  sdot_ctx *handle() const { return ctx_; }
+
+ private:
+  sdot_state state() const {
+    sdot_state s;
+    check(sdot_get_state(ctx_, &s));
+    return s;
+  }
+  sdot_ctx *ctx_ = nullptr;
+  Params params_{};
+  sdot_solve_info info_{};
+  int64_t n_ = 0;
+};
+
+}  // namespace sdot

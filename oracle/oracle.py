@@ -84,6 +84,9 @@ def lib():
                                      C.c_double, _dp, _dp, _dp, _dp]
         L.orc_candidate_stats.argtypes = [C.POINTER(_Source), C.POINTER(_Target), C.c_double,
                                           _i64p, C.POINTER(C.c_uint64)]
+        L.orc_discarded_fraction.argtypes = [C.POINTER(_Source), C.POINTER(_Target),
+                                             C.POINTER(_Params), C.c_double, _dp, C.c_int64,
+                                             _i64p, _dp]
         L.orc_hessian.argtypes = [C.POINTER(_Source), C.POINTER(_Target), C.POINTER(_Params),
                                   C.c_double, _dp, _dp]
         L.orc_solve.argtypes = [C.POINTER(_Source), C.POINTER(_Target), C.POINTER(_Params),
@@ -236,6 +239,14 @@ def candidate_stats(src, tgt, R):
     lib().orc_candidate_stats(C.byref(src._s), C.byref(tgt._t), float(R),
                               cnt.ctypes.data_as(_i64p), h.ctypes.data_as(C.POINTER(C.c_uint64)))
     return cnt, h
+
+
+def discarded_fraction(src, tgt, prm, R, psi, q_list):
+    q = np.ascontiguousarray(q_list, dtype=np.int64)
+    out = np.zeros(q.size)
+    lib().orc_discarded_fraction(C.byref(src._s), C.byref(tgt._t), C.byref(prm), float(R),
+                                 _p(_f64(psi)), q.size, q.ctypes.data_as(_i64p), _p(out))
+    return out
 
 
 def hessian(src, tgt, prm, R, psi):

@@ -514,6 +514,38 @@ int orc_evaluate(const orc_source_t *s, const orc_target_t *t,
   return rc;
 }
 
+/* Discarded mass of the truncation (what the pointwise/integral/geometric
+ * bounds of SotSolver.cc:379-551 are meant to keep below tau): for each listed
+ * quadrature point, the fraction of its untruncated mass
+ * sum_j nu_j exp(e_qj) that falls on targets outside its cell's candidate set. */
+int orc_discarded_fraction(const orc_source_t *s, const orc_target_t *t,
+                           const orc_params_t *p, double R, const double *psi,
+                           int64_t n_list, const int64_t *q_list, double *frac) {
+  const int dim = s->dim;
+  const double inv = 1.0 / p->epsilon;
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t k = 0; k < n_list; ++k) {
+    const int64_t g = q_list[k];
+    const int64_t c = g / s->nq;
+    const double *xq = s->x + g * dim;
+    double mx = -DBL_MAX;
+    for (int64_t j = 0; j < t->n; ++j) {
+      double d2 = pow(orc_dist(xq, t->y + j * dim, dim), 2);
+      double e = (psi[j] - 0.5 * d2) * inv;
+      mx = e > mx ? e : mx;
+    }
+    double all = 0.0, out = 0.0;
+    for (int64_t j = 0; j < t->n; ++j) {
+      double d2 = pow(orc_dist(xq, t->y + j * dim, dim), 2);
+      double v = t->nu[j] * exp((psi[j] - 0.5 * d2) * inv - mx);
+      all += v;
+      if (!(orc_dist(t->y + j * dim, s->center + c * dim, dim) <= R)) out += v;
+    }
+    frac[k] = all > 0 ? out / all : 0.0;
+  }
+  return 0;
+}
+
 /* Dense Hessian of the (truncated) dual functional.  compute_hessian is only
  * declared in the reference (SotSolver.h:338-344); the matrix is the
  * derivative of the gradient assembled above:

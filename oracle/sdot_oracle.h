@@ -103,6 +103,9 @@ int orc_evaluate_R(const orc_source_t *s, const orc_target_t *t,
                    double *grad, double *F, double *C);
 int orc_candidate_stats(const orc_source_t *s, const orc_target_t *t, double R,
                         int64_t *count, uint64_t *hash);
+int orc_discarded_fraction(const orc_source_t *s, const orc_target_t *t,
+                           const orc_params_t *p, double R, const double *psi,
+                           int64_t n_list, const int64_t *q_list, double *frac);
 int orc_hessian(const orc_source_t *s, const orc_target_t *t,
                 const orc_params_t *p, double R, const double *psi, double *H);
 

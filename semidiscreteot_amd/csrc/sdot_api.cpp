@@ -141,6 +141,7 @@ struct sdot_ctx {
   int last_iterations = 0, converged = 0;
   int64_t last_pairs = -1;
   bool have_source = false, have_target = false;
+  int force_mode = -1;  // overrides the arithmetic mode (the Hessian runs FP64 LSE)
   Plan plan;
   Dev dev;
   std::vector<double> h_nu;
@@ -191,6 +192,7 @@ int prepare(sdot_ctx *c, double R) {
   if (!(p.eps > 0) || !std::isfinite(p.eps)) return fail(SDOT_ERR_INVALID, "epsilon must be > 0");
   const bool lse = c->params.use_log_sum_exp_trick != 0;
   p.mode = !lse ? kF64Plain : (c->params.precision == SDOT_PREC_FP32_LSE ? kF32Lse : kF64Lse);
+  if (c->force_mode >= 0) p.mode = c->force_mode;
   // dense when every center-target distance is surely <= R
   double diag2 = 0.0;
   for (int d = 0; d < p.dim; ++d) {
@@ -670,10 +672,47 @@ extern "C" int sdot_synchronize(sdot_ctx *c) {
 
 extern "C" int sdot_hessian(sdot_ctx *c, const double *psi, double *H, int64_t ld) {
   if (ctx_check(c)) return SDOT_ERR_INVALID;
-  (void)psi;
-  (void)H;
-  (void)ld;
-  return fail(SDOT_ERR_UNSUPPORTED, "sdot_hessian: not built yet");
+  if (!c->have_source || !c->have_target)
+    return fail(SDOT_ERR_STATE, "Invalid measures configuration");
+  if (!psi || !H) return fail(SDOT_ERR_INVALID, "null argument");
+  const int64_t N = c->plan.N;
+  if (ld < N) return fail(SDOT_ERR_INVALID, "ld < N");
+  if (N > 32768) return fail(SDOT_ERR_UNSUPPORTED, "dense Hessian limited to N <= 32768");
+  HIP_TRY(hipSetDevice(c->device));
+  double R;
+  int rc = threshold(c, psi, &R);
+  if (rc) return rc;
+  // p_qj in FP64 with the max shift, whatever the evaluation mode
+  c->force_mode = kF64Lse;
+  double F, C;
+  rc = host_eval(c, R, psi, nullptr, &F, &C);
+  c->force_mode = -1;
+  if (rc) return rc;
+  const Plan &p = c->plan;
+  const int64_t Npad = (N + 63) / 64 * 64;
+  const int64_t budget = (int64_t)1 << 28;  // bytes of one P chunk
+  int64_t rows = std::max<int64_t>(4, budget / (8 * Npad) / 4 * 4);
+  rows = std::min<int64_t>(rows, (p.nQ + 3) / 4 * 4);
+  double *dA = nullptr, *dP = nullptr, *dH = nullptr;
+  auto cleanup = [&]() {
+    if (dA) (void)hipFree(dA);
+    if (dP) (void)hipFree(dP);
+    if (dH) (void)hipFree(dH);
+  };
+  hipError_t e = hipMalloc((void **)&dA, sizeof(double) * Npad * Npad);
+  if (e == hipSuccess) e = hipMalloc((void **)&dP, sizeof(double) * rows * Npad);
+  if (e == hipSuccess) e = hipMalloc((void **)&dH, sizeof(double) * N * N);
+  if (e == hipSuccess) e = hipMemsetAsync(dA, 0, sizeof(double) * Npad * Npad, c->stream);
+  for (int64_t q0 = 0; e == hipSuccess && q0 < p.nQ; q0 += rows)
+    e = launch_hessian_chunk(p, c->dev, q0, rows, dP, dA, Npad, c->stream);
+  if (e == hipSuccess) e = launch_hessian_final(p, dA, Npad, c->dev.grad, c->dev.tnu_o, dH, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpy2DAsync(H, sizeof(double) * ld, dH, sizeof(double) * N, sizeof(double) * N, N,
+                         hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  cleanup();
+  if (e != hipSuccess) return fail(SDOT_ERR_HIP, std::string("hessian: ") + hipGetErrorString(e));
+  return SDOT_OK;
 }
 
 extern "C" int sdot_solve(sdot_ctx *c, double *psi, sdot_solve_info *info) {

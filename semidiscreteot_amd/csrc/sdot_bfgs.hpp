@@ -210,3 +210,103 @@ BfgsResult lbfgs_solve(int64_t n, double *x, Eval &&eval, Check &&check, int his
 }
 
 }  // namespace sdot
+
+namespace sdot {
+
+// Resumable form of wolfe_line_search for drivers that evaluate many line
+// searches in lock-step (one batched evaluation per step): the caller
+// evaluates at trial() and feeds (f, slope) back.  The decisions are those of
+// wolfe_line_search, step for step.
+class LineSearchSM {
+ public:
+  enum { kEval = 0, kAccept = 1, kFail = -1 };
+
+  void start(double f0, double g0, double a1) {
+    f0_ = f0;
+    g0_ = g0;
+    tol_ = 0.9 * std::fabs(g0);
+    floor_ = f0 + DBL_MAX * 0.001 * g0;
+    used_ = 0;
+    phase_ = 0;
+    prev_a_ = 0.0;
+    prev_f_ = f0;
+    prev_g_ = g0;
+    a_ = a1;
+  }
+  double trial() const { return a_; }
+  unsigned evaluations() const { return used_; }
+
+  int feed(double f, double g) {
+    constexpr unsigned max_evals = 20;
+    ++used_;
+    const double a = a_;
+    if (phase_ == 0) {
+      if (f <= floor_ || a == DBL_MAX) return kAccept;
+      if (!armijo(a, f) || (f >= prev_f_ && used_ > 1)) {
+        lo_a_ = prev_a_; lo_f_ = prev_f_; lo_g_ = prev_g_;
+        hi_a_ = a; hi_f_ = f; hi_g_ = g;
+        return to_sectioning(max_evals);
+      }
+      if (std::fabs(g) <= tol_) return kAccept;
+      if (g >= 0) {
+        lo_a_ = a; lo_f_ = f; lo_g_ = g;
+        hi_a_ = prev_a_; hi_f_ = prev_f_; hi_g_ = prev_g_;
+        return to_sectioning(max_evals);
+      }
+      const double next = std::fmin(DBL_MAX, a + 9.0 * (a - prev_a_));
+      prev_a_ = a; prev_f_ = f; prev_g_ = g;
+      a_ = next;
+      return used_ < max_evals ? kEval : kFail;
+    }
+    if (!armijo(a, f) || f >= lo_f_) {
+      hi_a_ = a; hi_f_ = f; hi_g_ = g;
+    } else {
+      if (std::fabs(g) <= tol_) return kAccept;
+      if (g * (hi_a_ - lo_a_) >= 0) {
+        hi_a_ = lo_a_; hi_f_ = lo_f_; hi_g_ = lo_g_;
+      }
+      lo_a_ = a; lo_f_ = f; lo_g_ = g;
+    }
+    if (used_ >= max_evals) return kFail;
+    a_ = section_trial();
+    return kEval;
+  }
+
+ private:
+  bool armijo(double a, double f) const { return f <= f0_ + a * 0.001 * g0_; }
+  int to_sectioning(unsigned max_evals) {
+    phase_ = 1;
+    if (used_ >= max_evals) return kFail;
+    a_ = section_trial();
+    return kEval;
+  }
+  double section_trial() const {
+    const double s1 = lo_a_ + 0.1 * (hi_a_ - lo_a_);
+    const double s2 = hi_a_ - 0.5 * (hi_a_ - lo_a_);
+    const double bl = std::fmin(s1, s2), bh = std::fmax(s1, s2);
+    const double b1 = lo_g_ + hi_g_ - 3.0 * (lo_f_ - hi_f_) / (lo_a_ - hi_a_);
+    const double disc = b1 * b1 - lo_g_ * hi_g_;
+    if (disc >= 0) {
+      const double b2 = std::sqrt(disc);
+      const double den = lo_a_ < hi_a_ ? hi_g_ - lo_g_ + 2.0 * b2 : lo_g_ - hi_g_ + 2.0 * b2;
+      if (den != 0.0) {
+        const double t = lo_a_ < hi_a_ ? hi_a_ - (hi_a_ - lo_a_) * (hi_g_ + b2 - b1) / den
+                                       : lo_a_ - (lo_a_ - hi_a_) * (lo_g_ + b2 - b1) / den;
+        if (t >= bl && t <= bh) return t;
+      }
+    }
+    const double den = 2.0 * lo_g_ * hi_a_ - 2.0 * lo_g_ * lo_a_ - 2.0 * hi_f_ + 2.0 * lo_f_;
+    if (den != 0) {
+      const double t = (lo_g_ * (hi_a_ * hi_a_ - lo_a_ * lo_a_) + 2.0 * (lo_f_ - hi_f_) * lo_a_) / den;
+      if (t >= bl && t <= bh) return t;
+    }
+    return 0.5 * (bl + bh);
+  }
+  double f0_ = 0, g0_ = 0, tol_ = 0, floor_ = 0;
+  unsigned used_ = 0;
+  int phase_ = 0;
+  double a_ = 0, prev_a_ = 0, prev_f_ = 0, prev_g_ = 0;
+  double lo_a_ = 0, lo_f_ = 0, lo_g_ = 0, hi_a_ = 0, hi_f_ = 0, hi_g_ = 0;
+};
+
+}  // namespace sdot

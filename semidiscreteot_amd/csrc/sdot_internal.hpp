@@ -112,7 +112,9 @@ hipError_t launch_eval(const Plan &p, Dev &d, const double *d_psi, double *d_gra
 hipError_t launch_candidate_stats(const Plan &p, Dev &d, int64_t *d_count,
                                   uint64_t *d_hash, hipStream_t s);
 hipError_t launch_covering(const Plan &p, Dev &d, double *d_cellmin, hipStream_t s);
-hipError_t launch_hessian(const Plan &p, Dev &d, const double *d_psi, double *d_H,
-                          int64_t ld, hipStream_t s);
+hipError_t launch_hessian_chunk(const Plan &p, Dev &d, int64_t q_begin, int64_t rows,
+                                double *d_P, double *d_A, int64_t Npad, hipStream_t s);
+hipError_t launch_hessian_final(const Plan &p, const double *d_A, int64_t Npad, const double *d_g,
+                                const double *d_nu, double *d_H, hipStream_t s);
 
 }  // namespace sdot

@@ -26,6 +26,8 @@
 // s <= smax with smax the largest double whose sqrt is <= R (host-computed).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "sdot_internal.hpp"
 
 namespace sdot {
@@ -308,8 +310,9 @@ __global__ __launch_bounds__(256) void k_qfin(QFArgs a) {
     for (int d = 0; d < DIM; ++d) r[d] = x[d];
     r[DIM] = Mrec;
     r[DIM + 1] = scale;
+    r[DIM + 2] = (scale > 0.0) ? S : 0.0;
 #pragma unroll
-    for (int t = DIM + 2; t < kRec; ++t) r[t] = 0.0;
+    for (int t = DIM + 3; t < kRec; ++t) r[t] = 0.0;
   }
   const double fs = block_sum_256(Fq, sh);
   __syncthreads();
@@ -498,6 +501,132 @@ __global__ __launch_bounds__(256) void k_cover(const double *cc, const double *t
     best = fmin(best, s);
   }
   if (active) cellmin[c] = best;
+}
+
+// ---------------------------------------------------------------------------
+// Dense Hessian  H = (1/eps) [ diag(g + nu) - P^T P ],
+//   P[q][j] = sqrt(w_q) p_qj,  p_qj = nu_j exp(t_qj - M_q) / S_q  (0 off C(cell(q)))
+// built in chunks of quadrature rows; P^T P by FP64 MFMA (16x16x4), upper
+// block triangle only, accumulated chunk after chunk in a fixed order.
+// ---------------------------------------------------------------------------
+template <int DIM>
+__global__ __launch_bounds__(256) void k_pbuild(const double *trec_m, const int32_t *pos_m,
+                                                const double *qrec_r, const double *cc_r,
+                                                int64_t N, int64_t Npad, int nq, int64_t q_begin,
+                                                int64_t rows, int64_t nQ, int dense, double smax,
+                                                double *P) {
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j >= Npad) return;
+  double y[DIM], yl[DIM], ta = 0.0, nu = 0.0;
+  if (j < N) {
+    const double *r = trec_m + (int64_t)pos_m[j] * kRec;
+    ta = r[0];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      yl[d] = r[1 + d];
+      y[d] = r[1 + DIM + d];
+    }
+    nu = r[1 + 2 * DIM];
+  }
+  for (int64_t row = blockIdx.y; row < rows; row += gridDim.y) {
+    const int64_t qr = q_begin + row;
+    double v = 0.0;
+    if (j < N && qr < nQ) {
+      const double *qp = qrec_r + qr * kRec;
+      const double scale = qp[DIM + 1];
+      if (scale > 0.0) {
+        bool in = true;
+        if (!dense) {
+          double c[DIM];
+#pragma unroll
+          for (int d = 0; d < DIM; ++d) c[d] = cc_r[(qr / nq) * 4 + d];
+          in = in_set<DIM>(y, c, smax);
+        }
+        if (in) {
+          double t = ta;
+#pragma unroll
+          for (int d = 0; d < DIM; ++d) t = fma(qp[d], yl[d], t);
+          v = nu * exp(t - qp[DIM]) * sqrt(scale / qp[DIM + 2]);
+        }
+      }
+    }
+    P[row * Npad + j] = v;
+  }
+}
+
+typedef double sdot_d4 __attribute__((ext_vector_type(4)));
+
+// A[j][k] += sum_q P[q][j] P[q][k] on 64x64 blocks with bk >= bj.  Four waves
+// per block, each a 32x32 quadrant as 2x2 MFMA 16x16 tiles.
+__global__ __launch_bounds__(256) void k_syrk(const double *__restrict__ P, int64_t rows,
+                                              int64_t Npad, double *__restrict__ A) {
+  const int bj = blockIdx.y, bk = blockIdx.x;
+  if (bk < bj) return;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int wj = w >> 1, wk = w & 1;
+  const int64_t j0 = (int64_t)bj * 64 + wj * 32, k0 = (int64_t)bk * 64 + wk * 32;
+  sdot_d4 acc[2][2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int u = 0; u < 2; ++u) acc[t][u] = sdot_d4{0.0, 0.0, 0.0, 0.0};
+  const int lr = l >> 4, lc = l & 15;
+  for (int64_t q0 = 0; q0 < rows; q0 += 4) {
+    const double *row = P + (q0 + lr) * Npad;
+    double a[2], b[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      a[t] = row[j0 + t * 16 + lc];
+      b[t] = row[k0 + t * 16 + lc];
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+        acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], b[u], acc[t][u], 0, 0, 0);
+  }
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int64_t jj = j0 + t * 16 + lr * 4 + i, kk = k0 + u * 16 + lc;
+        A[jj * Npad + kk] += acc[t][u][i];
+      }
+}
+
+__global__ __launch_bounds__(256) void k_hfin(const double *A, int64_t Npad, const double *g,
+                                              const double *nu, int64_t N, double inv_eps,
+                                              double *H) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= N) return;
+  for (int64_t j = blockIdx.y; j < N; j += gridDim.y) {
+    const int64_t lo = j < k ? j : k, hi = j < k ? k : j;
+    const double diag = (j == k) ? g[j] + nu[j] : 0.0;
+    H[j * N + k] = (diag - A[lo * Npad + hi]) * inv_eps;
+  }
+}
+
+hipError_t launch_hessian_chunk(const Plan &p, Dev &d, int64_t q_begin, int64_t rows, double *d_P,
+                                double *d_A, int64_t Npad, hipStream_t s) {
+  const dim3 gb((unsigned)((Npad + 255) / 256), (unsigned)std::min<int64_t>(rows, 4096));
+  if (p.dim == 2)
+    hipLaunchKernelGGL(k_pbuild<2>, gb, dim3(256), 0, s, d.trec_m, d.tpos_m, d.qrec_r, d.cc_r,
+                       p.N, Npad, p.nq, q_begin, rows, p.nQ, p.dense, p.smax, d_P);
+  else
+    hipLaunchKernelGGL(k_pbuild<3>, gb, dim3(256), 0, s, d.trec_m, d.tpos_m, d.qrec_r, d.cc_r,
+                       p.N, Npad, p.nq, q_begin, rows, p.nQ, p.dense, p.smax, d_P);
+  const unsigned nb = (unsigned)(Npad / 64);
+  hipLaunchKernelGGL(k_syrk, dim3(nb, nb), dim3(256), 0, s, d_P, rows, Npad, d_A);
+  return hipGetLastError();
+}
+
+hipError_t launch_hessian_final(const Plan &p, const double *d_A, int64_t Npad, const double *d_g,
+                                const double *d_nu, double *d_H, hipStream_t s) {
+  const dim3 g((unsigned)((p.N + 255) / 256), (unsigned)std::min<int64_t>(p.N, 4096));
+  hipLaunchKernelGGL(k_hfin, g, dim3(256), 0, s, d_A, Npad, d_g, d_nu, p.N, 1.0 / p.eps, d_H);
+  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------

@@ -130,3 +130,30 @@ def test_mesh_argument_errors():
         sd.structured_source(4, [2, 2, 2], [0] * 3, [1] * 3, 2)
     with pytest.raises(sd.SdotError):
         sd.simplex_source(3, np.zeros(12), np.array([0, 1, 2, 3]), 3)
+
+
+def test_cpp_facade_compiles_and_links(tmp_path):
+    """include/sdot_sot_solver.hpp (the SotSolver-shaped C++ facade) builds
+    against the C ABI and reads the reference defaults without a GPU."""
+    import subprocess
+    src = tmp_path / "facade.cpp"
+    src.write_text('#include "sdot_sot_solver.hpp"\n'
+                   "int main() { auto p = sdot::SotSolverGPU<3>::default_parameters();\n"
+                   "  return (p.max_iterations == 1000 && p.tau == 1e-8) ? 0 : 1; }\n")
+    exe = tmp_path / "facade"
+    libdir = os.path.dirname(L.LIB_PATH)
+    subprocess.run(["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), str(src),
+                    "-L", libdir, "-lsdot", f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True)
+    assert subprocess.run([str(exe)]).returncode == 0
+
+
+def test_lockstep_line_search_matches_loop_form(tmp_path):
+    """The resumable line search used by the batched server takes the same
+    decisions as the loop form used by sdot_solve (bitwise iterates on
+    Rosenbrock, tests/bfgs_sm_check.cpp)."""
+    import subprocess
+    exe = tmp_path / "sm"
+    subprocess.run(["g++", "-std=c++17", "-O2", os.path.join(ROOT, "tests", "bfgs_sm_check.cpp"),
+                    "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout

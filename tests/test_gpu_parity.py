@@ -229,3 +229,25 @@ def test_solve_matches_oracle():
     assert np.max(np.abs((psi - psi.mean()) - (psi_o - psi_o.mean()))) < 1e-6
     Fo, go, _ = O.evaluate_R(src, tgt, prm, DMAX, psi)
     assert np.max(np.abs(go) / tgt.nu) <= 1e-7 * 1.001
+
+
+@pytest.mark.parametrize("dim,tau", [(2, 0.0), (3, 1e-6)])
+def test_hessian_matches_oracle(dim, tau):
+    """compute_hessian (declared at SotSolver.h:338-344): GPU MFMA FP64 SYRK
+    against the oracle's dense assembly; H*1 = 0 and symmetry."""
+    say(f"hessian dim={dim} tau={tau}")
+    if dim == 2:
+        src, tgt = structured_case(2, 16, 3, 150, seed=21)
+    else:
+        src, tgt = structured_case(3, 6, 2, 200, seed=22, gmm_targets=True, dirichlet=5)
+    psi = 0.01 * np.random.default_rng(4).standard_normal(tgt.n)
+    s = make(src, tgt, epsilon=2e-2, tau=tau, use_log_sum_exp_trick=True)
+    R = s.distance_threshold(psi)
+    if dim == 3:
+        assert R < 1.0
+    H = s.compute_hessian(psi)
+    Ho = O.hessian(src, tgt, O.params(epsilon=2e-2, use_lse=1), R, psi)
+    scale = np.max(np.abs(Ho))
+    assert np.max(np.abs(H - Ho)) <= 1e-10 * scale
+    assert np.max(np.abs(H - H.T)) <= 1e-12 * scale
+    assert np.max(np.abs(H.sum(1))) <= 1e-10 * scale
