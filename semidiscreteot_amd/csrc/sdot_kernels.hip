@@ -48,6 +48,19 @@ __device__ __forceinline__ bool in_set(const double *y, const double *c, double 
   return s <= smax;
 }
 
+// Squared distance from point p to the box [lo, hi] (0 inside).  Only used
+// as a conservative prefilter in front of the exact candidate test.
+template <int DIM>
+__device__ __forceinline__ double box_dist2(const double *p, const double *lo, const double *hi) {
+  double s = 0.0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const double v = fmax(0.0, fmax(lo[d] - p[d], p[d] - hi[d]));
+    s += v * v;
+  }
+  return s;
+}
+
 template <int MODE>
 __device__ __forceinline__ double expo(double d) {
   if constexpr (MODE == kF32Lse) {
@@ -216,6 +229,7 @@ __global__ __launch_bounds__(64) void k_pass1(P1Args a) {
     hi[d] = a.blkbox[(int64_t)blockIdx.x * 2 * DIM + DIM + d];
   }
 
+  const double Rq2 = a.Rq * a.Rq;
   auto body = [&](int j0, int j1) {
     for (int j = j0; j < j1; ++j) {
       const double *__restrict__ r = a.trec + (int64_t)j * kRec;
@@ -223,6 +237,8 @@ __global__ __launch_bounds__(64) void k_pass1(P1Args a) {
         double yr[DIM];
 #pragma unroll
         for (int d = 0; d < DIM; ++d) yr[d] = r[1 + DIM + d];
+        // uniform: skip targets out of reach of every cell of the wave
+        if (box_dist2<DIM>(yr, lo, hi) > Rq2) continue;
         if (!in_set<DIM>(yr, c, a.smax)) continue;
       }
       const double ta = r[0];
@@ -338,7 +354,7 @@ struct GArgs {
   double smax, Rq;
 };
 
-template <int DIM, int MODE>
+template <int DIM, int MODE, int G>
 __global__ __launch_bounds__(64) void k_grad(GArgs a) {
   const int seg = blockIdx.y;
   const int64_t j = (int64_t)blockIdx.x * kWave + threadIdx.x;
@@ -357,6 +373,7 @@ __global__ __launch_bounds__(64) void k_grad(GArgs a) {
     lo[d] = a.twbox[(int64_t)blockIdx.x * 2 * DIM + d];
     hi[d] = a.twbox[(int64_t)blockIdx.x * 2 * DIM + DIM + d];
   }
+  const double Rq2 = a.Rq * a.Rq;
   double acc = 0.0;
   const int nq = a.nq;
   auto body = [&](int c0, int c1) {
@@ -366,14 +383,35 @@ __global__ __launch_bounds__(64) void k_grad(GArgs a) {
         double cv[DIM];
 #pragma unroll
         for (int d = 0; d < DIM; ++d) cv[d] = cp[d];
+        // uniform: skip cells out of reach of every target of the wave
+        if (box_dist2<DIM>(cv, lo, hi) > Rq2) continue;
         if (!in_set<DIM>(yr, cv, a.smax)) continue;
       }
       const double *__restrict__ qp = a.qrec_r + (int64_t)c * nq * kRec;
-      for (int k = 0; k < nq; ++k, qp += kRec) {
-        double t = ta;
+      for (int k0 = 0; k0 < nq; k0 += G, qp += G * kRec) {
+        // the chunk's records are read together, then consumed
+        double qx[G][DIM], qm[G], qs[G];
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) t = fma(qp[d], yl[d], t);
-        acc = fma(qp[DIM + 1], expo<MODE>(t - qp[DIM]), acc);
+        for (int k = 0; k < G; ++k) {
+#pragma unroll
+          for (int d = 0; d < DIM; ++d) qx[k][d] = qp[k * kRec + d];
+          qm[k] = qp[k * kRec + DIM];
+          qs[k] = qp[k * kRec + DIM + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+          double t = ta;
+#pragma unroll
+          for (int d = 0; d < DIM; ++d) t = fma(qx[k][d], yl[d], t);
+          const double e = expo<MODE>(t - qm[k]);
+          if (MODE == kF64Plain) {
+            // the reference adds only exp_terms > 0 (SotSolver.cc:292); in plain
+            // mode w/S_q can overflow to inf when S_q is subnormal
+            if (e > 0.0) acc = fma(qs[k], e, acc);
+          } else {
+            acc = fma(qs[k], e, acc);
+          }
+        }
       }
     }
   };
@@ -591,7 +629,8 @@ __global__ __launch_bounds__(256) void k_syrk(const double *__restrict__ P, int6
     for (int u = 0; u < 2; ++u)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int64_t jj = j0 + t * 16 + lr * 4 + i, kk = k0 + u * 16 + lc;
+        // f64 16x16x4 output: register i of lane l holds row 4*i + l/16, column l%16
+        const int64_t jj = j0 + t * 16 + i * 4 + lr, kk = k0 + u * 16 + lc;
         A[jj * Npad + kk] += acc[t][u][i];
       }
 }
@@ -743,8 +782,20 @@ static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_g
   g.dense = p.dense;
   g.smax = p.smax;
   g.Rq = p.Rq;
-  hipLaunchKernelGGL((k_grad<DIM, MODE>), dim3((unsigned)p.n_twaves, (unsigned)p.K2), dim3(64),
-                     0, s, g);
+  {
+    const dim3 gg((unsigned)p.n_twaves, (unsigned)p.K2);
+    switch (p.G) {
+      case 1: hipLaunchKernelGGL((k_grad<DIM, MODE, 1>), gg, dim3(64), 0, s, g); break;
+      case 2: hipLaunchKernelGGL((k_grad<DIM, MODE, 2>), gg, dim3(64), 0, s, g); break;
+      case 3: hipLaunchKernelGGL((k_grad<DIM, MODE, 3>), gg, dim3(64), 0, s, g); break;
+      case 4: hipLaunchKernelGGL((k_grad<DIM, MODE, 4>), gg, dim3(64), 0, s, g); break;
+      case 5: hipLaunchKernelGGL((k_grad<DIM, MODE, 5>), gg, dim3(64), 0, s, g); break;
+      case 6: hipLaunchKernelGGL((k_grad<DIM, MODE, 6>), gg, dim3(64), 0, s, g); break;
+      case 7: hipLaunchKernelGGL((k_grad<DIM, MODE, 7>), gg, dim3(64), 0, s, g); break;
+      case 8: hipLaunchKernelGGL((k_grad<DIM, MODE, 8>), gg, dim3(64), 0, s, g); break;
+      default: hipLaunchKernelGGL((k_grad<DIM, MODE, 9>), gg, dim3(64), 0, s, g); break;
+    }
+  }
   if (timing) (void)hipEventRecord(prof->ev[4], s);
   hipLaunchKernelGGL((k_gfin<DIM>), dim3((unsigned)((p.N + 255) / 256)), dim3(256), 0, s,
                      d.Gpart, d.trec_m, d.tm2o, d_grad, p.N, p.K2);
