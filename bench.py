@@ -2,11 +2,11 @@
 
 Workload: 3D unit cube, 32^3 Q1 hexes, QGauss(2) = 262,144 quadrature points,
 Gaussian-mixture density (seed 1); 50,000 Gaussian-mixture targets with
-Dirichlet(5) weights; eps = 1e-3; pointwise truncation (tau = 1e-16).
-A step is one evaluation of F(psi) and grad F(psi) (SotSolver::evaluate_functional)
-at a potential psi taken from a short L-BFGS run, with psi, grad and the
-scalar outputs resident in HBM; the metric is (quadrature point, candidate
-target) pair evaluations per second.
+Dirichlet(5) weights; eps = 1e-3; truncation radius calibrated so that the
+discarded mass is below 1e-12.  A step is one evaluation of F(psi) and
+grad F(psi) (SotSolver::evaluate_functional) at a potential psi taken from an
+untruncated L-BFGS run, with psi, grad and the scalar outputs resident in HBM;
+the metric is (quadrature point, candidate target) pair evaluations per second.
 
 Prints one JSON line.  Only N = 1 GPU is supported.
 """
@@ -30,7 +30,7 @@ def log(msg):
 
 # MI355X published vector peaks (non-packed FMA issue: 256 CU x 4 SIMD x 16 lanes x 2.4 GHz)
 FMA_PER_S = 256 * 64 * 2.4e9          # 3.93e13 FP64 or FP32 FMA/s (78.6 TFLOP/s)
-EXP_PER_S = FMA_PER_S / 4.0           # v_exp_f32 issues at quarter rate
+EXP_PER_S = FMA_PER_S / 4.0           # conservative: v_exp_f32 measured at >= 1.85e13/s
 
 
 def pass_peak_pairs(dim):
@@ -79,7 +79,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--psi-iters", type=int, default=40)
+    ap.add_argument("--psi-iters", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
@@ -97,22 +97,43 @@ def main():
     s = sd.SotSolver(3)
     s.setup_source(wl.source.x, wl.source.w, wl.source.centers, wl.source.nq)
     s.setup_target(wl.y, wl.nu)
-    params = sd.SolverParameters(epsilon=wl.epsilon, tau=wl.tau, use_log_sum_exp_trick=True,
+    # psi: an untruncated FP32 L-BFGS run from 0 (tau = 0 disables truncation)
+    params = sd.SolverParameters(epsilon=wl.epsilon, tau=0.0, use_log_sum_exp_trick=True,
                                  solver_control_type="componentwise", tolerance=1e-6,
                                  max_iterations=args.psi_iters, precision="fp32_lse")
     s.configure(params)
     t1 = time.perf_counter()
     psi = s.solve()
     info = s.get_last_solve_info()
+    nu = wl.nu
     log(f"psi from {info['iterations']} L-BFGS iterations ({info['evaluations']} evaluations, "
         f"{time.perf_counter() - t1:.2f}s, status {info['status']}, "
-        f"max rel marginal err {info['check_value']:.3e} over tol)")
-    R = s.distance_threshold(psi)
+        f"max_j |g_j| - 1e-6 nu_j = {info['check_value']:.3e})")
+    # truncation radius: the smallest R (bisection) whose discarded mass
+    # sum_q w_q (1 - Z_q(R)/Z_q(inf)) stays below 1e-12 at this psi
+    t2 = time.perf_counter()
+    params.tau = wl.tau
+    s.configure(params)
+    lz_full = s.q_log_partition(np.finfo(float).max, psi)
+    lo_r, hi_r = 0.02, float(np.sqrt(3.0))
+    for _ in range(14):
+        mid = 0.5 * (lo_r + hi_r)
+        if s.discarded_mass(mid, psi, wl.source.w, lz_full) < 1e-12:
+            hi_r = mid
+        else:
+            lo_r = mid
+    R = hi_r
+    dm = s.discarded_mass(R, psi, wl.source.w, lz_full)
     _, _, pairs = s.candidate_stats(R)
     n_q = wl.source.x.size // 3
-    log(f"R = {R:.6f}, candidate pairs = {pairs} ({pairs / (n_q * wl.nu.size):.3f} of dense)")
+    log(f"R = {R:.6f} (discarded mass {dm:.2e}, calibration {time.perf_counter() - t2:.1f}s), "
+        f"candidate pairs = {pairs} ({pairs / (n_q * wl.nu.size):.3f} of dense)")
 
-    d_psi = devmem.DeviceArray.from_host(psi)
+    # the timed loop alternates between psi and a nearby potential (exponent
+    # changes of ~0.1 log2 units, the size of a late L-BFGS step) so that the
+    # FP32 sweep pays its realistic reference-value updates
+    psi_b = psi + 0.05 * wl.epsilon * np.random.default_rng(7).standard_normal(psi.size)
+    d_psi = [devmem.DeviceArray.from_host(psi), devmem.DeviceArray.from_host(psi_b)]
     d_grad = devmem.DeviceArray(wl.nu.size)
     d_fc = devmem.DeviceArray(4)
 
@@ -120,14 +141,14 @@ def main():
     for prec in ("fp64", "fp32_lse"):
         params.precision = prec
         s.configure(params)
-        for _ in range(args.warmup):
-            s.eval_device(R, d_psi.address, d_grad.address, d_fc.address)
+        for i in range(args.warmup):
+            s.eval_device(R, d_psi[i % 2].address, d_grad.address, d_fc.address)
         s.synchronize()
         s.profile_enable(True)
         s.synchronize()
         ta = time.perf_counter()
-        for _ in range(args.steps):
-            s.eval_device(R, d_psi.address, d_grad.address, d_fc.address)
+        for i in range(args.steps):
+            s.eval_device(R, d_psi[i % 2].address, d_grad.address, d_fc.address)
         s.synchronize()
         tb = time.perf_counter()
         prof = s.profile_get()
@@ -145,6 +166,7 @@ def main():
     dom = max(kern, key=kern.get)
     achieved = pairs / (kern[dom] * 1e-3)
     peak = pass_peak_pairs(3)
+    eval_frac = (pairs / (r["ms"] * 1e-3)) / (peak / 2.0)
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -164,12 +186,16 @@ def main():
         "dtype": "f64+f32exp",
         "data": "synthetic (seeded Gaussian mixtures)",
         "config": {"workload": wl.name, "n_q": n_q, "n_targets": int(wl.nu.size),
-                   "epsilon": wl.epsilon, "tau": wl.tau, "R": R, "pairs_per_step": int(pairs),
+                   "epsilon": wl.epsilon, "R": R, "discarded_mass": dm,
+                   "psi": f"{info['iterations']} L-BFGS iterations from 0", "pairs_per_step": int(pairs),
                    "precision": "fp32_lse", "parallelism": "single-gpu"},
         "roofline": {"bound": "valu", "kernel": f"k_{dom}", "achieved": achieved / 1e9,
                      "peak": peak / 1e9, "unit": "Gpair/s", "frac": achieved / peak,
                      "traffic": None,
-                     "peak_basis": "1/max(1/R_exp, (d+3)/R_fma), R_fma=3.93e13/s, R_exp=R_fma/4"},
+                     "eval_frac": eval_frac,
+                     "peak_basis": "per pair sweep 1/max(1/R_exp,(d+3)/R_fma) with R_fma = 3.93e13 FMA/s "
+                                   "(published 78.6 TFLOP/s FP64/FP32 vector) and R_exp >= R_fma/4; "
+                                   "an evaluation is two sweeps (eval_frac)"},
         "fp64": {"ms_per_step": results["fp64"]["ms"],
                  "value": pairs / (results["fp64"]["ms"] * 1e-3),
                  "F_rel_diff_vs_fp32": abs(results["fp64"]["F"] - r["F"]) / abs(results["fp64"]["F"])},

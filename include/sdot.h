@@ -159,6 +159,12 @@ int sdot_set_distance_threshold(sdot_ctx *ctx, double R);
  * v ^= v>>29.  Returns the total pair count in *pairs (may be NULL). */
 int sdot_candidate_stats(sdot_ctx *ctx, double R, int64_t *count, uint64_t *hash,
                          int64_t *pairs);
+/* Audit of a truncation radius: for every quadrature point (caller order,
+ * cell-major) the log of its candidate partition sum, in FP64,
+ *   logZ_q = log sum_{j in C(cell(q))} nu_j exp((psi_j - |x_q - y_j|^2/2)/eps) + |x_q'|^2/(2 eps)
+ * (the constant shift is the same for every R, -inf when the sum vanishes).  The mass
+ * a radius R discards at q is 1 - exp(logZ_q(R) - logZ_q(inf)). */
+int sdot_q_log_partition(sdot_ctx *ctx, double R, const double *psi, double *logZ);
 int sdot_get_state(sdot_ctx *ctx, sdot_state *st);
 int sdot_set_state(sdot_ctx *ctx, const sdot_state *st);
 int sdot_profile_enable(sdot_ctx *ctx, int on);

@@ -78,6 +78,7 @@ SIGNATURES = {
     "sdot_distance_threshold": (C.c_int, [C.c_void_p, _dp, _dp]),
     "sdot_set_distance_threshold": (C.c_int, [C.c_void_p, C.c_double]),
     "sdot_candidate_stats": (C.c_int, [C.c_void_p, C.c_double, _i64p, _u64p, _i64p]),
+    "sdot_q_log_partition": (C.c_int, [C.c_void_p, C.c_double, _dp, _dp]),
     "sdot_get_state": (C.c_int, [C.c_void_p, C.POINTER(State)]),
     "sdot_set_state": (C.c_int, [C.c_void_p, C.POINTER(State)]),
     "sdot_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),

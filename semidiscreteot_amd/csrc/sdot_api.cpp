@@ -142,13 +142,16 @@ struct sdot_ctx {
   int64_t last_pairs = -1;
   bool have_source = false, have_target = false;
   int force_mode = -1;  // overrides the arithmetic mode (the Hessian runs FP64 LSE)
+  bool hist_valid = false;  // device holds psi and M_q of a previous FP32 evaluation
   Plan plan;
   Dev dev;
   std::vector<double> h_nu;
+  std::vector<int64_t> h_cell_r2o;  // row-order cell -> caller cell
   double src_lo[3] = {0, 0, 0}, src_hi[3] = {0, 0, 0};
   double tgt_lo[3] = {0, 0, 0}, tgt_hi[3] = {0, 0, 0};
   double coord_scale = 1.0;
   size_t cap_trec = 0, cap_qrec = 0, cap_M = 0, cap_G = 0, cap_F = 0, cap_P = 0, cap_N = 0;
+  size_t cap_Mh = 0;
   // profiling: a pool of event sets resolved at synchronisation points
   bool prof_on = false;
   std::vector<std::array<hipEvent_t, 6>> ev_pool;
@@ -204,11 +207,13 @@ int prepare(sdot_ctx *c, double R) {
   p.smax = smax_for(R);
   p.dense = (R >= diag * (1.0 + 1e-6) + 1e-300) ? 1 : 0;
   p.Rq = R * (1.0 + 1e-9) + 1e-9 * (p.tg.h + p.cg.h + c->coord_scale);
+  p.hist = (p.mode == kF32Lse && c->hist_valid) ? 1 : 0;
   if (!std::isfinite(p.Rq)) p.dense = std::isnan(R) ? 0 : 1;
   if (std::isnan(R)) p.Rq = 0.0;
-  const int64_t target_waves = 8192;
-  int64_t K1 = std::max<int64_t>(1, target_waves / std::max<int64_t>(1, p.n_blocks));
-  int64_t K2 = std::max<int64_t>(1, target_waves / std::max<int64_t>(1, p.n_twaves));
+  // enough waves to fill 256 CUs: the q-major kernel runs 4 waves per SIMD, the
+  // target-major one up to 12
+  int64_t K1 = std::max<int64_t>(1, 8192 / std::max<int64_t>(1, p.n_blocks));
+  int64_t K2 = std::max<int64_t>(1, 16384 / std::max<int64_t>(1, p.n_twaves));
   if (p.dense) {
     K1 = std::min<int64_t>(K1, std::max<int64_t>(1, p.N / 32));
     K2 = std::min<int64_t>(K2, std::max<int64_t>(1, p.n_cells / 8));
@@ -240,18 +245,31 @@ int prepare(sdot_ctx *c, double R) {
   if (c->cap_F < (size_t)p.nFpart) {
     HIP_TRY(dalloc(d.Fpart, p.nFpart));
     HIP_TRY(dalloc(d.Cpart, p.nFpart));
+    HIP_TRY(dalloc(d.Upart, p.nFpart));
     c->cap_F = p.nFpart;
   }
   if (c->cap_P < (size_t)p.nPpart) {
     HIP_TRY(dalloc(d.PNpart, p.nPpart));
+    HIP_TRY(dalloc(d.DMpart, p.nPpart));
+    HIP_TRY(dalloc(d.Dmpart, p.nPpart));
     c->cap_P = p.nPpart;
   }
   if (c->cap_N < (size_t)p.N) {
     HIP_TRY(dalloc(d.psi, p.N));
     HIP_TRY(dalloc(d.grad, p.N));
+    HIP_TRY(dalloc(d.psi_prev, p.N));
     c->cap_N = p.N;
+    c->hist_valid = false;
+    p.hist = 0;
+  }
+  if (c->cap_Mh < (size_t)p.nQ) {
+    HIP_TRY(dalloc(d.Mhist_m, p.nQ));
+    c->cap_Mh = p.nQ;
+    c->hist_valid = false;
+    p.hist = 0;
   }
   if (!d.out) HIP_TRY(dalloc(d.out, 4));
+  if (!d.dstat) HIP_TRY(dalloc(d.dstat, 4));
   return SDOT_OK;
 }
 
@@ -271,6 +289,8 @@ int run_eval(sdot_ctx *c, double R, const double *d_psi, double *d_grad, double 
     pp = &pr;
   }
   HIP_TRY(launch_eval(c->plan, c->dev, d_psi, d_grad, d_FC, c->stream, pp));
+  // an FP32 evaluation leaves psi and M_q behind to seed the next one
+  if (c->plan.mode == kF32Lse) c->hist_valid = true;
   return SDOT_OK;
 }
 
@@ -282,8 +302,23 @@ int host_eval(sdot_ctx *c, double R, const double *psi, double *grad, double *F,
   int rc = prepare(c, R);
   if (rc) return rc;
   HIP_TRY(hipMemcpyAsync(c->dev.psi, psi, sizeof(double) * N, hipMemcpyHostToDevice, c->stream));
+  const bool seeded = c->plan.hist != 0;
   rc = run_eval(c, R, c->dev.psi, c->dev.grad, c->dev.out);
   if (rc) return rc;
+  if (seeded) {
+    // a seed far above the new maxima would have flushed the FP32 terms: redo unseeded
+    double lost = 0.0;
+    HIP_TRY(hipMemcpyAsync(&lost, c->dev.dstat + 2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (lost > 0.0) {
+      c->hist_valid = false;
+      HIP_TRY(hipMemcpyAsync(c->dev.psi, psi, sizeof(double) * N, hipMemcpyHostToDevice, c->stream));
+      rc = prepare(c, R);
+      if (rc) return rc;
+      rc = run_eval(c, R, c->dev.psi, c->dev.grad, c->dev.out);
+      if (rc) return rc;
+    }
+  }
   double fc[2];
   if (grad)
     HIP_TRY(hipMemcpyAsync(grad, c->dev.grad, sizeof(double) * N, hipMemcpyDeviceToHost, c->stream));
@@ -419,7 +454,8 @@ extern "C" int sdot_ctx_destroy(sdot_ctx *c) {
   void *bufs[] = {d.qx_m, d.qw_m, d.qx2_m, d.cc_m, d.blkbox, d.cc_r, d.cstart, d.cell_r2m,
                   d.cell_m2o, d.ty_o, d.tnu_o, d.tpos_r, d.tpos_m, d.tr2o, d.tm2o, d.tstart,
                   d.twbox, d.ty_m, d.trec_r, d.trec_m, d.qrec_r, d.Mpart, d.Spart, d.Gpart,
-                  d.Fpart, d.Cpart, d.PNpart, d.psi, d.grad, d.out, d.scratch, d.i64scratch};
+                  d.Fpart, d.Cpart, d.PNpart, d.psi, d.grad, d.out, d.scratch, d.i64scratch,
+                  d.Upart, d.psi_prev, d.Mhist_m, d.DMpart, d.Dmpart, d.dstat};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto &set : c->ev_pool)
@@ -435,6 +471,7 @@ extern "C" int sdot_set_params(sdot_ctx *c, const sdot_params *p) {
   if (p->precision != SDOT_PREC_FP64 && p->precision != SDOT_PREC_FP32_LSE)
     return fail(SDOT_ERR_INVALID, "unknown precision");
   c->params = *p;
+  c->hist_valid = false;
   return SDOT_OK;
 }
 
@@ -555,7 +592,10 @@ extern "C" int sdot_set_source(sdot_ctx *c, int dim, int64_t n_cells, int nq, co
   HIP_TRY(upload(dv.cstart, cstart));
   HIP_TRY(upload(dv.cell_r2m, r2m));
   HIP_TRY(upload(dv.cell_m2o, m2o));
-  c->cap_M = c->cap_qrec = c->cap_F = 0;
+  c->h_cell_r2o.resize(n_cells);
+  for (int64_t r = 0; r < n_cells; ++r) c->h_cell_r2o[r] = m2o[r2m[r]];
+  c->cap_M = c->cap_qrec = c->cap_F = c->cap_Mh = 0;
+  c->hist_valid = false;
   c->have_source = true;
   return SDOT_OK;
 }
@@ -636,6 +676,7 @@ extern "C" int sdot_set_target(sdot_ctx *c, int64_t n, const double *y, const do
   HIP_TRY(upload(dv.ty_m, tym));
   c->h_nu = nv;
   c->cap_G = c->cap_trec = c->cap_P = c->cap_N = 0;
+  c->hist_valid = false;
   c->have_target = true;
   return SDOT_OK;
 }
@@ -824,6 +865,28 @@ extern "C" int sdot_candidate_stats(sdot_ctx *c, double R, int64_t *count, uint6
   if (hash) std::memcpy(hash, hh.data(), sizeof(uint64_t) * p.n_cells);
   if (pairs) *pairs = tot * p.nq;
   c->last_pairs = tot * p.nq;
+  return SDOT_OK;
+}
+
+extern "C" int sdot_q_log_partition(sdot_ctx *c, double R, const double *psi, double *logZ) {
+  if (ctx_check(c) || !psi || !logZ) return fail(SDOT_ERR_INVALID, "null argument");
+  if (!c->have_source || !c->have_target) return fail(SDOT_ERR_STATE, "measures not set");
+  HIP_TRY(hipSetDevice(c->device));
+  c->force_mode = kF64Lse;
+  int rc = host_eval(c, R, psi, nullptr, nullptr, nullptr);
+  c->force_mode = -1;
+  if (rc) return rc;
+  const Plan &p = c->plan;
+  std::vector<double> rec((size_t)p.nQ * kRec);
+  HIP_TRY(hipMemcpy(rec.data(), c->dev.qrec_r, sizeof(double) * rec.size(), hipMemcpyDeviceToHost));
+  for (int64_t cr = 0; cr < p.n_cells; ++cr) {
+    const int64_t co = c->h_cell_r2o[cr];
+    for (int k = 0; k < p.nq; ++k) {
+      const double *r = &rec[(size_t)(cr * p.nq + k) * kRec];
+      const double M = r[p.dim], scale = r[p.dim + 1], S = r[p.dim + 2];
+      logZ[co * p.nq + k] = scale > 0.0 ? M + std::log(S) : -INFINITY;
+    }
+  }
   return SDOT_OK;
 }
 

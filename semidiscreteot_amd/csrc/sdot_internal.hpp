@@ -78,7 +78,13 @@ struct Dev {
   double *PNpart = nullptr;    // [nPpart]
   double *psi = nullptr;       // [N] staging
   double *grad = nullptr;      // [N] staging
-  double *out = nullptr;       // [4] F, C, ...
+  double *out = nullptr;       // [4] F, C, count of q whose FP32 sum lost range
+  double *Upart = nullptr;     // [nFpart]
+  double *psi_prev = nullptr;  // [N] psi of the previous FP32 evaluation
+  double *Mhist_m = nullptr;   // [nQ] M_q of the previous FP32 evaluation (Morton order)
+  double *DMpart = nullptr;    // [nPpart] block max of the psi change
+  double *Dmpart = nullptr;    // [nPpart] block min of the psi change
+  double *dstat = nullptr;     // [2] global max / min of the psi change
   double *scratch = nullptr;   // misc
   size_t scratch_bytes = 0;
   int64_t *i64scratch = nullptr;
@@ -97,6 +103,7 @@ struct Plan {
   double eps = 1.0;
   double smax = 0.0;   // candidate rule: s <= smax  <=>  sqrt(s) <= R
   double Rq = 0.0;     // conservative enumeration radius
+  int hist = 0;        // FP32 mode: seed the online sweep with the previous evaluation
   int dense = 0;       // every pair is a candidate (no enumeration, no test)
 };
 

@@ -2,19 +2,24 @@
 // evaluation (the work of SotSolver::evaluate_functional / local_assemble,
 // SotSolver.cc:217-377 and 769-869).
 //
-// One evaluation is six short launches on one stream:
-//   k_prep     per target: exponent coefficients a_j = (psi_j - |y'_j|^2/2)/eps,
+// One evaluation is a handful of launches on one stream:
+//   k_prep     per target: exponent coefficients a_j = (psi_j - |y'_j|^2/2)/eps and
 //              y'_j/eps (scaled by log2(e) in the FP32 mode), written in the two
-//              target orders the passes stream; block partials of psi.nu.
-//   k_pass1<0> q-major: per quadrature point the max exponent over its cell's
-//              candidates (LSE mode only).  Lane = group of G quadrature points
-//              of one cell; wave = 64 Morton-consecutive groups; grid.y splits
-//              the candidate rows into K1 segments.
-//   k_pass1<1> q-major: sum_j nu_j exp(t_qj - M_q) per segment.
-//   k_qfin     per q: merge segments in fixed order, F_q, w_q/S_q, C_q, write the
-//              q record the gradient pass streams; block partials of F and C.
-//   k_grad     target-major: lane = target, loops over the cells near its wave
-//              and their q records; per-segment partial sums, no atomics.
+//              target orders the passes stream; block partials of psi.nu and of
+//              the change of psi since the previous evaluation.
+//   k_pass1    q-major sweep over each cell's candidates.  Lane = group of G
+//              quadrature points of one cell; wave = 64 Morton-consecutive groups;
+//              grid.y splits the candidate bin rows into K1 segments.
+//                PASS 0: max exponent (FP64 modes),
+//                PASS 1: sum_j nu_j exp(t_qj - M_q) with the global max (FP64 modes),
+//                PASS 2: one online sweep (FP32 mode): per segment an integer
+//                        reference value M and S = sum nu exp2(t - M); M is raised
+//                        (rarely) to ceil(t) when a term would exceed it, which
+//                        rescales S by an exact power of two.
+//   k_qfin     per q: merge segments in fixed order, F_q, w_q/S_q, C_q, the q record
+//              the gradient pass streams; block partials of F and C.
+//   k_grad     target-major: lane = target, loops over the cells near its wave and
+//              their q records; per-segment partial sums, no atomics.
 //   k_gfin / k_final  fixed-order segment merge, linear term, scalar totals.
 // Every reduction has a fixed order, so results are bitwise reproducible run to
 // run and independent of scheduling.
@@ -46,19 +51,6 @@ __device__ __forceinline__ bool in_set(const double *y, const double *c, double 
     s = s + t * t;
   }
   return s <= smax;
-}
-
-// Squared distance from point p to the box [lo, hi] (0 inside).  Only used
-// as a conservative prefilter in front of the exact candidate test.
-template <int DIM>
-__device__ __forceinline__ double box_dist2(const double *p, const double *lo, const double *hi) {
-  double s = 0.0;
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) {
-    const double v = fmax(0.0, fmax(lo[d] - p[d], p[d] - hi[d]));
-    s += v * v;
-  }
-  return s;
 }
 
 template <int MODE>
@@ -110,7 +102,7 @@ __device__ __forceinline__ void for_rows(const Grid &g, const int32_t *__restric
   }
 }
 
-// Fixed-pattern block sum (blockDim.x == 256). Result valid in thread 0.
+// Fixed-pattern block sum / max (blockDim.x == 256). Result valid in all threads.
 __device__ __forceinline__ double block_sum_256(double v, double *sh) {
   sh[threadIdx.x] = v;
   __syncthreads();
@@ -119,12 +111,41 @@ __device__ __forceinline__ double block_sum_256(double v, double *sh) {
     if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
     __syncthreads();
   }
-  return sh[0];
+  const double r = sh[0];
+  __syncthreads();
+  return r;
 }
+__device__ __forceinline__ double block_max_256(double v, double *sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+#pragma unroll
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// Target record (64 bytes):
+//   [0]=a  [1..DIM]=y'/eps  [1+DIM..2DIM]=y raw  [1+2DIM]=nu
+template <int DIM>
+struct TRec {
+  double a, yl[DIM], y[DIM], nu;
+  __device__ __forceinline__ void load(const double *__restrict__ r) {
+    a = r[0];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      yl[d] = r[1 + d];
+      y[d] = r[1 + DIM + d];
+    }
+    nu = r[1 + 2 * DIM];
+  }
+};
 
 // ---------------------------------------------------------------------------
 // k_prep: target records
-// record layout: [0]=a  [1..DIM]=y'/eps  [1+DIM..2DIM]=y raw  [1+2DIM]=nu
 // ---------------------------------------------------------------------------
 struct PrepArgs {
   const double *y;     // [N*dim] caller order
@@ -133,7 +154,10 @@ struct PrepArgs {
   const int32_t *pos_r, *pos_m;
   double *trec_r, *trec_m;
   double *PNpart;
+  double *psi_prev;    // [N] psi of the previous FP32 evaluation (history mode)
+  double *DMpart, *Dmpart;  // block max / min of psi - psi_prev
   int64_t N;
+  int hist;
   double o[3];
   double coef;         // inv_eps * log-base scale
 };
@@ -142,7 +166,7 @@ template <int DIM>
 __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
   __shared__ double sh[256];
   const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  double pn = 0.0;
+  double pn = 0.0, dmx = -INFINITY, dmn = INFINITY;
   if (j < a.N) {
     double yr[DIM], yp[DIM];
     double y2 = 0.0;
@@ -171,13 +195,46 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
       r2[k] = rec[k];
     }
     pn = psi * nu;
+    if (a.psi_prev) {
+      if (a.hist) {
+        const double dd = psi - a.psi_prev[j];
+        dmx = dd;
+        dmn = dd;
+      }
+      a.psi_prev[j] = psi;
+    }
   }
   const double s = block_sum_256(pn, sh);
   if (threadIdx.x == 0) a.PNpart[blockIdx.x] = s;
+  if (a.psi_prev) {
+    const double m1 = block_max_256(dmx, sh);
+    const double m2 = -block_max_256(-dmn, sh);
+    if (threadIdx.x == 0) {
+      a.DMpart[blockIdx.x] = m1;
+      a.Dmpart[blockIdx.x] = m2;
+    }
+  }
+}
+
+// Global max / min of the psi change (history mode): dstat[0] = max, [1] = min.
+__global__ __launch_bounds__(256) void k_dred(const double *DMpart, const double *Dmpart,
+                                              int64_t n, double *dstat) {
+  __shared__ double sh[256];
+  double mx = -INFINITY, mn = INFINITY;
+  for (int64_t i = threadIdx.x; i < n; i += 256) {
+    mx = fmax(mx, DMpart[i]);
+    mn = fmin(mn, Dmpart[i]);
+  }
+  const double a = block_max_256(mx, sh);
+  const double b = -block_max_256(-mn, sh);
+  if (threadIdx.x == 0) {
+    dstat[0] = a;
+    dstat[1] = b;
+  }
 }
 
 // ---------------------------------------------------------------------------
-// k_pass1: q-major LSE passes
+// k_pass1: q-major sweeps
 // ---------------------------------------------------------------------------
 struct P1Args {
   const double *qx;      // [nQ*DIM] Morton order, shifted
@@ -186,12 +243,15 @@ struct P1Args {
   const double *blkbox;  // [n_blocks*2*DIM]
   const double *trec;    // [N*kRec] row order
   const int32_t *tstart;
-  const double *Mpart;   // [K1*nQ]  read by PASS 1 in LSE modes
-  double *out;           // [K1*nQ]
+  const double *Mpart;   // [K1*nQ]  read by PASS 1
+  const double *Mhist;   // [nQ]     PASS 2: M of the previous evaluation
+  const double *dstat;   // PASS 2: max / min psi change since then
+  double *out;           // [K1*nQ]  M (PASS 0/2) or S (PASS 1)
+  double *out2;          // [K1*nQ]  S (PASS 2)
   Grid tg;
   int64_t n_groups, nQ, N;
-  int nq, gpc, K1, dense;
-  double smax, Rq, plain_coef;
+  int nq, gpc, K1, dense, hist;
+  double smax, Rq, plain_coef, coef;
 };
 
 template <int DIM, int G, int MODE, int PASS>
@@ -207,6 +267,16 @@ __global__ __launch_bounds__(64) void k_pass1(P1Args a) {
   double c[DIM], x[G][DIM], M[G], S[G];
 #pragma unroll
   for (int d = 0; d < DIM; ++d) c[d] = a.cc[cell * DIM + d];
+  // history seed of the online sweep: the previous evaluation's M_q raised by
+  // the largest psi increase bounds every new exponent of the targets that were
+  // candidates before; skipped when the psi change spread could underflow
+  bool seed = false;
+  double shift = 0.0;
+  if (PASS == 2 && a.hist) {
+    const double dmx = a.dstat[0], dmn = a.dstat[1];
+    seed = (dmx - dmn) * a.coef < 100.0;
+    shift = dmx * a.coef;
+  }
 #pragma unroll
   for (int k = 0; k < G; ++k) {
 #pragma unroll
@@ -214,6 +284,9 @@ __global__ __launch_bounds__(64) void k_pass1(P1Args a) {
     S[k] = 0.0;
     if (PASS == 0) {
       M[k] = -INFINITY;
+    } else if (PASS == 2) {
+      M[k] = seed ? ceil(a.Mhist[q0 + k] + shift) : -INFINITY;
+      if (!(M[k] < INFINITY)) M[k] = -INFINITY;
     } else if (MODE == kF64Plain) {
       M[k] = a.plain_coef * a.qx2[q0 + k];
     } else {
@@ -228,41 +301,52 @@ __global__ __launch_bounds__(64) void k_pass1(P1Args a) {
     lo[d] = a.blkbox[(int64_t)blockIdx.x * 2 * DIM + d];
     hi[d] = a.blkbox[(int64_t)blockIdx.x * 2 * DIM + DIM + d];
   }
-
-  const double Rq2 = a.Rq * a.Rq;
+  auto one = [&](const TRec<DIM> &r) {
+    if (!a.dense && !in_set<DIM>(r.y, c, a.smax)) return;
+    if (PASS == 0) {
+#pragma unroll
+      for (int k = 0; k < G; ++k) {
+        double t = r.a;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) t = fma(x[k][d], r.yl[d], t);
+        M[k] = fmax(M[k], t);
+      }
+    } else if (PASS == 1) {
+#pragma unroll
+      for (int k = 0; k < G; ++k) {
+        double t = r.a;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) t = fma(x[k][d], r.yl[d], t);
+        S[k] = fma(r.nu, expo<MODE>(t - M[k]), S[k]);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < G; ++k) {
+        double t = r.a;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) t = fma(x[k][d], r.yl[d], t);
+        double dlt = t - M[k];
+        if (dlt > 0.0) {
+          // rare: raise the reference value to the next integer, so the
+          // rescale of the running sum is an exact power of two
+          const double Mn = ceil(t);
+          S[k] = (M[k] > -INFINITY) ? ldexp(S[k], (int)(M[k] - Mn)) : 0.0;
+          M[k] = Mn;
+          dlt = t - Mn;
+        }
+        S[k] = fma(r.nu, (double)__builtin_amdgcn_exp2f((float)dlt), S[k]);
+      }
+    }
+  };
   auto body = [&](int j0, int j1) {
-    for (int j = j0; j < j1; ++j) {
-      const double *__restrict__ r = a.trec + (int64_t)j * kRec;
-      if (!a.dense) {
-        double yr[DIM];
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) yr[d] = r[1 + DIM + d];
-        // uniform: skip targets out of reach of every cell of the wave
-        if (box_dist2<DIM>(yr, lo, hi) > Rq2) continue;
-        if (!in_set<DIM>(yr, c, a.smax)) continue;
-      }
-      const double ta = r[0];
-      double yl[DIM];
-#pragma unroll
-      for (int d = 0; d < DIM; ++d) yl[d] = r[1 + d];
-      if (PASS == 0) {
-#pragma unroll
-        for (int k = 0; k < G; ++k) {
-          double t = ta;
-#pragma unroll
-          for (int d = 0; d < DIM; ++d) t = fma(x[k][d], yl[d], t);
-          M[k] = fmax(M[k], t);
-        }
-      } else {
-        const double nu = r[1 + 2 * DIM];
-#pragma unroll
-        for (int k = 0; k < G; ++k) {
-          double t = ta;
-#pragma unroll
-          for (int d = 0; d < DIM; ++d) t = fma(x[k][d], yl[d], t);
-          S[k] = fma(nu, expo<MODE>(t - M[k]), S[k]);
-        }
-      }
+    // records are fetched two at a time ahead of their use
+    for (int j = j0; j < j1; j += 2) {
+      const int j2 = (j + 1 < j1) ? j + 1 : j;
+      TRec<DIM> r0, r1;
+      r0.load(a.trec + (int64_t)j * kRec);
+      r1.load(a.trec + (int64_t)j2 * kRec);
+      one(r0);
+      if (j + 1 < j1) one(r1);
     }
   };
 
@@ -274,8 +358,15 @@ __global__ __launch_bounds__(64) void k_pass1(P1Args a) {
   }
   if (active) {
 #pragma unroll
-    for (int k = 0; k < G; ++k)
-      a.out[(int64_t)seg * a.nQ + q0 + k] = (PASS == 0) ? M[k] : S[k];
+    for (int k = 0; k < G; ++k) {
+      const int64_t o = (int64_t)seg * a.nQ + q0 + k;
+      if (PASS == 0) a.out[o] = M[k];
+      if (PASS == 1) a.out[o] = S[k];
+      if (PASS == 2) {
+        a.out[o] = M[k];
+        a.out2[o] = S[k];
+      }
+    }
   }
 }
 
@@ -285,7 +376,7 @@ __global__ __launch_bounds__(64) void k_pass1(P1Args a) {
 struct QFArgs {
   const double *qx, *qw, *qx2, *Mpart, *Spart;
   const int64_t *cell_r2m;
-  double *qrec_r, *Fpart, *Cpart;
+  double *qrec_r, *Fpart, *Cpart, *Upart, *Mhist;
   int64_t nQ;
   int nq, K1;
   double eps, plain_coef, ln_scale;
@@ -295,7 +386,7 @@ template <int DIM, int MODE>
 __global__ __launch_bounds__(256) void k_qfin(QFArgs a) {
   __shared__ double sh[256];
   const int64_t qr = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  double Fq = 0.0, Cq = 0.0;
+  double Fq = 0.0, Cq = 0.0, Uq = 0.0;
   if (qr < a.nQ) {
     const int64_t cr = qr / a.nq, k = qr - cr * a.nq;
     const int64_t qm = a.cell_r2m[cr] * a.nq + k;
@@ -303,15 +394,28 @@ __global__ __launch_bounds__(256) void k_qfin(QFArgs a) {
 #pragma unroll
     for (int d = 0; d < DIM; ++d) x[d] = a.qx[qm * DIM + d];
     const double x2 = a.qx2[qm];
-    double M;
+    double M, S = 0.0;
     if (MODE == kF64Plain) {
       M = a.plain_coef * x2;
-    } else {
+      for (int s = 0; s < a.K1; ++s) S += a.Spart[(int64_t)s * a.nQ + qm];
+    } else if (MODE == kF64Lse) {
       M = -INFINITY;
       for (int s = 0; s < a.K1; ++s) M = fmax(M, a.Mpart[(int64_t)s * a.nQ + qm]);
+      for (int s = 0; s < a.K1; ++s) S += a.Spart[(int64_t)s * a.nQ + qm];
+    } else {
+      // segments carry their own reference values: S = sum S_s 2^(M_s - M)
+      M = -INFINITY;
+      for (int s = 0; s < a.K1; ++s) M = fmax(M, a.Mpart[(int64_t)s * a.nQ + qm]);
+      if (M > -INFINITY)
+        for (int s = 0; s < a.K1; ++s) {
+          const double ss = a.Spart[(int64_t)s * a.nQ + qm];
+          if (ss > 0.0) S += ldexp(ss, (int)(a.Mpart[(int64_t)s * a.nQ + qm] - M));
+        }
+      a.Mhist[qm] = M;
+      // every term far below the reference value: the FP32 exponentials lost
+      // their range; the host repeats the evaluation without the seed
+      if (M > -INFINITY && S < 1e-40) Uq = 1.0;
     }
-    double S = 0.0;
-    for (int s = 0; s < a.K1; ++s) S += a.Spart[(int64_t)s * a.nQ + qm];
     const double w = a.qw[qm];
     double Mrec = INFINITY, scale = 0.0;
     // q-points whose sum vanished are skipped (SotSolver.cc:854)
@@ -331,11 +435,13 @@ __global__ __launch_bounds__(256) void k_qfin(QFArgs a) {
     for (int t = DIM + 3; t < kRec; ++t) r[t] = 0.0;
   }
   const double fs = block_sum_256(Fq, sh);
-  __syncthreads();
-  if (threadIdx.x == 0) a.Fpart[blockIdx.x] = fs;
-  __syncthreads();
   const double cs = block_sum_256(Cq, sh);
-  if (threadIdx.x == 0) a.Cpart[blockIdx.x] = cs;
+  const double us = block_sum_256(Uq, sh);
+  if (threadIdx.x == 0) {
+    a.Fpart[blockIdx.x] = fs;
+    a.Cpart[blockIdx.x] = cs;
+    a.Upart[blockIdx.x] = us;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -359,23 +465,40 @@ __global__ __launch_bounds__(64) void k_grad(GArgs a) {
   const int seg = blockIdx.y;
   const int64_t j = (int64_t)blockIdx.x * kWave + threadIdx.x;
   const bool active = j < a.N;
-  const double *r = a.trec_m + (active ? j : 0) * kRec;
-  const double ta = r[0];
-  double yl[DIM], yr[DIM];
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) {
-    yl[d] = r[1 + d];
-    yr[d] = r[1 + DIM + d];
-  }
+  TRec<DIM> me;
+  me.load(a.trec_m + (active ? j : 0) * kRec);
   double lo[DIM], hi[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
     lo[d] = a.twbox[(int64_t)blockIdx.x * 2 * DIM + d];
     hi[d] = a.twbox[(int64_t)blockIdx.x * 2 * DIM + DIM + d];
   }
-  const double Rq2 = a.Rq * a.Rq;
   double acc = 0.0;
   const int nq = a.nq;
+  auto chunk = [&](const double *__restrict__ qp) {
+    double qx[G][DIM], qm[G], qs[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) qx[k][d] = qp[k * kRec + d];
+      qm[k] = qp[k * kRec + DIM];
+      qs[k] = qp[k * kRec + DIM + 1];
+    }
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+      double t = me.a;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) t = fma(qx[k][d], me.yl[d], t);
+      const double e = expo<MODE>(t - qm[k]);
+      if (MODE == kF64Plain) {
+        // the reference adds only exp_terms > 0 (SotSolver.cc:292); in plain
+        // mode w/S_q can overflow to inf when S_q is subnormal
+        if (e > 0.0) acc = fma(qs[k], e, acc);
+      } else {
+        acc = fma(qs[k], e, acc);
+      }
+    }
+  };
   auto body = [&](int c0, int c1) {
     for (int c = c0; c < c1; ++c) {
       if (!a.dense) {
@@ -383,36 +506,10 @@ __global__ __launch_bounds__(64) void k_grad(GArgs a) {
         double cv[DIM];
 #pragma unroll
         for (int d = 0; d < DIM; ++d) cv[d] = cp[d];
-        // uniform: skip cells out of reach of every target of the wave
-        if (box_dist2<DIM>(cv, lo, hi) > Rq2) continue;
-        if (!in_set<DIM>(yr, cv, a.smax)) continue;
+        if (!in_set<DIM>(me.y, cv, a.smax)) continue;
       }
-      const double *__restrict__ qp = a.qrec_r + (int64_t)c * nq * kRec;
-      for (int k0 = 0; k0 < nq; k0 += G, qp += G * kRec) {
-        // the chunk's records are read together, then consumed
-        double qx[G][DIM], qm[G], qs[G];
-#pragma unroll
-        for (int k = 0; k < G; ++k) {
-#pragma unroll
-          for (int d = 0; d < DIM; ++d) qx[k][d] = qp[k * kRec + d];
-          qm[k] = qp[k * kRec + DIM];
-          qs[k] = qp[k * kRec + DIM + 1];
-        }
-#pragma unroll
-        for (int k = 0; k < G; ++k) {
-          double t = ta;
-#pragma unroll
-          for (int d = 0; d < DIM; ++d) t = fma(qx[k][d], yl[d], t);
-          const double e = expo<MODE>(t - qm[k]);
-          if (MODE == kF64Plain) {
-            // the reference adds only exp_terms > 0 (SotSolver.cc:292); in plain
-            // mode w/S_q can overflow to inf when S_q is subnormal
-            if (e > 0.0) acc = fma(qs[k], e, acc);
-          } else {
-            acc = fma(qs[k], e, acc);
-          }
-        }
-      }
+      const double *qp = a.qrec_r + (int64_t)c * nq * kRec;
+      for (int k0 = 0; k0 < nq; k0 += G) chunk(qp + (int64_t)k0 * kRec);
     }
   };
   if (a.dense) {
@@ -439,25 +536,28 @@ __global__ __launch_bounds__(256) void k_gfin(const double *Gpart, const double 
   grad[tm2o[j]] = nu * s - nu;
 }
 
-// Fixed-order totals: out[0] = sum F_q - sum psi.nu, out[1] = sum C_q.
-__global__ __launch_bounds__(256) void k_final(const double *Fpart, int64_t nF,
-                                               const double *Cpart, const double *PNpart,
-                                               int64_t nP, double *out) {
+// Fixed-order totals: out[0] = sum F_q - sum psi.nu, out[1] = sum C_q,
+// *ucount = number of q-points whose FP32 sum lost its range.
+__global__ __launch_bounds__(256) void k_final(const double *Fpart, const double *Cpart,
+                                               const double *Upart, int64_t nF,
+                                               const double *PNpart, int64_t nP, double *out,
+                                               double *ucount) {
   __shared__ double sh[256];
-  double f = 0.0, c = 0.0, p = 0.0;
+  double f = 0.0, c = 0.0, u = 0.0, p = 0.0;
   for (int64_t i = threadIdx.x; i < nF; i += 256) {
     f += Fpart[i];
     c += Cpart[i];
+    u += Upart[i];
   }
   for (int64_t i = threadIdx.x; i < nP; i += 256) p += PNpart[i];
   const double fs = block_sum_256(f, sh);
-  __syncthreads();
   const double cs = block_sum_256(c, sh);
-  __syncthreads();
+  const double us = block_sum_256(u, sh);
   const double ps = block_sum_256(p, sh);
   if (threadIdx.x == 0) {
     out[0] = fs - ps;
     out[1] = cs;
+    *ucount = us;
   }
 }
 
@@ -696,6 +796,22 @@ static hipError_t launch_pass1_g(int G, const P1Args &a, int64_t nb, int K1, hip
 }
 
 template <int DIM, int MODE>
+static void launch_grad_g(int G, const GArgs &g, int64_t ntw, int K2, hipStream_t s) {
+  const dim3 gg((unsigned)ntw, (unsigned)K2);
+  switch (G) {
+    case 1: hipLaunchKernelGGL((k_grad<DIM, MODE, 1>), gg, dim3(64), 0, s, g); break;
+    case 2: hipLaunchKernelGGL((k_grad<DIM, MODE, 2>), gg, dim3(64), 0, s, g); break;
+    case 3: hipLaunchKernelGGL((k_grad<DIM, MODE, 3>), gg, dim3(64), 0, s, g); break;
+    case 4: hipLaunchKernelGGL((k_grad<DIM, MODE, 4>), gg, dim3(64), 0, s, g); break;
+    case 5: hipLaunchKernelGGL((k_grad<DIM, MODE, 5>), gg, dim3(64), 0, s, g); break;
+    case 6: hipLaunchKernelGGL((k_grad<DIM, MODE, 6>), gg, dim3(64), 0, s, g); break;
+    case 7: hipLaunchKernelGGL((k_grad<DIM, MODE, 7>), gg, dim3(64), 0, s, g); break;
+    case 8: hipLaunchKernelGGL((k_grad<DIM, MODE, 8>), gg, dim3(64), 0, s, g); break;
+    default: hipLaunchKernelGGL((k_grad<DIM, MODE, 9>), gg, dim3(64), 0, s, g); break;
+  }
+}
+
+template <int DIM, int MODE>
 static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_grad,
                          double *d_FC, hipStream_t s, Profile *prof) {
   const double ln_scale = (MODE == kF32Lse) ? 0.69314718055994530942 : 1.0;
@@ -712,10 +828,16 @@ static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_g
   pa.trec_r = d.trec_r;
   pa.trec_m = d.trec_m;
   pa.PNpart = d.PNpart;
+  pa.psi_prev = (MODE == kF32Lse) ? d.psi_prev : nullptr;
+  pa.DMpart = d.DMpart;
+  pa.Dmpart = d.Dmpart;
   pa.N = p.N;
+  pa.hist = p.hist;
   for (int k = 0; k < 3; ++k) pa.o[k] = p.origin[k];
   pa.coef = coef;
   hipLaunchKernelGGL((k_prep<DIM>), dim3((unsigned)p.nPpart), dim3(256), 0, s, pa);
+  if (MODE == kF32Lse && p.hist)
+    hipLaunchKernelGGL(k_dred, dim3(1), dim3(256), 0, s, d.DMpart, d.Dmpart, p.nPpart, d.dstat);
 
   P1Args a{};
   a.qx = d.qx_m;
@@ -725,6 +847,8 @@ static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_g
   a.trec = d.trec_r;
   a.tstart = d.tstart;
   a.Mpart = d.Mpart;
+  a.Mhist = d.Mhist_m;
+  a.dstat = d.dstat;
   a.tg = p.tg;
   a.n_groups = p.n_groups;
   a.nQ = p.nQ;
@@ -733,18 +857,26 @@ static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_g
   a.gpc = p.gpc;
   a.K1 = p.K1;
   a.dense = p.dense;
+  a.hist = p.hist;
   a.smax = p.smax;
   a.Rq = p.Rq;
   a.plain_coef = 0.5 / p.eps;
+  a.coef = coef;
   hipError_t e;
-  if (MODE != kF64Plain) {
+  if (MODE == kF64Lse) {
     a.out = d.Mpart;
     e = launch_pass1_g<DIM, MODE, 0>(p.G, a, p.n_blocks, p.K1, s);
     if (e != hipSuccess) return e;
   }
   if (timing) (void)hipEventRecord(prof->ev[1], s);
-  a.out = d.Spart;
-  e = launch_pass1_g<DIM, MODE, 1>(p.G, a, p.n_blocks, p.K1, s);
+  if (MODE == kF32Lse) {
+    a.out = d.Mpart;
+    a.out2 = d.Spart;
+    e = launch_pass1_g<DIM, MODE, 2>(p.G, a, p.n_blocks, p.K1, s);
+  } else {
+    a.out = d.Spart;
+    e = launch_pass1_g<DIM, MODE, 1>(p.G, a, p.n_blocks, p.K1, s);
+  }
   if (e != hipSuccess) return e;
   if (timing) (void)hipEventRecord(prof->ev[2], s);
 
@@ -758,6 +890,8 @@ static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_g
   q.qrec_r = d.qrec_r;
   q.Fpart = d.Fpart;
   q.Cpart = d.Cpart;
+  q.Upart = d.Upart;
+  q.Mhist = d.Mhist_m;
   q.nQ = p.nQ;
   q.nq = p.nq;
   q.K1 = p.K1;
@@ -782,25 +916,12 @@ static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_g
   g.dense = p.dense;
   g.smax = p.smax;
   g.Rq = p.Rq;
-  {
-    const dim3 gg((unsigned)p.n_twaves, (unsigned)p.K2);
-    switch (p.G) {
-      case 1: hipLaunchKernelGGL((k_grad<DIM, MODE, 1>), gg, dim3(64), 0, s, g); break;
-      case 2: hipLaunchKernelGGL((k_grad<DIM, MODE, 2>), gg, dim3(64), 0, s, g); break;
-      case 3: hipLaunchKernelGGL((k_grad<DIM, MODE, 3>), gg, dim3(64), 0, s, g); break;
-      case 4: hipLaunchKernelGGL((k_grad<DIM, MODE, 4>), gg, dim3(64), 0, s, g); break;
-      case 5: hipLaunchKernelGGL((k_grad<DIM, MODE, 5>), gg, dim3(64), 0, s, g); break;
-      case 6: hipLaunchKernelGGL((k_grad<DIM, MODE, 6>), gg, dim3(64), 0, s, g); break;
-      case 7: hipLaunchKernelGGL((k_grad<DIM, MODE, 7>), gg, dim3(64), 0, s, g); break;
-      case 8: hipLaunchKernelGGL((k_grad<DIM, MODE, 8>), gg, dim3(64), 0, s, g); break;
-      default: hipLaunchKernelGGL((k_grad<DIM, MODE, 9>), gg, dim3(64), 0, s, g); break;
-    }
-  }
+  launch_grad_g<DIM, MODE>(p.G, g, p.n_twaves, p.K2, s);
   if (timing) (void)hipEventRecord(prof->ev[4], s);
   hipLaunchKernelGGL((k_gfin<DIM>), dim3((unsigned)((p.N + 255) / 256)), dim3(256), 0, s,
                      d.Gpart, d.trec_m, d.tm2o, d_grad, p.N, p.K2);
-  hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, s, d.Fpart, p.nFpart, d.Cpart, d.PNpart,
-                     p.nPpart, d_FC);
+  hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, s, d.Fpart, d.Cpart, d.Upart, p.nFpart,
+                     d.PNpart, p.nPpart, d_FC, d.dstat + 2);
   if (timing) (void)hipEventRecord(prof->ev[5], s);
   return hipGetLastError();
 }

@@ -239,6 +239,19 @@ class SotSolver:
                                               C.byref(pairs)))
         return cnt, h, pairs.value
 
+    def q_log_partition(self, R, potential):
+        out = np.zeros(self.n_cells * self.nq)
+        L.check(self.lib.sdot_q_log_partition(self._h, float(R), _p(_f64(potential)), _p(out)))
+        return out
+
+    def discarded_mass(self, R, potential, w, logZ_full=None):
+        """Mass the truncation radius R drops: sum_q w_q (1 - Z_q(R)/Z_q(inf))."""
+        if logZ_full is None:
+            logZ_full = self.q_log_partition(np.finfo(float).max, potential)
+        lz = self.q_log_partition(R, potential)
+        frac = np.where(np.isfinite(lz), -np.expm1(np.minimum(lz - logZ_full, 0.0)), 1.0)
+        return float(np.sum(np.asarray(w) * frac))
+
     # ---- device-resident evaluation and profiling ----
     def eval_device(self, R, d_psi: int, d_grad: int, d_FC: int):
         L.check(self.lib.sdot_eval_device(self._h, float(R), C.c_void_p(d_psi),
