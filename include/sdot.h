@@ -170,6 +170,76 @@ int sdot_set_state(sdot_ctx *ctx, const sdot_state *st);
 int sdot_profile_enable(sdot_ctx *ctx, int on);
 int sdot_profile_get(sdot_ctx *ctx, sdot_profile *out);
 
+/* ---- batched serving ------------------------------------------------------
+ * No counterpart in the reference, whose SotSolver::solve (SotSolver.cc:105-215)
+ * runs one solve at a time; each solve here follows that function's algorithm
+ * (L-BFGS m = 5 with the same line search and stopping tests).
+ *
+ * A server borrows the source measure of a context (the resident mesh; that
+ * context's sdot_set_source and sdot_ctx_destroy fail while the server
+ * exists) and advances many solves against it together: every step runs one
+ * evaluation of each active solve through work-list kernels (replayed as one
+ * captured hipGraph), with each solve's potential, gradient, L-BFGS history
+ * and partial sums resident in HBM.  Solves are admitted in submission order
+ * while a slot is free and their buffers fit the HBM budget.  A solve's
+ * arithmetic does not depend on which other solves share its steps: its
+ * result is bitwise the same alone and in any mix.
+ * submit / poll may be called from any thread; step from one thread at a time. */
+typedef struct sdot_server sdot_server;
+
+typedef struct {
+  int32_t precision;         /* sdot_precision of the pair kernels (default FP32_LSE) */
+  int32_t max_slots;         /* solves advanced together (default 256, at most 4096) */
+  int64_t hbm_budget_bytes;  /* device arena for per-solve buffers (default 16 GiB) */
+  int32_t use_graph;         /* 1: replay one captured hipGraph per step (default 1) */
+  int32_t pass_targets;      /* targets per q-major work item (0: tuned or default) */
+  int32_t grad_qpoints;      /* q-points per target-major work item (0: tuned or default) */
+} sdot_server_config;
+
+typedef struct {
+  int64_t n;                   /* number of targets (>= 1) */
+  const double *y;             /* [n*dim] target points */
+  const double *nu;            /* [n] target weights (> 0) */
+  const double *psi0;          /* [n] initial potential, or NULL for zeros */
+  double epsilon;              /* regularisation (> 0) */
+  double tolerance;            /* stopping tolerance (SotSolver.cc:156-162) */
+  int32_t solver_control_type; /* sdot_control_type */
+  uint32_t max_iterations;
+  double radius;               /* truncation radius R of the candidate sets; <= 0 or
+                                  DBL_MAX: every target is a candidate */
+} sdot_request;
+
+typedef struct {
+  int64_t id;
+  int32_t status;              /* sdot_solve_status */
+  int32_t iterations, evaluations;
+  double functional;           /* F at the final potential */
+  double check_value;          /* last stopping-test value */
+  double rel_marginal_error;   /* max_j |g_j| / nu_j at the final potential */
+  double t_submit, t_admit, t_done;  /* seconds on the server clock */
+} sdot_result;
+
+int sdot_server_config_default(sdot_server_config *cfg);
+int sdot_server_create(sdot_ctx *source, const sdot_server_config *cfg, sdot_server **out);
+int sdot_server_destroy(sdot_server *srv);
+/* Copies the request's arrays and builds its target layout; *id >= 0. */
+int sdot_server_submit(sdot_server *srv, const sdot_request *req, int64_t *id);
+/* Admits what fits, runs one batched step, advances every active solve's
+ * line search on the host.  Outputs (may be NULL): solves active after the
+ * step, requests still queued. */
+int sdot_server_step(sdot_server *srv, int32_t *n_active, int32_t *n_queued);
+/* *done = 1 when solve `id` has finished: fills *res and, if psi is not NULL,
+ * the final potential (n doubles), then forgets the solve. */
+int sdot_server_poll(sdot_server *srv, int64_t id, int32_t *done, sdot_result *res, double *psi);
+/* Seconds since the server was created (the clock of sdot_result). */
+int sdot_server_clock(sdot_server *srv, double *t);
+/* Device bytes the arena would hold for a request of n targets. */
+int sdot_server_request_bytes(sdot_server *srv, int64_t n, double radius, int64_t *bytes);
+/* Times candidate work-item sizes on synthetic target sets of the server's
+ * source and keeps the fastest (pass_targets, grad_qpoints); call before the
+ * first submit.  Outputs may be NULL. */
+int sdot_server_autotune(sdot_server *srv, int32_t *pass_targets, int32_t *grad_qpoints);
+
 /* ---- source-measure extraction (the FEValues stand-in) --------------------- */
 /* Structured Q1 box mesh (GridGenerator::subdivided_hyper_rectangle), QGauss
  * of `qorder` points per direction, Q1-interpolated nodal density (vertices

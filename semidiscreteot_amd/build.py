@@ -17,8 +17,8 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libsdot.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
-HIP_SOURCES = ["sdot_kernels.hip"]
-CXX_SOURCES = ["sdot_api.cpp", "sdot_mesh.cpp", "sdot_prm.cpp"]
+HIP_SOURCES = ["sdot_kernels.hip", "sdot_serve_kernels.hip"]
+CXX_SOURCES = ["sdot_api.cpp", "sdot_mesh.cpp", "sdot_prm.cpp", "sdot_serve.cpp"]
 
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

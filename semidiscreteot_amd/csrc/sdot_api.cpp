@@ -110,9 +110,14 @@ int bin_linear(const Grid &g, const double *p, int dim) {
   return (b[2] * g.n[1] + b[1]) * g.n[0] + b[0];
 }
 
+constexpr int kMaxSeg = 128;
+constexpr int kProfSets = 128;
+
+}  // namespace
+
 // Largest double s with sqrt(s) <= R (IEEE sqrt is correctly rounded and
 // monotone), so that `s <= smax` is the reference's `sqrt(s) <= R`.
-double smax_for(double R) {
+double sdot::smax_for(double R) {
   if (std::isnan(R) || R < 0) return -1.0;
   if (std::isinf(R)) return INFINITY;
   double s = R * R;
@@ -126,10 +131,72 @@ double smax_for(double R) {
   return s;
 }
 
-constexpr int kMaxSeg = 128;
-constexpr int kProfSets = 128;
+int sdot::set_error(int code, const std::string &msg) { return fail(code, msg); }
 
-}  // namespace
+// Orders of the target points (Morton for the gradient waves, row-major bins
+// for the q-major sweeps), the bin grid and the per-wave bounding boxes.
+void sdot::build_target_layout(int dim, int64_t n, const double *y, TargetLayout &L) {
+  L.n = n;
+  L.n_twaves = (n + kWave - 1) / kWave;
+  double lo[3], hi[3];
+  for (int d = 0; d < dim; ++d) {
+    lo[d] = INFINITY;
+    hi[d] = -INFINITY;
+  }
+  for (int64_t j = 0; j < n; ++j)
+    for (int d = 0; d < dim; ++d) {
+      lo[d] = std::min(lo[d], y[j * dim + d]);
+      hi[d] = std::max(hi[d], y[j * dim + d]);
+    }
+  L.scale = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    L.lo[d] = d < dim ? lo[d] : 0.0;
+    L.hi[d] = d < dim ? hi[d] : 0.0;
+    if (d < dim) L.scale = std::max({L.scale, std::fabs(lo[d]), std::fabs(hi[d])});
+  }
+  // Morton order (gradient waves)
+  std::vector<uint64_t> key(n);
+  for (int64_t j = 0; j < n; ++j) key[j] = morton_key(y + j * dim, lo, hi, dim);
+  std::vector<int32_t> &m2o = L.m2o, &pos_m = L.pos_m;
+  m2o.resize(n);
+  pos_m.resize(n);
+  std::iota(m2o.begin(), m2o.end(), 0);
+  std::stable_sort(m2o.begin(), m2o.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+  for (int64_t m = 0; m < n; ++m) pos_m[m2o[m]] = (int32_t)m;
+  std::vector<double> &box = L.twbox, &tym = L.tym;
+  box.assign((size_t)L.n_twaves * 2 * dim, 0.0);
+  tym.assign((size_t)n * 4, 0.0);
+  for (int64_t w = 0; w < L.n_twaves; ++w) {
+    double bl[3] = {INFINITY, INFINITY, INFINITY}, bh[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t m = w * kWave; m < std::min<int64_t>(n, (w + 1) * kWave); ++m)
+      for (int d = 0; d < dim; ++d) {
+        bl[d] = std::min(bl[d], y[(int64_t)m2o[m] * dim + d]);
+        bh[d] = std::max(bh[d], y[(int64_t)m2o[m] * dim + d]);
+      }
+    for (int d = 0; d < dim; ++d) {
+      box[w * 2 * dim + d] = bl[d];
+      box[w * 2 * dim + dim + d] = bh[d];
+    }
+  }
+  for (int64_t m = 0; m < n; ++m)
+    for (int d = 0; d < dim; ++d) tym[m * 4 + d] = y[(int64_t)m2o[m] * dim + d];
+  // row (bin) order (q-major passes stream contiguous bin runs)
+  L.tg = make_grid(lo, hi, dim, n, 8.0);
+  const int64_t ntb = (int64_t)L.tg.n[0] * L.tg.n[1] * L.tg.n[2];
+  std::vector<int> tb(n);
+  for (int64_t j = 0; j < n; ++j) tb[j] = bin_linear(L.tg, y + j * dim, dim);
+  std::vector<int32_t> &r2o = L.r2o, &pos_r = L.pos_r;
+  r2o.resize(n);
+  pos_r.resize(n);
+  std::iota(r2o.begin(), r2o.end(), 0);
+  std::stable_sort(r2o.begin(), r2o.end(), [&](int32_t a, int32_t b) { return tb[a] < tb[b]; });
+  for (int64_t r = 0; r < n; ++r) pos_r[r2o[r]] = (int32_t)r;
+  std::vector<int32_t> &tstart = L.tstart;
+  tstart.assign(ntb + 1, 0);
+  for (int64_t j = 0; j < n; ++j) tstart[tb[j] + 1]++;
+  for (int64_t b = 0; b < ntb; ++b) tstart[b + 1] += tstart[b];
+
+}
 
 struct sdot_ctx {
   int device = 0;
@@ -143,6 +210,7 @@ struct sdot_ctx {
   bool have_source = false, have_target = false;
   int force_mode = -1;  // overrides the arithmetic mode (the Hessian runs FP64 LSE)
   bool hist_valid = false;  // device holds psi and M_q of a previous FP32 evaluation
+  int n_servers = 0;        // serving queues reading this context's source arrays
   Plan plan;
   Dev dev;
   std::vector<double> h_nu;
@@ -448,6 +516,8 @@ extern "C" int sdot_ctx_create(int device, sdot_ctx **out) {
 
 extern "C" int sdot_ctx_destroy(sdot_ctx *c) {
   if (!c) return SDOT_OK;
+  if (c->n_servers > 0)
+    return fail(SDOT_ERR_STATE, "a serving queue reads this context's source: destroy it first");
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   Dev &d = c->dev;
@@ -482,6 +552,8 @@ extern "C" int sdot_set_source(sdot_ctx *c, int dim, int64_t n_cells, int nq, co
   if (n_cells < 1 || n_cells >= (1ll << 31) - 1) return fail(SDOT_ERR_INVALID, "n_cells out of range");
   if (nq < 1 || nq > 4096) return fail(SDOT_ERR_INVALID, "nq out of range");
   if (!x || !w || !centers) return fail(SDOT_ERR_INVALID, "null source array");
+  if (c->n_servers > 0)
+    return fail(SDOT_ERR_STATE, "a serving queue reads this context's source: destroy it first");
   if (c->have_target && dim != c->plan.dim)
     return fail(SDOT_ERR_INVALID, "source dim differs from the target dim");
   const int64_t nQ = n_cells * nq;
@@ -609,60 +681,19 @@ extern "C" int sdot_set_target(sdot_ctx *c, int64_t n, const double *y, const do
   if (!all_finite(y, n * dim) || !all_finite(nu, n)) return fail(SDOT_ERR_INVALID, "non-finite target values");
   HIP_TRY(hipSetDevice(c->device));
   Plan &p = c->plan;
+  TargetLayout L;
+  build_target_layout(dim, n, y, L);
   p.N = n;
-  p.n_twaves = (n + kWave - 1) / kWave;
-  double lo[3], hi[3];
-  for (int d = 0; d < dim; ++d) {
-    lo[d] = INFINITY;
-    hi[d] = -INFINITY;
-  }
-  for (int64_t j = 0; j < n; ++j)
-    for (int d = 0; d < dim; ++d) {
-      lo[d] = std::min(lo[d], y[j * dim + d]);
-      hi[d] = std::max(hi[d], y[j * dim + d]);
-    }
-  double scale = 0.0;
+  p.n_twaves = L.n_twaves;
+  p.tg = L.tg;
   for (int d = 0; d < 3; ++d) {
-    c->tgt_lo[d] = d < dim ? lo[d] : 0.0;
-    c->tgt_hi[d] = d < dim ? hi[d] : 0.0;
-    if (d < dim) scale = std::max({scale, std::fabs(lo[d]), std::fabs(hi[d])});
+    c->tgt_lo[d] = L.lo[d];
+    c->tgt_hi[d] = L.hi[d];
   }
-  c->coord_scale = std::max(c->coord_scale, scale);
-  // Morton order (gradient waves)
-  std::vector<uint64_t> key(n);
-  for (int64_t j = 0; j < n; ++j) key[j] = morton_key(y + j * dim, lo, hi, dim);
-  std::vector<int32_t> m2o(n), pos_m(n);
-  std::iota(m2o.begin(), m2o.end(), 0);
-  std::stable_sort(m2o.begin(), m2o.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
-  for (int64_t m = 0; m < n; ++m) pos_m[m2o[m]] = (int32_t)m;
-  std::vector<double> box((size_t)p.n_twaves * 2 * dim), tym((size_t)n * 4, 0.0);
-  for (int64_t w = 0; w < p.n_twaves; ++w) {
-    double bl[3] = {INFINITY, INFINITY, INFINITY}, bh[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int64_t m = w * kWave; m < std::min<int64_t>(n, (w + 1) * kWave); ++m)
-      for (int d = 0; d < dim; ++d) {
-        bl[d] = std::min(bl[d], y[(int64_t)m2o[m] * dim + d]);
-        bh[d] = std::max(bh[d], y[(int64_t)m2o[m] * dim + d]);
-      }
-    for (int d = 0; d < dim; ++d) {
-      box[w * 2 * dim + d] = bl[d];
-      box[w * 2 * dim + dim + d] = bh[d];
-    }
-  }
-  for (int64_t m = 0; m < n; ++m)
-    for (int d = 0; d < dim; ++d) tym[m * 4 + d] = y[(int64_t)m2o[m] * dim + d];
-  // row (bin) order (q-major passes stream contiguous bin runs)
-  p.tg = make_grid(lo, hi, dim, n, 8.0);
-  const int64_t ntb = (int64_t)p.tg.n[0] * p.tg.n[1] * p.tg.n[2];
-  std::vector<int> tb(n);
-  for (int64_t j = 0; j < n; ++j) tb[j] = bin_linear(p.tg, y + j * dim, dim);
-  std::vector<int32_t> r2o(n), pos_r(n);
-  std::iota(r2o.begin(), r2o.end(), 0);
-  std::stable_sort(r2o.begin(), r2o.end(), [&](int32_t a, int32_t b) { return tb[a] < tb[b]; });
-  for (int64_t r = 0; r < n; ++r) pos_r[r2o[r]] = (int32_t)r;
-  std::vector<int32_t> tstart(ntb + 1, 0);
-  for (int64_t j = 0; j < n; ++j) tstart[tb[j] + 1]++;
-  for (int64_t b = 0; b < ntb; ++b) tstart[b + 1] += tstart[b];
-
+  c->coord_scale = std::max(c->coord_scale, L.scale);
+  const std::vector<int32_t> &pos_r = L.pos_r, &pos_m = L.pos_m, &r2o = L.r2o, &m2o = L.m2o,
+                             &tstart = L.tstart;
+  const std::vector<double> &box = L.twbox, &tym = L.tym;
   Dev &dv = c->dev;
   std::vector<double> yv(y, y + n * dim), nv(nu, nu + n);
   HIP_TRY(upload(dv.ty_o, yv));
@@ -935,3 +966,40 @@ extern "C" int sdot_profile_get(sdot_ctx *c, sdot_profile *out) {
   *out = c->prof_acc;
   return SDOT_OK;
 }
+
+// ---------------------------------------------------------------------------
+// source arrays lent to the serving queue (sdot_serve.cpp)
+// ---------------------------------------------------------------------------
+int sdot::ctx_source_view(sdot_ctx *c, SourceView &v) {
+  if (!c) return fail(SDOT_ERR_INVALID, "null context");
+  if (!c->have_source) return fail(SDOT_ERR_STATE, "the context has no source measure");
+  const Plan &p = c->plan;
+  const Dev &d = c->dev;
+  v.device = c->device;
+  v.dim = p.dim;
+  v.nq = p.nq;
+  v.G = p.G;
+  v.gpc = p.gpc;
+  v.n_cells = p.n_cells;
+  v.nQ = p.nQ;
+  v.n_groups = p.n_groups;
+  v.n_blocks = p.n_blocks;
+  v.cg = p.cg;
+  for (int k = 0; k < 3; ++k) {
+    v.origin[k] = p.origin[k];
+    v.lo[k] = c->src_lo[k];
+    v.hi[k] = c->src_hi[k];
+  }
+  v.coord_scale = c->coord_scale;
+  v.qx_m = d.qx_m;
+  v.qw_m = d.qw_m;
+  v.qx2_m = d.qx2_m;
+  v.cc_m = d.cc_m;
+  v.blkbox = d.blkbox;
+  v.cc_r = d.cc_r;
+  v.cstart = d.cstart;
+  v.cell_r2m = d.cell_r2m;
+  return SDOT_OK;
+}
+
+void sdot::ctx_attach(sdot_ctx *c, int delta) { c->n_servers += delta; }

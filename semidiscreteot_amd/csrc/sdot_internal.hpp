@@ -107,11 +107,113 @@ struct Plan {
   int dense = 0;       // every pair is a candidate (no enumeration, no test)
 };
 
+
+// Argument blocks of the evaluation passes (one per solve; shared by the
+// single-solve grids and the serving work lists).
+struct PrepArgs {
+  const double *y;     // [N*dim] caller order
+  const double *nu;    // [N]
+  const double *psi;   // [N]
+  const int32_t *pos_r, *pos_m;
+  double *trec_r, *trec_m;
+  double *PNpart;
+  double *psi_prev;    // [N] psi of the previous FP32 evaluation (history mode)
+  double *DMpart, *Dmpart;  // block max / min of psi - psi_prev
+  int64_t N;
+  int hist;
+  double o[3];
+  double coef;         // inv_eps * log-base scale
+};
+
+struct P1Args {
+  const double *qx;      // [nQ*DIM] Morton order, shifted
+  const double *qx2;     // [nQ]
+  const double *cc;      // [n_cells*DIM] raw centers, Morton order
+  const double *blkbox;  // [n_blocks*2*DIM]
+  const double *trec;    // [N*kRec] row order
+  const int32_t *tstart;
+  const double *Mpart;   // [K1*nQ]  read by PASS 1
+  const double *Mhist;   // [nQ]     PASS 2: M of the previous evaluation
+  const double *dstat;   // PASS 2: max / min psi change since then
+  double *out;           // [K1*nQ]  M (PASS 0/2) or S (PASS 1)
+  double *out2;          // [K1*nQ]  S (PASS 2)
+  Grid tg;
+  int64_t n_groups, nQ, N;
+  int nq, gpc, K1, dense, hist;
+  double smax, Rq, plain_coef, coef;
+};
+
+struct QFArgs {
+  const double *qx, *qw, *qx2, *Mpart, *Spart;
+  const int64_t *cell_r2m;
+  double *qrec_r, *Fpart, *Cpart, *Upart, *Mhist;
+  int64_t nQ;
+  int nq, K1;
+  double eps, plain_coef, ln_scale;
+};
+
+struct GArgs {
+  const double *trec_m;  // [N*kRec] Morton order
+  const double *twbox;   // [n_twaves*2*DIM]
+  const double *cc_r;    // [n_cells*4] raw centers, row order
+  const int32_t *cstart;
+  const double *qrec_r;  // [nQ*kRec]
+  double *Gpart;         // [K2*N]
+  Grid cg;
+  int64_t N, n_cells;
+  int nq, K2, dense;
+  double smax, Rq;
+};
+
+struct GFArgs {
+  const double *Gpart, *trec_m;
+  const int32_t *tm2o;
+  double *grad;
+  int64_t N;
+  int K2;
+};
+
+struct FinArgs {
+  const double *Fpart, *Cpart, *Upart, *PNpart;
+  int64_t nF, nP;
+  double *out, *ucount;
+};
+
 struct Profile {
   bool on = false;
   hipEvent_t ev[6] = {};
   sdot_profile acc{};
 };
+
+// Host-side layout of a target set (sdot_api.cpp).
+struct TargetLayout {
+  Grid tg{};
+  int64_t n = 0, n_twaves = 0;
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}, scale = 0.0;
+  std::vector<int32_t> m2o, pos_m, r2o, pos_r, tstart;
+  std::vector<double> twbox, tym;
+};
+void build_target_layout(int dim, int64_t n, const double *y, TargetLayout &L);
+
+// Largest double s with sqrt(s) <= R (candidate test s <= smax).
+double smax_for(double R);
+// Sets the thread's sdot_last_error message; returns code.
+int set_error(int code, const std::string &msg);
+
+// Read-only view of a context's source arrays (serving queues borrow them).
+struct SourceView {
+  int device = 0, dim = 0, nq = 0, G = 1, gpc = 1;
+  int64_t n_cells = 0, nQ = 0, n_groups = 0, n_blocks = 0;
+  Grid cg{};
+  double origin[3] = {0, 0, 0}, lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}, coord_scale = 1.0;
+  const double *qx_m = nullptr, *qw_m = nullptr, *qx2_m = nullptr, *cc_m = nullptr;
+  const double *blkbox = nullptr, *cc_r = nullptr;
+  const int32_t *cstart = nullptr;
+  const int64_t *cell_r2m = nullptr;
+};
+int ctx_source_view(sdot_ctx *c, SourceView &v);
+// Count of serving queues attached to the context (its source stays fixed meanwhile).
+void ctx_attach(sdot_ctx *c, int delta);
 
 // Kernel launchers (sdot_kernels.hip).
 hipError_t launch_eval(const Plan &p, Dev &d, const double *d_psi, double *d_grad,

@@ -1,0 +1,937 @@
+// sdot_serve.cpp -- continuous-batching solve queue (include/sdot.h, "batched
+// serving").  Host side of the serving path:
+//   * admission: requests wait in submission order and are admitted while a
+//     slot is free and their buffers fit the device arena (one hipMalloc at
+//     creation, first-fit sub-allocation, so admission and retirement never
+//     call hipMalloc / hipFree on the hot path);
+//   * per solve: argument blocks of every evaluation pass (the same blocks the
+//     single-solve launcher builds), the potential, gradient and L-BFGS
+//     history in the arena, and the line-search state machine on the host
+//     (LineSearchSM, the decisions of SotSolver::solve's deal.II line search,
+//     SotSolver.cc:190-198);
+//   * one step: upload the batch's actions, replay the captured step graph
+//     (sdot_serve_kernels.hip), read back F, slopes and stopping statistics,
+//     advance every solve's state machine; finished solves are retired at once
+//     and their slots refilled at the next step.
+// A solve's kernel arguments and segment counts depend only on its own
+// request, so its arithmetic is the same alone and in any batch.
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+#include "sdot_bfgs.hpp"
+#include "sdot_serve.hpp"
+
+using namespace sdot;
+
+#define SV_TRY(expr)                                                                      \
+  do {                                                                                    \
+    hipError_t _e = (expr);                                                               \
+    if (_e != hipSuccess)                                                                 \
+      return set_error(SDOT_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));  \
+  } while (0)
+
+namespace {
+
+constexpr size_t kAlign = 256;
+constexpr int kMaxSlots = 4096;
+constexpr int kMaxSegServe = 128;
+constexpr int kDefaultPassTargets = 2048;
+constexpr int kDefaultGradQ = 16384;
+
+size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
+
+// First-fit allocator over one device allocation (offsets, 256-byte aligned).
+class Arena {
+ public:
+  hipError_t init(size_t bytes) {
+    size_ = align_up(bytes);
+    hipError_t e = hipMalloc(&base_, size_);
+    if (e == hipSuccess) free_[0] = size_;
+    return e;
+  }
+  void release() {
+    if (base_) (void)hipFree(base_);
+    base_ = nullptr;
+    free_.clear();
+  }
+  size_t alloc(size_t bytes) {
+    bytes = align_up(bytes);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < bytes) continue;
+      const size_t off = it->first, rem = it->second - bytes;
+      free_.erase(it);
+      if (rem) free_[off + bytes] = rem;
+      used_ += bytes;
+      return off;
+    }
+    return SIZE_MAX;
+  }
+  void release_block(size_t off, size_t bytes) {
+    bytes = align_up(bytes);
+    used_ -= bytes;
+    auto it = free_.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+  char *at(size_t off) const { return static_cast<char *>(base_) + off; }
+  size_t size() const { return size_; }
+  size_t used() const { return used_; }
+
+ private:
+  void *base_ = nullptr;
+  size_t size_ = 0, used_ = 0;
+  std::map<size_t, size_t> free_;
+};
+
+// Byte offsets of one solve's buffers inside its arena block.
+struct Carve {
+  size_t off = 0;
+  template <class T>
+  size_t take(size_t n) {
+    const size_t o = off;
+    off = align_up(off + std::max<size_t>(n, 1) * sizeof(T));
+    return o;
+  }
+};
+
+struct SolveShape {
+  int64_t N = 0, ntb = 0, n_twaves = 0, nPpart = 0;
+  int K1 = 1, K2 = 1;
+  bool fp32 = false;
+};
+
+struct SolveOffsets {
+  size_t y, nu, pos_r, pos_m, tm2o, tstart, twbox;
+  size_t trec_r, trec_m, qrec_r, Mpart, Spart, Gpart;
+  size_t Fpart, Cpart, Upart, PNpart, DMpart, Dmpart, dstat, psi_prev, Mhist;
+  size_t x, g, x0, gv, p, S[kHist], Y[kHist], hist;
+  size_t total;
+};
+
+SolveOffsets carve(const SourceView &src, const SolveShape &s) {
+  Carve c;
+  SolveOffsets o{};
+  const int64_t nFpart = (src.nQ + 255) / 256;
+  o.y = c.take<double>((size_t)s.N * src.dim);
+  o.nu = c.take<double>(s.N);
+  o.pos_r = c.take<int32_t>(s.N);
+  o.pos_m = c.take<int32_t>(s.N);
+  o.tm2o = c.take<int32_t>(s.N);
+  o.tstart = c.take<int32_t>(s.ntb + 1);
+  o.twbox = c.take<double>((size_t)s.n_twaves * 2 * src.dim);
+  o.trec_r = c.take<double>((size_t)s.N * kRec);
+  o.trec_m = c.take<double>((size_t)s.N * kRec);
+  o.qrec_r = c.take<double>((size_t)src.nQ * kRec);
+  o.Mpart = c.take<double>((size_t)s.K1 * src.nQ);
+  o.Spart = c.take<double>((size_t)s.K1 * src.nQ);
+  o.Gpart = c.take<double>((size_t)s.K2 * s.N);
+  o.Fpart = c.take<double>(nFpart);
+  o.Cpart = c.take<double>(nFpart);
+  o.Upart = c.take<double>(nFpart);
+  o.PNpart = c.take<double>(s.nPpart);
+  o.DMpart = c.take<double>(s.nPpart);
+  o.Dmpart = c.take<double>(s.nPpart);
+  o.dstat = c.take<double>(4);
+  o.psi_prev = s.fp32 ? c.take<double>(s.N) : SIZE_MAX;
+  o.Mhist = s.fp32 ? c.take<double>(src.nQ) : SIZE_MAX;
+  o.x = c.take<double>(s.N);
+  o.g = c.take<double>(s.N);
+  o.x0 = c.take<double>(s.N);
+  o.gv = c.take<double>(s.N);
+  o.p = c.take<double>(s.N);
+  for (int k = 0; k < kHist; ++k) o.S[k] = c.take<double>(s.N);
+  for (int k = 0; k < kHist; ++k) o.Y[k] = c.take<double>(s.N);
+  o.hist = c.take<SlotHist>(1);
+  o.total = c.off;
+  return o;
+}
+
+enum Phase { kStart = 0, kNewDir = 1, kTrial = 2 };
+
+struct Request {
+  int64_t id = 0;
+  int64_t N = 0;
+  double eps = 1.0, tol = 0.0, R = INFINITY;
+  int control = SDOT_CTRL_L1NORM;
+  uint32_t max_it = 0;
+  std::vector<double> y, nu, psi0;
+  TargetLayout L;
+  SolveShape shape;
+  int dense = 0;
+  double smax = 0.0, Rq = 0.0;
+  size_t bytes = 0;
+  double t_submit = 0.0;
+};
+
+struct Solve {
+  std::unique_ptr<Request> req;
+  size_t off = SIZE_MAX;
+  SolveOffsets o{};
+  SlotDesc desc{};
+  SlotVecs vecs{};
+  int64_t items[5] = {0, 0, 0, 0, 0};
+  // host side of the solve (lbfgs_solve's loop, unrolled into steps)
+  Phase phase = kStart;
+  LineSearchSM ls;
+  double f = 0.0, f_prev = 0.0, check = 0.0, relerr = 0.0;
+  bool first = true;
+  bool seeded = false;    // FP32: the next evaluation starts from the history seed
+  bool redo = false;      // repeat the evaluation unseeded
+  bool dir_ready = false; // a redone direction step: g0 / a1 already known
+  double g0 = 0.0, a1 = 0.0;
+  int k = 0, evals = 0;
+  double t_admit = 0.0;
+};
+
+struct Done {
+  sdot_result res{};
+  std::vector<double> psi;
+};
+
+}  // namespace
+
+struct sdot_server {
+  sdot_ctx *ctx = nullptr;
+  SourceView src;
+  int device = 0;
+  int mode = kF32Lse;
+  int max_slots = 256;
+  int use_graph = 1;
+  int pass_targets = kDefaultPassTargets, grad_qpoints = kDefaultGradQ;
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  float last_step_ms = 0.0f;
+  Arena arena;
+  // batch block: device copy and pinned host staging, same layout
+  //   [desc][vecs][base 5 x bstride][in][nact]
+  size_t off_desc = 0, off_vecs = 0, off_base = 0, off_in = 0, off_nact = 0, block_bytes = 0;
+  char *d_block = nullptr, *h_block = nullptr;
+  SlotOut *d_out = nullptr, *h_out = nullptr;
+  int64_t bstride = 0;
+  ServeDev sd{};
+  std::vector<std::unique_ptr<Solve>> active;
+  bool dirty = true;
+  // shared with submit / poll
+  std::mutex mu;
+  std::deque<std::unique_ptr<Request>> queue;
+  std::unordered_map<int64_t, Done> done;
+  int64_t next_id = 0;
+  std::chrono::steady_clock::time_point t0;
+
+  double now() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+};
+
+namespace {
+
+int fill_shape(sdot_server *s, Request &r) {
+  const SourceView &src = s->src;
+  SolveShape &sh = r.shape;
+  sh.N = r.N;
+  sh.ntb = (int64_t)r.L.tg.n[0] * r.L.tg.n[1] * r.L.tg.n[2];
+  sh.n_twaves = r.L.n_twaves;
+  sh.nPpart = (r.N + 255) / 256;
+  sh.fp32 = s->mode == kF32Lse;
+  // candidate rule and enumeration radius, as for a single solve
+  double diag2 = 0.0;
+  for (int d = 0; d < src.dim; ++d) {
+    const double lo = std::min(src.lo[d], r.L.lo[d]);
+    const double hi = std::max(src.hi[d], r.L.hi[d]);
+    diag2 += (hi - lo) * (hi - lo);
+  }
+  const double diag = std::sqrt(diag2);
+  const double R = r.R;
+  r.smax = smax_for(R);
+  r.dense = (R >= diag * (1.0 + 1e-6) + 1e-300) ? 1 : 0;
+  r.Rq = R * (1.0 + 1e-9) + 1e-9 * (r.L.tg.h + src.cg.h + std::max(src.coord_scale, r.L.scale));
+  if (!std::isfinite(r.Rq)) r.dense = 1;
+  // segment counts from the solve's own size: work items of about
+  // pass_targets targets (q-major) and grad_qpoints q-points (target-major)
+  const int64_t k1 = (r.N + s->pass_targets - 1) / s->pass_targets;
+  const int64_t k2 = (src.nQ + s->grad_qpoints - 1) / s->grad_qpoints;
+  sh.K1 = (int)std::clamp<int64_t>(k1, 1, kMaxSegServe);
+  sh.K2 = (int)std::clamp<int64_t>(k2, 1, kMaxSegServe);
+  r.bytes = carve(src, sh).total;
+  return SDOT_OK;
+}
+
+template <class T>
+T *dptr(sdot_server *s, const Solve &v, size_t rel) {
+  return rel == SIZE_MAX ? nullptr : reinterpret_cast<T *>(s->arena.at(v.off + rel));
+}
+
+// Argument blocks of the solve's evaluation (as the single-solve launcher
+// builds them), pointing into its arena block.
+void build_desc(sdot_server *s, Solve &v) {
+  const SourceView &src = s->src;
+  const Request &r = *v.req;
+  const SolveShape &sh = r.shape;
+  const bool fp32 = s->mode == kF32Lse;
+  const double coef = (1.0 / r.eps) * (fp32 ? 1.44269504088896340736 : 1.0);
+  const double ln_scale = fp32 ? 0.69314718055994530942 : 1.0;
+  double *trec_r = dptr<double>(s, v, v.o.trec_r), *trec_m = dptr<double>(s, v, v.o.trec_m);
+  double *qrec_r = dptr<double>(s, v, v.o.qrec_r);
+  double *Mpart = dptr<double>(s, v, v.o.Mpart), *Spart = dptr<double>(s, v, v.o.Spart);
+  double *Gpart = dptr<double>(s, v, v.o.Gpart);
+  double *Fpart = dptr<double>(s, v, v.o.Fpart), *Cpart = dptr<double>(s, v, v.o.Cpart);
+  double *Upart = dptr<double>(s, v, v.o.Upart), *PNpart = dptr<double>(s, v, v.o.PNpart);
+  double *dstat = dptr<double>(s, v, v.o.dstat), *Mhist = dptr<double>(s, v, v.o.Mhist);
+  double *x = dptr<double>(s, v, v.o.x), *g = dptr<double>(s, v, v.o.g);
+  const int64_t nFpart = (src.nQ + 255) / 256;
+
+  PrepArgs &pa = v.desc.prep;
+  pa = PrepArgs{};
+  pa.y = dptr<double>(s, v, v.o.y);
+  pa.nu = dptr<double>(s, v, v.o.nu);
+  pa.psi = x;
+  pa.pos_r = dptr<int32_t>(s, v, v.o.pos_r);
+  pa.pos_m = dptr<int32_t>(s, v, v.o.pos_m);
+  pa.trec_r = trec_r;
+  pa.trec_m = trec_m;
+  pa.PNpart = PNpart;
+  pa.psi_prev = dptr<double>(s, v, v.o.psi_prev);
+  pa.DMpart = dptr<double>(s, v, v.o.DMpart);
+  pa.Dmpart = dptr<double>(s, v, v.o.Dmpart);
+  pa.N = r.N;
+  pa.hist = v.seeded ? 1 : 0;
+  for (int k = 0; k < 3; ++k) pa.o[k] = src.origin[k];
+  pa.coef = coef;
+
+  P1Args &a = v.desc.p1;
+  a = P1Args{};
+  a.qx = src.qx_m;
+  a.qx2 = src.qx2_m;
+  a.cc = src.cc_m;
+  a.blkbox = src.blkbox;
+  a.trec = trec_r;
+  a.tstart = dptr<int32_t>(s, v, v.o.tstart);
+  a.Mpart = Mpart;
+  a.Mhist = Mhist;
+  a.dstat = dstat;
+  // FP32: PASS 2 writes M to out and S to out2; FP64: PASS 0 writes M to
+  // out2, PASS 1 writes S to out
+  a.out = fp32 ? Mpart : Spart;
+  a.out2 = fp32 ? Spart : Mpart;
+  a.tg = r.L.tg;
+  a.n_groups = src.n_groups;
+  a.nQ = src.nQ;
+  a.N = r.N;
+  a.nq = src.nq;
+  a.gpc = src.gpc;
+  a.K1 = sh.K1;
+  a.dense = r.dense;
+  a.hist = v.seeded ? 1 : 0;
+  a.smax = r.smax;
+  a.Rq = r.Rq;
+  a.plain_coef = 0.5 / r.eps;
+  a.coef = coef;
+
+  QFArgs &q = v.desc.qf;
+  q = QFArgs{};
+  q.qx = src.qx_m;
+  q.qw = src.qw_m;
+  q.qx2 = src.qx2_m;
+  q.Mpart = Mpart;
+  q.Spart = Spart;
+  q.cell_r2m = src.cell_r2m;
+  q.qrec_r = qrec_r;
+  q.Fpart = Fpart;
+  q.Cpart = Cpart;
+  q.Upart = Upart;
+  q.Mhist = Mhist;
+  q.nQ = src.nQ;
+  q.nq = src.nq;
+  q.K1 = sh.K1;
+  q.eps = r.eps;
+  q.plain_coef = 0.5 / r.eps;
+  q.ln_scale = ln_scale;
+
+  GArgs &gr = v.desc.gr;
+  gr = GArgs{};
+  gr.trec_m = trec_m;
+  gr.twbox = dptr<double>(s, v, v.o.twbox);
+  gr.cc_r = src.cc_r;
+  gr.cstart = src.cstart;
+  gr.qrec_r = qrec_r;
+  gr.Gpart = Gpart;
+  gr.cg = src.cg;
+  gr.N = r.N;
+  gr.n_cells = src.n_cells;
+  gr.nq = src.nq;
+  gr.K2 = sh.K2;
+  gr.dense = r.dense;
+  gr.smax = r.smax;
+  gr.Rq = r.Rq;
+
+  v.desc.gf = GFArgs{Gpart, trec_m, dptr<int32_t>(s, v, v.o.tm2o), g, r.N, sh.K2};
+  // out / ucount are set per batch position in upload_batch
+  v.desc.fin = FinArgs{Fpart, Cpart, Upart, PNpart, nFpart, sh.nPpart, nullptr, nullptr};
+  v.desc.nPpart = sh.nPpart;
+
+  SlotVecs &w = v.vecs;
+  w.x = x;
+  w.g = g;
+  w.x0 = dptr<double>(s, v, v.o.x0);
+  w.gv = dptr<double>(s, v, v.o.gv);
+  w.p = dptr<double>(s, v, v.o.p);
+  for (int k = 0; k < kHist; ++k) {
+    w.S[k] = dptr<double>(s, v, v.o.S[k]);
+    w.Y[k] = dptr<double>(s, v, v.o.Y[k]);
+  }
+  w.nu = pa.nu;
+  w.hist = dptr<SlotHist>(s, v, v.o.hist);
+  w.N = r.N;
+
+  v.items[0] = sh.nPpart;
+  v.items[1] = src.n_blocks * sh.K1;
+  v.items[2] = nFpart;
+  v.items[3] = sh.n_twaves * sh.K2;
+  v.items[4] = (r.N + 255) / 256;
+}
+
+// Uploads an admitted request into its arena block (offset `off`).
+int admit(sdot_server *s, std::unique_ptr<Request> r, const SolveOffsets &o, size_t off) {
+  auto v = std::make_unique<Solve>();
+  v->o = o;
+  v->off = off;
+  v->req = std::move(r);
+  v->t_admit = s->now();
+  Request &q = *v->req;
+  const TargetLayout &L = q.L;
+  const int dim = s->src.dim;
+  auto up = [&](size_t rel, const void *h, size_t bytes) {
+    return hipMemcpyAsync(s->arena.at(v->off + rel), h, bytes, hipMemcpyHostToDevice, s->stream);
+  };
+  SV_TRY(up(v->o.y, q.y.data(), sizeof(double) * q.N * dim));
+  SV_TRY(up(v->o.nu, q.nu.data(), sizeof(double) * q.N));
+  SV_TRY(up(v->o.pos_r, L.pos_r.data(), sizeof(int32_t) * q.N));
+  SV_TRY(up(v->o.pos_m, L.pos_m.data(), sizeof(int32_t) * q.N));
+  SV_TRY(up(v->o.tm2o, L.m2o.data(), sizeof(int32_t) * q.N));
+  SV_TRY(up(v->o.tstart, L.tstart.data(), sizeof(int32_t) * L.tstart.size()));
+  SV_TRY(up(v->o.twbox, L.twbox.data(), sizeof(double) * L.twbox.size()));
+  SV_TRY(up(v->o.x, q.psi0.data(), sizeof(double) * q.N));
+  static const SlotHist h0 = [] {
+    SlotHist h{};
+    h.count = 0;
+    h.newest = -1;
+    return h;
+  }();
+  SV_TRY(up(v->o.hist, &h0, sizeof(SlotHist)));
+  build_desc(s, *v);
+  s->active.push_back(std::move(v));
+  s->dirty = true;
+  return SDOT_OK;
+}
+
+// Writes the batch block (staging) and uploads it.
+int upload_batch(sdot_server *s) {
+  const int n = (int)s->active.size();
+  char *h = s->h_block;
+  if (s->dirty) {
+    auto *desc = reinterpret_cast<SlotDesc *>(h + s->off_desc);
+    auto *vecs = reinterpret_cast<SlotVecs *>(h + s->off_vecs);
+    auto *base = reinterpret_cast<int64_t *>(h + s->off_base);
+    for (int k = 0; k < 5; ++k) base[k * s->bstride] = 0;
+    for (int i = 0; i < n; ++i) {
+      Solve &v = *s->active[i];
+      v.desc.prep.hist = v.seeded ? 1 : 0;
+      v.desc.p1.hist = v.seeded ? 1 : 0;
+      v.desc.fin.out = &s->d_out[i].F;
+      v.desc.fin.ucount = &s->d_out[i].lost;
+      desc[i] = v.desc;
+      vecs[i] = v.vecs;
+      for (int k = 0; k < 5; ++k) base[k * s->bstride + i + 1] = base[k * s->bstride + i] + v.items[k];
+    }
+    SV_TRY(hipMemcpyAsync(s->d_block, h, s->off_in, hipMemcpyHostToDevice, s->stream));
+    s->dirty = false;
+  }
+  auto *in = reinterpret_cast<SlotIn *>(h + s->off_in);
+  for (int i = 0; i < n; ++i) {
+    Solve &v = *s->active[i];
+    SlotIn c{};
+    c.action = kActNone;
+    if (!v.redo) {
+      if (v.phase == kNewDir && !v.dir_ready) c.action = kActIter;
+      if (v.phase == kTrial) c.action = kActTrial;
+    }
+    c.first = v.first ? 1 : 0;
+    c.has_prev = v.first ? 0 : 1;
+    c.alpha = v.phase == kTrial ? v.ls.trial() : 0.0;
+    c.f = v.f;
+    c.f_prev = v.f_prev;
+    c.tol = v.req->tol;
+    in[i] = c;
+  }
+  *reinterpret_cast<int *>(h + s->off_nact) = n;
+  SV_TRY(hipMemcpyAsync(s->d_block + s->off_in, h + s->off_in, s->block_bytes - s->off_in,
+                        hipMemcpyHostToDevice, s->stream));
+  return SDOT_OK;
+}
+
+// Stopping test of SotSolver::solve (SotSolver.cc:156-162, SotSolver.h:382-446).
+int check_state(const Solve &v, const SlotOut &o, double &cv) {
+  const Request &r = *v.req;
+  bool ok;
+  if (r.control == SDOT_CTRL_COMPONENTWISE) {
+    cv = o.cw;
+    ok = cv <= DBL_MIN;
+  } else {
+    cv = o.l1;
+    ok = cv <= r.tol;
+  }
+  if (ok) return kSuccess;
+  if ((unsigned)v.k >= r.max_it || std::isnan(cv)) return kFailure;
+  return kIterate;
+}
+
+int retire(sdot_server *s, size_t i, int status) {
+  Solve &v = *s->active[i];
+  const Request &r = *v.req;
+  Done d;
+  d.psi.resize(r.N);
+  SV_TRY(hipMemcpyAsync(d.psi.data(), v.vecs.x, sizeof(double) * r.N, hipMemcpyDeviceToHost,
+                        s->stream));
+  SV_TRY(hipStreamSynchronize(s->stream));
+  d.res.id = r.id;
+  d.res.status = status;
+  d.res.iterations = v.k;
+  d.res.evaluations = v.evals;
+  d.res.functional = v.f;
+  d.res.check_value = v.check;
+  d.res.rel_marginal_error = v.relerr;
+  d.res.t_submit = r.t_submit;
+  d.res.t_admit = v.t_admit;
+  d.res.t_done = s->now();
+  s->arena.release_block(v.off, v.o.total);
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    s->done.emplace(r.id, std::move(d));
+  }
+  s->active.erase(s->active.begin() + i);
+  s->dirty = true;
+  return SDOT_OK;
+}
+
+// One solve's reaction to the evaluation of this step; returns a status >= 0
+// when the solve has finished, -1 otherwise.
+int advance(sdot_server *s, Solve &v, const SlotOut &o) {
+  const bool was_seeded = v.seeded;
+  if (s->mode == kF32Lse) {
+    if (!v.seeded) s->dirty = true;
+    v.seeded = true;
+  }
+  if (was_seeded && o.lost > 0.0) {
+    // the seeded sweep lost range somewhere: evaluate again without the seed
+    if (v.phase == kNewDir && !v.dir_ready) {
+      v.g0 = o.g0;
+      v.a1 = o.a1;
+      v.dir_ready = true;
+    }
+    v.seeded = false;
+    v.redo = true;
+    s->dirty = true;
+    return -1;
+  }
+  v.redo = false;
+  ++v.evals;
+  v.relerr = o.relerr;
+  int r;
+  switch (v.phase) {
+    case kStart: {
+      v.f = o.F;
+      const int st = check_state(v, o, v.check);
+      if (st == kSuccess) return SDOT_SOLVE_CONVERGED;
+      if (st == kFailure) return std::isnan(v.check) ? SDOT_SOLVE_NONFINITE : SDOT_SOLVE_MAX_ITER;
+      v.phase = kNewDir;
+      return -1;
+    }
+    case kNewDir: {
+      const double g0 = v.dir_ready ? v.g0 : o.g0;
+      const double a1 = v.dir_ready ? v.a1 : o.a1;
+      v.dir_ready = false;
+      v.ls.start(v.f, g0, a1);
+      v.f_prev = v.f;
+      r = v.ls.feed(o.F, o.gp);
+      break;
+    }
+    default:
+      r = v.ls.feed(o.F, o.gp);
+      break;
+  }
+  if (r == LineSearchSM::kEval) {
+    v.phase = kTrial;
+    return -1;
+  }
+  if (r == LineSearchSM::kFail) {
+    ++v.k;
+    v.f = o.F;
+    return SDOT_SOLVE_LINESEARCH_FAIL;
+  }
+  // accepted: the point just evaluated is the new iterate
+  v.f = o.F;
+  v.first = false;
+  ++v.k;
+  const int st = check_state(v, o, v.check);
+  if (st == kSuccess) return SDOT_SOLVE_CONVERGED;
+  if (st == kFailure) return std::isnan(v.check) ? SDOT_SOLVE_NONFINITE : SDOT_SOLVE_MAX_ITER;
+  v.phase = kNewDir;
+  return -1;
+}
+
+int launch_step(sdot_server *s) {
+  if (s->exec) {
+    SV_TRY(hipGraphLaunch(s->exec, s->stream));
+  } else {
+    SV_TRY(launch_serve_step(s->src.dim, s->src.G, s->mode, s->sd, s->stream));
+  }
+  return SDOT_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" int sdot_server_config_default(sdot_server_config *cfg) {
+  if (!cfg) return set_error(SDOT_ERR_INVALID, "cfg is null");
+  cfg->precision = SDOT_PREC_FP32_LSE;
+  cfg->max_slots = 256;
+  cfg->hbm_budget_bytes = (int64_t)16 << 30;
+  cfg->use_graph = 1;
+  cfg->pass_targets = 0;
+  cfg->grad_qpoints = 0;
+  return SDOT_OK;
+}
+
+extern "C" int sdot_server_create(sdot_ctx *ctx, const sdot_server_config *cfg_in,
+                                  sdot_server **out) {
+  if (!out) return set_error(SDOT_ERR_INVALID, "out is null");
+  *out = nullptr;
+  sdot_server_config cfg;
+  sdot_server_config_default(&cfg);
+  if (cfg_in) cfg = *cfg_in;
+  if (cfg.precision != SDOT_PREC_FP64 && cfg.precision != SDOT_PREC_FP32_LSE)
+    return set_error(SDOT_ERR_INVALID, "unknown precision");
+  if (cfg.max_slots < 1 || cfg.max_slots > kMaxSlots)
+    return set_error(SDOT_ERR_INVALID, "max_slots must be in [1, 4096]");
+  if (cfg.hbm_budget_bytes < ((int64_t)1 << 20))
+    return set_error(SDOT_ERR_INVALID, "hbm_budget_bytes must be at least 1 MiB");
+  if (cfg.pass_targets < 0 || cfg.grad_qpoints < 0)
+    return set_error(SDOT_ERR_INVALID, "work-item sizes must be >= 0");
+  auto s = std::make_unique<sdot_server>();
+  int rc = ctx_source_view(ctx, s->src);
+  if (rc) return rc;
+  s->ctx = ctx;
+  s->device = s->src.device;
+  s->mode = cfg.precision == SDOT_PREC_FP32_LSE ? kF32Lse : kF64Lse;
+  s->max_slots = cfg.max_slots;
+  s->use_graph = cfg.use_graph ? 1 : 0;
+  if (cfg.pass_targets) s->pass_targets = cfg.pass_targets;
+  if (cfg.grad_qpoints) s->grad_qpoints = cfg.grad_qpoints;
+  s->t0 = std::chrono::steady_clock::now();
+  SV_TRY(hipSetDevice(s->device));
+  SV_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  SV_TRY(hipEventCreate(&s->ev0));
+  SV_TRY(hipEventCreate(&s->ev1));
+  SV_TRY(s->arena.init((size_t)cfg.hbm_budget_bytes));
+  // batch block
+  s->bstride = s->max_slots + 1;
+  Carve c;
+  s->off_desc = c.take<SlotDesc>(s->max_slots);
+  s->off_vecs = c.take<SlotVecs>(s->max_slots);
+  s->off_base = c.take<int64_t>(5 * s->bstride);
+  s->off_in = c.take<SlotIn>(s->max_slots);
+  s->off_nact = c.take<int>(1);
+  s->block_bytes = c.off;
+  SV_TRY(hipMalloc((void **)&s->d_block, s->block_bytes));
+  SV_TRY(hipHostMalloc((void **)&s->h_block, s->block_bytes, hipHostMallocDefault));
+  std::memset(s->h_block, 0, s->block_bytes);
+  SV_TRY(hipMemcpy(s->d_block, s->h_block, s->block_bytes, hipMemcpyHostToDevice));
+  SV_TRY(hipMalloc((void **)&s->d_out, sizeof(SlotOut) * s->max_slots));
+  SV_TRY(hipHostMalloc((void **)&s->h_out, sizeof(SlotOut) * s->max_slots, hipHostMallocDefault));
+  int cus = 256;
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device);
+  cus = std::max(cus, 1);
+  ServeDev &sd = s->sd;
+  sd.desc = reinterpret_cast<SlotDesc *>(s->d_block + s->off_desc);
+  sd.vecs = reinterpret_cast<SlotVecs *>(s->d_block + s->off_vecs);
+  sd.base = reinterpret_cast<int64_t *>(s->d_block + s->off_base);
+  sd.bstride = s->bstride;
+  sd.in = reinterpret_cast<SlotIn *>(s->d_block + s->off_in);
+  sd.nact = reinterpret_cast<int *>(s->d_block + s->off_nact);
+  sd.out = s->d_out;
+  // one-wave kernels run up to 16 waves per CU, the 256-thread ones 8 blocks
+  sd.grid_w = (unsigned)cus * 16;
+  sd.grid_b = (unsigned)cus * 8;
+  sd.grid_slots = (unsigned)s->max_slots;
+  if (s->use_graph) {
+    SV_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t le = launch_serve_step(s->src.dim, s->src.G, s->mode, sd, s->stream);
+    hipGraph_t g = nullptr;
+    hipError_t ce = hipStreamEndCapture(s->stream, &g);
+    if (le != hipSuccess) return set_error(SDOT_ERR_HIP, std::string("step capture: ") + hipGetErrorString(le));
+    if (ce != hipSuccess) return set_error(SDOT_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+    s->graph = g;
+    SV_TRY(hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0));
+  }
+  ctx_attach(ctx, +1);
+  *out = s.release();
+  return SDOT_OK;
+}
+
+extern "C" int sdot_server_destroy(sdot_server *s) {
+  if (!s) return SDOT_OK;
+  (void)hipSetDevice(s->device);
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  if (s->exec) (void)hipGraphExecDestroy(s->exec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  if (s->ev0) (void)hipEventDestroy(s->ev0);
+  if (s->ev1) (void)hipEventDestroy(s->ev1);
+  if (s->d_block) (void)hipFree(s->d_block);
+  if (s->h_block) (void)hipHostFree(s->h_block);
+  if (s->d_out) (void)hipFree(s->d_out);
+  if (s->h_out) (void)hipHostFree(s->h_out);
+  s->arena.release();
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  if (s->ctx) ctx_attach(s->ctx, -1);
+  delete s;
+  return SDOT_OK;
+}
+
+extern "C" int sdot_server_submit(sdot_server *s, const sdot_request *q, int64_t *id) {
+  if (!s || !q) return set_error(SDOT_ERR_INVALID, "null argument");
+  if (q->n < 1 || q->n >= (1ll << 31) - 1) return set_error(SDOT_ERR_INVALID, "n must be >= 1");
+  if (!q->y || !q->nu) return set_error(SDOT_ERR_INVALID, "null target array");
+  if (!(q->epsilon > 0) || !std::isfinite(q->epsilon))
+    return set_error(SDOT_ERR_INVALID, "epsilon must be > 0");
+  if (q->solver_control_type != SDOT_CTRL_L1NORM && q->solver_control_type != SDOT_CTRL_COMPONENTWISE)
+    return set_error(SDOT_ERR_INVALID, "unknown solver_control_type");
+  const int dim = s->src.dim;
+  auto r = std::make_unique<Request>();
+  r->N = q->n;
+  r->eps = q->epsilon;
+  r->tol = q->tolerance;
+  r->control = q->solver_control_type;
+  r->max_it = q->max_iterations;
+  r->R = (q->radius <= 0.0 || q->radius >= DBL_MAX || std::isnan(q->radius)) ? INFINITY : q->radius;
+  r->y.assign(q->y, q->y + q->n * dim);
+  r->nu.assign(q->nu, q->nu + q->n);
+  for (double v : r->y)
+    if (!std::isfinite(v)) return set_error(SDOT_ERR_INVALID, "non-finite target point");
+  for (double v : r->nu)
+    if (!(v > 0) || !std::isfinite(v)) return set_error(SDOT_ERR_INVALID, "target weights must be > 0");
+  if (q->psi0) {
+    r->psi0.assign(q->psi0, q->psi0 + q->n);
+  } else {
+    r->psi0.assign(q->n, 0.0);
+  }
+  build_target_layout(dim, r->N, r->y.data(), r->L);
+  int rc = fill_shape(s, *r);
+  if (rc) return rc;
+  if (r->bytes > s->arena.size())
+    return set_error(SDOT_ERR_NOMEM, "request needs " + std::to_string(r->bytes) +
+                                         " device bytes, more than the server's HBM budget");
+  std::lock_guard<std::mutex> g(s->mu);
+  r->id = s->next_id++;
+  r->t_submit = s->now();
+  if (id) *id = r->id;
+  s->queue.push_back(std::move(r));
+  return SDOT_OK;
+}
+
+extern "C" int sdot_server_step(sdot_server *s, int32_t *n_active, int32_t *n_queued) {
+  if (!s) return set_error(SDOT_ERR_INVALID, "null server");
+  SV_TRY(hipSetDevice(s->device));
+  // admission: submission order, while a slot is free and the arena has room
+  for (;;) {
+    std::unique_ptr<Request> r;
+    {
+      std::lock_guard<std::mutex> g(s->mu);
+      if (s->queue.empty() || (int)s->active.size() >= s->max_slots) break;
+      r = std::move(s->queue.front());
+      s->queue.pop_front();
+    }
+    const SolveOffsets o = carve(s->src, r->shape);
+    const size_t off = s->arena.alloc(o.total);
+    if (off == SIZE_MAX) {
+      std::lock_guard<std::mutex> g(s->mu);
+      s->queue.push_front(std::move(r));
+      break;
+    }
+    int rc = admit(s, std::move(r), o, off);
+    if (rc) return rc;
+  }
+  const int n = (int)s->active.size();
+  if (n > 0) {
+    int rc = upload_batch(s);
+    if (rc) return rc;
+    SV_TRY(hipEventRecord(s->ev0, s->stream));
+    rc = launch_step(s);
+    if (rc) return rc;
+    SV_TRY(hipEventRecord(s->ev1, s->stream));
+    SV_TRY(hipMemcpyAsync(s->h_out, s->d_out, sizeof(SlotOut) * n, hipMemcpyDeviceToHost, s->stream));
+    SV_TRY(hipStreamSynchronize(s->stream));
+    SV_TRY(hipEventElapsedTime(&s->last_step_ms, s->ev0, s->ev1));
+    std::vector<int> status(n);
+    for (int i = 0; i < n; ++i) status[i] = advance(s, *s->active[i], s->h_out[i]);
+    for (int i = n - 1; i >= 0; --i)
+      if (status[i] >= 0) {
+        rc = retire(s, (size_t)i, status[i]);
+        if (rc) return rc;
+      }
+  }
+  if (n_active) *n_active = (int32_t)s->active.size();
+  if (n_queued) {
+    std::lock_guard<std::mutex> g(s->mu);
+    *n_queued = (int32_t)s->queue.size();
+  }
+  return SDOT_OK;
+}
+
+extern "C" int sdot_server_poll(sdot_server *s, int64_t id, int32_t *done, sdot_result *res,
+                                double *psi) {
+  if (!s || !done) return set_error(SDOT_ERR_INVALID, "null argument");
+  std::lock_guard<std::mutex> g(s->mu);
+  auto it = s->done.find(id);
+  if (it == s->done.end()) {
+    *done = 0;
+    if (id < 0 || id >= s->next_id) return set_error(SDOT_ERR_INVALID, "unknown request id");
+    return SDOT_OK;
+  }
+  *done = 1;
+  if (res) *res = it->second.res;
+  if (psi) std::memcpy(psi, it->second.psi.data(), sizeof(double) * it->second.psi.size());
+  s->done.erase(it);
+  return SDOT_OK;
+}
+
+extern "C" int sdot_server_clock(sdot_server *s, double *t) {
+  if (!s || !t) return set_error(SDOT_ERR_INVALID, "null argument");
+  *t = s->now();
+  return SDOT_OK;
+}
+
+extern "C" int sdot_server_request_bytes(sdot_server *s, int64_t n, double radius, int64_t *bytes) {
+  if (!s || !bytes || n < 1) return set_error(SDOT_ERR_INVALID, "bad argument");
+  // sizes only: a unit-cube target layout of n points stands in for the request
+  Request r;
+  r.N = n;
+  r.R = (radius <= 0.0 || radius >= DBL_MAX) ? INFINITY : radius;
+  std::vector<double> y((size_t)n * s->src.dim);
+  for (int64_t j = 0; j < n; ++j)
+    for (int d = 0; d < s->src.dim; ++d)
+      y[j * s->src.dim + d] = s->src.lo[d] + (s->src.hi[d] - s->src.lo[d]) * ((j * (d + 3) + d) % 997) / 997.0;
+  build_target_layout(s->src.dim, n, y.data(), r.L);
+  fill_shape(s, r);
+  *bytes = (int64_t)r.bytes;
+  return SDOT_OK;
+}
+
+extern "C" int sdot_server_autotune(sdot_server *s, int32_t *pass_targets, int32_t *grad_qpoints) {
+  if (!s) return set_error(SDOT_ERR_INVALID, "null server");
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    if (!s->queue.empty() || !s->active.empty() || !s->done.empty())
+      return set_error(SDOT_ERR_STATE, "autotune runs on an idle server (before the first submit)");
+  }
+  // synthetic batch: one solve per size class, targets spread over the source box
+  const int dim = s->src.dim;
+  const int64_t sizes[4] = {1024, 4096, 16384, 65536};
+  std::vector<std::vector<double>> ys(4), nus(4);
+  uint64_t st = 0x2545F4914F6CDD1Dull;
+  auto uni = [&]() {
+    st ^= st << 13;
+    st ^= st >> 7;
+    st ^= st << 17;
+    return (double)(st >> 11) * (1.0 / 9007199254740992.0);
+  };
+  for (int i = 0; i < 4; ++i) {
+    ys[i].resize((size_t)sizes[i] * dim);
+    nus[i].assign(sizes[i], 1.0 / (double)sizes[i]);
+    for (int64_t j = 0; j < sizes[i]; ++j)
+      for (int d = 0; d < dim; ++d)
+        ys[i][j * dim + d] = s->src.lo[d] + (s->src.hi[d] - s->src.lo[d]) * uni();
+  }
+  auto time_config = [&](int pt, int gq, double &ms) -> int {
+    s->pass_targets = pt;
+    s->grad_qpoints = gq;
+    double best = INFINITY;
+    for (int rep = 0; rep < 3; ++rep) {
+      std::vector<int64_t> ids;
+      for (int i = 0; i < 4; ++i) {
+        sdot_request q{};
+        q.n = sizes[i];
+        q.y = ys[i].data();
+        q.nu = nus[i].data();
+        q.epsilon = 1e-2;
+        q.tolerance = 0.0;
+        q.solver_control_type = SDOT_CTRL_L1NORM;
+        q.max_iterations = 0;  // one evaluation, then retired
+        int64_t id = -1;
+        int rc = sdot_server_submit(s, &q, &id);
+        if (rc) return rc;
+        ids.push_back(id);
+      }
+      int32_t na = 0, nq = 0;
+      int rc = sdot_server_step(s, &na, &nq);
+      if (rc) return rc;
+      if (na != 0 || nq != 0) return set_error(SDOT_ERR_NOMEM, "autotune batch did not fit the arena");
+      if (rep > 0) best = std::min(best, (double)s->last_step_ms);
+      for (int64_t id : ids) {
+        int32_t d = 0;
+        rc = sdot_server_poll(s, id, &d, nullptr, nullptr);
+        if (rc) return rc;
+      }
+    }
+    ms = best;
+    return SDOT_OK;
+  };
+  const int pts[4] = {1024, 2048, 4096, 8192};
+  const int gqs[4] = {8192, 16384, 32768, 65536};
+  int best_pt = s->pass_targets, best_gq = s->grad_qpoints;
+  double best = INFINITY, ms = 0.0;
+  for (int pt : pts) {
+    int rc = time_config(pt, best_gq, ms);
+    if (rc) return rc;
+    if (ms < best) {
+      best = ms;
+      best_pt = pt;
+    }
+  }
+  best = INFINITY;
+  for (int gq : gqs) {
+    int rc = time_config(best_pt, gq, ms);
+    if (rc) return rc;
+    if (ms < best) {
+      best = ms;
+      best_gq = gq;
+    }
+  }
+  s->pass_targets = best_pt;
+  s->grad_qpoints = best_gq;
+  if (pass_targets) *pass_targets = best_pt;
+  if (grad_qpoints) *grad_qpoints = best_gq;
+  return SDOT_OK;
+}

@@ -1,0 +1,326 @@
+// sdot_serve_kernels.hip -- batched kernels of the serving path: one launch
+// advances every active solve.  Each kernel walks a device work list (item ->
+// solve by binary search over per-solve item offsets) and runs the same bodies
+// as the single-solve kernels (sdot_device.hpp) on the solve's own argument
+// blocks, so a solve's arithmetic and reduction order do not depend on which
+// other solves share the launch.  The grid is persistent: it is fixed when the
+// launch is recorded and loops over however many items the current batch has,
+// which lets the host replay one captured graph for batches of any size.
+//
+// L-BFGS vector work stays on the device (one 1024-thread workgroup per solve,
+// fixed-order reductions): the trial step x = x0 + alpha p, the history update
+// and two-loop recursion (m = 5, H0 = I; the in-tree text of deal.II's
+// SolverBFGS is core/wasserstein_solver_bfgs.h:402-491), the first trial step
+// of a line search, and the slopes/stopping statistics the host-side line
+// search (sdot_bfgs.hpp LineSearchSM) consumes.
+#include <hip/hip_runtime.h>
+
+#include "sdot_device.hpp"
+#include "sdot_serve.hpp"
+
+namespace sdot {
+
+__device__ __forceinline__ int find_item(const int64_t *__restrict__ base, int n, int64_t item) {
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (base[mid] <= item) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// ---------------------------------------------------------------------------
+// evaluation work lists
+// ---------------------------------------------------------------------------
+template <int DIM>
+__global__ __launch_bounds__(256) void kl_prep(const SlotDesc *__restrict__ desc,
+                                               const int64_t *__restrict__ base,
+                                               const int *__restrict__ nact) {
+  __shared__ double sh[256];
+  const int n = *nact;
+  const int64_t total = base[n];
+  for (int64_t it = blockIdx.x; it < total; it += gridDim.x) {
+    const int s = find_item(base, n, it);
+    const PrepArgs a = desc[s].prep;
+    prep_body<DIM>(a, it - base[s], sh);
+  }
+}
+
+__global__ __launch_bounds__(256) void kl_dred(const SlotDesc *__restrict__ desc,
+                                               const int *__restrict__ nact) {
+  __shared__ double sh[256];
+  const int n = *nact;
+  for (int s = blockIdx.x; s < n; s += gridDim.x) {
+    const SlotDesc &d = desc[s];
+    if (d.p1.hist) dred_body(d.prep.DMpart, d.prep.Dmpart, d.nPpart, (double *)d.p1.dstat, sh);
+  }
+}
+
+template <int DIM, int G, int MODE, int PASS>
+__global__ __launch_bounds__(64) void kl_pass1(const SlotDesc *__restrict__ desc,
+                                               const int64_t *__restrict__ base,
+                                               const int *__restrict__ nact) {
+  const int n = *nact;
+  const int64_t total = base[n];
+  for (int64_t it = blockIdx.x; it < total; it += gridDim.x) {
+    const int s = find_item(base, n, it);
+    P1Args a = desc[s].p1;
+    if (PASS == 0) a.out = a.out2;  // max sweep writes the M partials
+    const int64_t loc = it - base[s];
+    const int64_t blk = loc / a.K1;
+    pass1_body<DIM, G, MODE, PASS>(a, blk, (int)(loc - blk * a.K1));
+  }
+}
+
+template <int DIM, int MODE>
+__global__ __launch_bounds__(256) void kl_qfin(const SlotDesc *__restrict__ desc,
+                                               const int64_t *__restrict__ base,
+                                               const int *__restrict__ nact) {
+  __shared__ double sh[256];
+  const int n = *nact;
+  const int64_t total = base[n];
+  for (int64_t it = blockIdx.x; it < total; it += gridDim.x) {
+    const int s = find_item(base, n, it);
+    const QFArgs a = desc[s].qf;
+    qfin_body<DIM, MODE>(a, it - base[s], sh);
+  }
+}
+
+template <int DIM, int MODE, int G>
+__global__ __launch_bounds__(64) void kl_grad(const SlotDesc *__restrict__ desc,
+                                              const int64_t *__restrict__ base,
+                                              const int *__restrict__ nact) {
+  const int n = *nact;
+  const int64_t total = base[n];
+  for (int64_t it = blockIdx.x; it < total; it += gridDim.x) {
+    const int s = find_item(base, n, it);
+    const GArgs a = desc[s].gr;
+    const int64_t loc = it - base[s];
+    const int64_t blk = loc / a.K2;
+    grad_body<DIM, MODE, G>(a, blk, (int)(loc - blk * a.K2));
+  }
+}
+
+template <int DIM>
+__global__ __launch_bounds__(256) void kl_gfin(const SlotDesc *__restrict__ desc,
+                                               const int64_t *__restrict__ base,
+                                               const int *__restrict__ nact) {
+  const int n = *nact;
+  const int64_t total = base[n];
+  for (int64_t it = blockIdx.x; it < total; it += gridDim.x) {
+    const int s = find_item(base, n, it);
+    const GFArgs a = desc[s].gf;
+    gfin_body<DIM>(a, it - base[s]);
+  }
+}
+
+__global__ __launch_bounds__(256) void kl_final(const SlotDesc *__restrict__ desc,
+                                                const int *__restrict__ nact) {
+  __shared__ double sh[256];
+  const int n = *nact;
+  for (int s = blockIdx.x; s < n; s += gridDim.x) {
+    const FinArgs a = desc[s].fin;
+    final_body(a, sh);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// per-solve L-BFGS vector work (one 1024-thread block per solve)
+// ---------------------------------------------------------------------------
+constexpr int kBT = 1024;
+
+__device__ __forceinline__ double bsum(double v, double *sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+#pragma unroll
+  for (int w = kBT / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+__device__ __forceinline__ double bmax(double v, double *sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+#pragma unroll
+  for (int w = kBT / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ double bdot(const double *a, const double *b, int64_t n, double *sh) {
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += kBT) s += a[i] * b[i];
+  return bsum(s, sh);
+}
+
+// actions: kActTrial  x = x0 + alpha p
+//          kActIter   (history update if has_prev) + two-loop direction + first
+//                     trial step of the line search
+__global__ __launch_bounds__(kBT) void kl_action(const SlotVecs *__restrict__ vecs,
+                                                 const SlotIn *__restrict__ in,
+                                                 SlotOut *__restrict__ out,
+                                                 const int *__restrict__ nact) {
+  __shared__ double sh[kBT];
+  const int n = *nact;
+  for (int s = blockIdx.x; s < n; s += gridDim.x) {
+    const SlotVecs v = vecs[s];
+    const SlotIn c = in[s];
+    const int64_t N = v.N;
+    if (c.action == kActTrial) {
+      for (int64_t i = threadIdx.x; i < N; i += kBT) v.x[i] = v.x0[i] + c.alpha * v.p[i];
+      continue;
+    }
+    if (c.action != kActIter) continue;
+    SlotHist &h = *v.hist;
+    int count = h.count, newest = h.newest;
+    if (c.has_prev) {
+      // s_k = x - x0, y_k = g - g(x0); stored only when s.y > 0
+      double cs = 0.0;
+      for (int64_t i = threadIdx.x; i < N; i += kBT) cs += (v.x[i] - v.x0[i]) * (v.g[i] - v.gv[i]);
+      const double curv = bsum(cs, sh);
+      if (curv > 0.0) {
+        const int slot = (newest + 1) % kHist;
+        for (int64_t i = threadIdx.x; i < N; i += kBT) {
+          v.S[slot][i] = v.x[i] - v.x0[i];
+          v.Y[slot][i] = v.g[i] - v.gv[i];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) h.rho[slot] = 1.0 / curv;
+        newest = slot;
+        count = count < kHist ? count + 1 : kHist;
+      }
+    }
+    // two-loop recursion (newest first), p = -H g
+    for (int64_t i = threadIdx.x; i < N; i += kBT) v.p[i] = v.g[i];
+    __syncthreads();
+    double c1[kHist];
+    for (int i = 0; i < count; ++i) {
+      const int sl = ((newest - i) % kHist + kHist) % kHist;
+      c1[i] = h.rho[sl] * bdot(v.S[sl], v.p, N, sh);
+      for (int64_t t = threadIdx.x; t < N; t += kBT) v.p[t] += -c1[i] * v.Y[sl][t];
+      __syncthreads();
+    }
+    for (int i = count - 1; i >= 0; --i) {
+      const int sl = ((newest - i) % kHist + kHist) % kHist;
+      const double c2 = h.rho[sl] * bdot(v.Y[sl], v.p, N, sh);
+      const double cc = c1[i] - c2;
+      for (int64_t t = threadIdx.x; t < N; t += kBT) v.p[t] += cc * v.S[sl][t];
+      __syncthreads();
+    }
+    for (int64_t i = threadIdx.x; i < N; i += kBT) v.p[i] = -v.p[i];
+    __syncthreads();
+    const double g0 = bdot(v.g, v.p, N, sh);
+    const double df = fmax(c.f_prev - c.f, 100.0 * 2.220446049250313e-16);
+    const double a1 = c.first ? 1.0 : fmin(1.0, -1.01 * 2.0 * df / g0);
+    for (int64_t i = threadIdx.x; i < N; i += kBT) {
+      v.x0[i] = v.x[i];
+      v.gv[i] = v.g[i];
+      v.x[i] = v.x[i] + a1 * v.p[i];
+    }
+    if (threadIdx.x == 0) {
+      out[s].g0 = g0;
+      out[s].a1 = a1;
+      h.count = count;
+      h.newest = newest;
+    }
+    __syncthreads();
+  }
+}
+
+// slope along p and stopping statistics of the gradient just evaluated
+__global__ __launch_bounds__(kBT) void kl_dots(const SlotVecs *__restrict__ vecs,
+                                               const SlotIn *__restrict__ in,
+                                               SlotOut *__restrict__ out,
+                                               const int *__restrict__ nact) {
+  __shared__ double sh[kBT];
+  const int n = *nact;
+  for (int s = blockIdx.x; s < n; s += gridDim.x) {
+    const SlotVecs v = vecs[s];
+    const double tol = in[s].tol;
+    const int64_t N = v.N;
+    double gp = 0.0, l1 = 0.0, cw = -INFINITY, rel = 0.0;
+    for (int64_t i = threadIdx.x; i < N; i += kBT) {
+      const double g = v.g[i];
+      gp += g * v.p[i];
+      l1 += fabs(g);
+      cw = fmax(cw, fabs(g) - v.nu[i] * tol);
+      rel = fmax(rel, fabs(g) / v.nu[i]);
+    }
+    const double a = bsum(gp, sh);
+    const double b = bsum(l1, sh);
+    const double c = bmax(cw, sh);
+    const double d = bmax(rel, sh);
+    if (threadIdx.x == 0) {
+      out[s].gp = a;
+      out[s].l1 = b;
+      out[s].cw = c;
+      out[s].relerr = d;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launcher: one step for the current batch
+// ---------------------------------------------------------------------------
+template <int DIM, int MODE, int G>
+static void step_t(const ServeDev &sd, hipStream_t s) {
+  const unsigned g1 = sd.grid_w, gb = sd.grid_b;
+  hipLaunchKernelGGL(kl_action, dim3(sd.grid_slots), dim3(kBT), 0, s, sd.vecs, sd.in, sd.out,
+                     sd.nact);
+  hipLaunchKernelGGL((kl_prep<DIM>), dim3(gb), dim3(256), 0, s, sd.desc, sd.base + 0 * sd.bstride,
+                     sd.nact);
+  if (MODE == kF32Lse) {
+    hipLaunchKernelGGL(kl_dred, dim3(sd.grid_slots), dim3(256), 0, s, sd.desc, sd.nact);
+    hipLaunchKernelGGL((kl_pass1<DIM, G, MODE, 2>), dim3(g1), dim3(64), 0, s, sd.desc,
+                       sd.base + 1 * sd.bstride, sd.nact);
+  } else {
+    hipLaunchKernelGGL((kl_pass1<DIM, G, MODE, 0>), dim3(g1), dim3(64), 0, s, sd.desc,
+                       sd.base + 1 * sd.bstride, sd.nact);
+    hipLaunchKernelGGL((kl_pass1<DIM, G, MODE, 1>), dim3(g1), dim3(64), 0, s, sd.desc,
+                       sd.base + 1 * sd.bstride, sd.nact);
+  }
+  hipLaunchKernelGGL((kl_qfin<DIM, MODE>), dim3(gb), dim3(256), 0, s, sd.desc,
+                     sd.base + 2 * sd.bstride, sd.nact);
+  hipLaunchKernelGGL((kl_grad<DIM, MODE, G>), dim3(g1), dim3(64), 0, s, sd.desc,
+                     sd.base + 3 * sd.bstride, sd.nact);
+  hipLaunchKernelGGL((kl_gfin<DIM>), dim3(gb), dim3(256), 0, s, sd.desc, sd.base + 4 * sd.bstride,
+                     sd.nact);
+  hipLaunchKernelGGL(kl_final, dim3(sd.grid_slots), dim3(256), 0, s, sd.desc, sd.nact);
+  hipLaunchKernelGGL(kl_dots, dim3(sd.grid_slots), dim3(kBT), 0, s, sd.vecs, sd.in, sd.out,
+                     sd.nact);
+}
+
+template <int DIM, int MODE>
+static hipError_t step_g(int G, const ServeDev &sd, hipStream_t s) {
+  switch (G) {
+    case 1: step_t<DIM, MODE, 1>(sd, s); break;
+    case 2: step_t<DIM, MODE, 2>(sd, s); break;
+    case 3: step_t<DIM, MODE, 3>(sd, s); break;
+    case 4: step_t<DIM, MODE, 4>(sd, s); break;
+    case 5: step_t<DIM, MODE, 5>(sd, s); break;
+    case 6: step_t<DIM, MODE, 6>(sd, s); break;
+    case 7: step_t<DIM, MODE, 7>(sd, s); break;
+    case 8: step_t<DIM, MODE, 8>(sd, s); break;
+    case 9: step_t<DIM, MODE, 9>(sd, s); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_serve_step(int dim, int G, int mode, const ServeDev &sd, hipStream_t s) {
+  if (dim == 2) {
+    if (mode == kF32Lse) return step_g<2, kF32Lse>(G, sd, s);
+    return step_g<2, kF64Lse>(G, sd, s);
+  }
+  if (mode == kF32Lse) return step_g<3, kF32Lse>(G, sd, s);
+  return step_g<3, kF64Lse>(G, sd, s);
+}
+
+}  // namespace sdot

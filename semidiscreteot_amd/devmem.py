@@ -74,5 +74,10 @@ class DeviceArray:
             pass
 
 
+def copy_d2d(dst: DeviceArray, src: DeviceArray):
+    assert dst.n == src.n
+    _ok(hip().hipMemcpy(dst.ptr, src.ptr, src.n * 8, 3), "hipMemcpy D2D")
+
+
 def synchronize():
     _ok(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
