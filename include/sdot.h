@@ -48,6 +48,17 @@ typedef enum { SDOT_CTRL_L1NORM = 0, SDOT_CTRL_COMPONENTWISE = 1 } sdot_control_
  *            every reduction in double.  Plain (non-LSE) mode always runs FP64. */
 typedef enum { SDOT_PREC_FP64 = 0, SDOT_PREC_FP32_LSE = 1 } sdot_precision;
 
+/* Line search of the BFGS driver.
+ *  REFERENCE:   deal.II's line minimisation as SotSolver::solve uses it.
+ *  NOISE_AWARE: the same search until the decrease the first trial predicts
+ *               falls below 100x the rounding noise of the functional (about
+ *               2e-15 |F| in FP64, 2e-9 |F| in FP32_LSE); from then on the
+ *               functional differences it compares are trapezoid integrals of
+ *               the measured slopes (Hager & Zhang's approximate Wolfe idea)
+ *               and the first trial step is 1.  Lets a solve reach tolerances
+ *               at which F differences are below their rounding noise. */
+typedef enum { SDOT_LS_REFERENCE = 0, SDOT_LS_NOISE_AWARE = 1 } sdot_line_search;
+
 /* SotParameterManager::SolverParameters (ParameterManager.h:56-72), the keys
  * of subsection rsot_solver (ParameterManager.cc:64-95), plus `precision`. */
 typedef struct {
@@ -67,6 +78,7 @@ typedef struct {
   uint32_t epsilon_scaling_steps; /* 5 */
   int32_t use_log_sum_exp_trick;  /* 0 */
   int32_t precision;              /* sdot_precision, FP64 */
+  int32_t line_search;            /* sdot_line_search, REFERENCE */
 } sdot_params;
 
 typedef struct {
@@ -194,6 +206,10 @@ typedef struct {
   int32_t use_graph;         /* 1: replay one captured hipGraph per step (default 1) */
   int32_t pass_targets;      /* targets per q-major work item (0: tuned or default) */
   int32_t grad_qpoints;      /* q-points per target-major work item (0: tuned or default) */
+  double polish_factor;      /* FP32_LSE: a solve switches to FP64 evaluations once its
+                                stopping value is within this factor of the tolerance (or
+                                an FP32 line search fails), so that it stops on an FP64
+                                evaluation; 0: stay FP32 (default 10) */
 } sdot_server_config;
 
 typedef struct {
@@ -207,6 +223,7 @@ typedef struct {
   uint32_t max_iterations;
   double radius;               /* truncation radius R of the candidate sets; <= 0 or
                                   DBL_MAX: every target is a candidate */
+  int32_t line_search;         /* sdot_line_search */
 } sdot_request;
 
 typedef struct {

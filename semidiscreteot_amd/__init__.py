@@ -8,6 +8,7 @@ package is the host-side mirror of the reference's SotSolver interface.
 from ._lib import LIB_PATH, SdotError, load  # noqa: F401
 from .mesh import SourceArrays, simplex_source, structured_source  # noqa: F401
 from .solver import SolverParameters, SotSolver  # noqa: F401
+from .server import ServeResult, SotServer  # noqa: F401
 
-__all__ = ["SotSolver", "SolverParameters", "SourceArrays", "structured_source",
+__all__ = ["SotSolver", "SolverParameters", "SotServer", "ServeResult", "SourceArrays", "structured_source",
            "simplex_source", "SdotError", "load", "LIB_PATH"]

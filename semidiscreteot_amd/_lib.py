@@ -35,7 +35,8 @@ class Params(C.Structure):
                 ("quadrature_order", C.c_uint32), ("nb_points", C.c_uint32),
                 ("number_of_threads", C.c_uint32), ("use_epsilon_scaling", C.c_int32),
                 ("epsilon_scaling_factor", C.c_double), ("epsilon_scaling_steps", C.c_uint32),
-                ("use_log_sum_exp_trick", C.c_int32), ("precision", C.c_int32)]
+                ("use_log_sum_exp_trick", C.c_int32), ("precision", C.c_int32),
+                ("line_search", C.c_int32)]
 
 
 class SolveInfo(C.Structure):
@@ -54,6 +55,27 @@ class Profile(C.Structure):
     _fields_ = [("evaluations", C.c_int64), ("ms_lse_max", C.c_double),
                 ("ms_lse_sum", C.c_double), ("ms_qfin", C.c_double), ("ms_grad", C.c_double),
                 ("ms_total", C.c_double)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("max_slots", C.c_int32),
+                ("hbm_budget_bytes", C.c_int64), ("use_graph", C.c_int32),
+                ("pass_targets", C.c_int32), ("grad_qpoints", C.c_int32),
+                ("polish_factor", C.c_double)]
+
+
+class Request(C.Structure):
+    _fields_ = [("n", C.c_int64), ("y", _dp), ("nu", _dp), ("psi0", _dp),
+                ("epsilon", C.c_double), ("tolerance", C.c_double),
+                ("solver_control_type", C.c_int32), ("max_iterations", C.c_uint32),
+                ("radius", C.c_double), ("line_search", C.c_int32)]
+
+
+class Result(C.Structure):
+    _fields_ = [("id", C.c_int64), ("status", C.c_int32), ("iterations", C.c_int32),
+                ("evaluations", C.c_int32), ("functional", C.c_double),
+                ("check_value", C.c_double), ("rel_marginal_error", C.c_double),
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double)]
 
 
 # name -> (restype, argtypes); every function declared in include/sdot.h
@@ -83,6 +105,17 @@ SIGNATURES = {
     "sdot_set_state": (C.c_int, [C.c_void_p, C.POINTER(State)]),
     "sdot_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "sdot_profile_get": (C.c_int, [C.c_void_p, C.POINTER(Profile)]),
+    "sdot_server_config_default": (C.c_int, [C.POINTER(ServerConfig)]),
+    "sdot_server_create": (C.c_int, [C.c_void_p, C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
+    "sdot_server_destroy": (C.c_int, [C.c_void_p]),
+    "sdot_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Request), _i64p]),
+    "sdot_server_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "sdot_server_poll": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_int32),
+                                   C.POINTER(Result), _dp]),
+    "sdot_server_clock": (C.c_int, [C.c_void_p, _dp]),
+    "sdot_server_request_bytes": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, _i64p]),
+    "sdot_server_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32),
+                                       C.POINTER(C.c_int32)]),
     "sdot_mesh_structured": (C.c_int, [C.c_int, C.POINTER(C.c_int), _dp, _dp, C.c_int, _dp,
                                        C.c_int, _dp, _dp, _dp]),
     "sdot_mesh_simplex": (C.c_int, [C.c_int, C.c_int64, _dp, C.c_int64, _i64p, C.c_int, _dp,

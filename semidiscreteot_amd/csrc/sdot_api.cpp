@@ -288,6 +288,8 @@ int prepare(sdot_ctx *c, double R) {
   }
   p.K1 = (int)std::min<int64_t>(K1, kMaxSeg);
   p.K2 = (int)std::min<int64_t>(K2, kMaxSeg);
+  // the gradient grid gives each XCD the segments s = xcd (mod 8)
+  if (p.K2 >= 8) p.K2 -= p.K2 % 8;
   p.nFpart = (p.nQ + 255) / 256;
   p.nPpart = (p.N + 255) / 256;
   Dev &d = c->dev;
@@ -540,6 +542,8 @@ extern "C" int sdot_set_params(sdot_ctx *c, const sdot_params *p) {
   if (!(p->epsilon > 0) || !std::isfinite(p->epsilon)) return fail(SDOT_ERR_INVALID, "epsilon must be finite and > 0");
   if (p->precision != SDOT_PREC_FP64 && p->precision != SDOT_PREC_FP32_LSE)
     return fail(SDOT_ERR_INVALID, "unknown precision");
+  if (p->line_search != SDOT_LS_REFERENCE && p->line_search != SDOT_LS_NOISE_AWARE)
+    return fail(SDOT_ERR_INVALID, "unknown line_search");
   c->params = *p;
   c->hist_valid = false;
   return SDOT_OK;
@@ -828,7 +832,11 @@ extern "C" int sdot_solve(sdot_ctx *c, double *psi, sdot_solve_info *info) {
     if ((unsigned)step >= max_it || std::isnan(cv)) return kFailure;
     return kIterate;
   };
-  BfgsResult r = lbfgs_solve(N, psi, eval, check);
+  double noise_rel = 0.0;
+  if (c->params.line_search == SDOT_LS_NOISE_AWARE)
+    noise_rel = (c->params.use_log_sum_exp_trick && c->params.precision == SDOT_PREC_FP32_LSE)
+                    ? kNoiseRelF32 : kNoiseRelF64;
+  BfgsResult r = lbfgs_solve(N, psi, eval, check, 5, noise_rel);
   if (eval_rc) return eval_rc;
   c->last_iterations = r.iterations;
   c->converged = r.status == kConverged;

@@ -12,6 +12,16 @@
 //     interval, line_minimization.h:423-455): bracketing growth 9, sectioning
 //     safeguards 0.1 / 0.5.  Running out of evaluations is reported as
 //     kLineSearchFail (stock deal.II throws there).
+//   * optional noise-aware mode (not in the reference): once the decrease the
+//     first trial predicts, |g0|, falls below 100 times the noise floor of the
+//     functional (noise_rel |f0|), every later line search of the solve
+//     compares functional differences taken from the trapezoid integral of the
+//     measured slopes, f(a) ~ f(b) + (a - b) (g(a) + g(b)) / 2 from the nearest
+//     evaluated point b, and starts from the step 1 (the measured differences
+//     of f are noise from then on, also for the first-step rule).  This is the idea of the approximate
+//     Wolfe conditions of Hager & Zhang (SIAM J. Optim. 16 (2005) 170-192):
+//     near the minimiser F changes by less than its rounding noise while the
+//     gradient is still accurate, and the integral is exact on quadratics.
 #pragma once
 
 #include <cfloat>
@@ -42,23 +52,28 @@ inline double bfgs_dot(const double *a, const double *b, int64_t n) {
 // One-dimensional Wolfe search along p from x0.  phi(a, f, g) evaluates the
 // objective at x0 + a p and returns f and the slope g = grad . p.
 template <class Phi>
-bool wolfe_line_search(Phi &&phi, double f0, double g0, double a1, double &alpha) {
+bool wolfe_line_search(Phi &&phi, double f0, double g0, double a1, double &alpha,
+                       bool surrogate = false) {
   constexpr double eta = 0.9, mu = 0.001, a_max = DBL_MAX;
   constexpr unsigned max_evals = 20;
   constexpr double grow = 9.0, shrink_lo = 0.1, shrink_hi = 0.5;
+  // in the noise-aware regime every value is an offset from f0 (its
+  // increments would be absorbed by the magnitude of f0 otherwise)
+  const double fb = surrogate ? 0.0 : f0;
   const double slope_tol = eta * std::fabs(g0);
-  const double f_floor = f0 + a_max * mu * g0;
-  auto armijo = [&](double a, double f) { return f <= f0 + a * mu * g0; };
+  const double f_floor = fb + a_max * mu * g0;
+  auto armijo = [&](double a, double f) { return f <= fb + a * mu * g0; };
   auto curvature = [&](double g) { return std::fabs(g) <= slope_tol; };
 
   double lo_a = NAN, lo_f = NAN, lo_g = NAN, hi_a = NAN, hi_f = NAN, hi_g = NAN;
   unsigned used = 0;
-  double prev_a = 0.0, prev_f = f0, prev_g = g0;
+  double prev_a = 0.0, prev_f = fb, prev_g = g0;
   double a = a1, f = 0.0, g = 0.0;
   bool bracketed = false;
   while (used < max_evals) {
     phi(a, f, g);
     ++used;
+    if (surrogate) f = prev_f + (a - prev_a) * (0.5 * (prev_g + g));
     if (f <= f_floor || a == a_max) { alpha = a; return true; }
     if (!armijo(a, f) || (f >= prev_f && used > 1)) {
       lo_a = prev_a; lo_f = prev_f; lo_g = prev_g;
@@ -110,6 +125,7 @@ bool wolfe_line_search(Phi &&phi, double f0, double g0, double a1, double &alpha
     a = trial;
     phi(a, f, g);
     ++used;
+    if (surrogate) f = lo_f + (a - lo_a) * (0.5 * (lo_g + g));
     if (!armijo(a, f) || f >= lo_f) {
       hi_a = a; hi_f = f; hi_g = g;
     } else {
@@ -124,8 +140,19 @@ bool wolfe_line_search(Phi &&phi, double f0, double g0, double a1, double &alpha
 }
 
 // eval(x, g) -> f ; check(step, g, &value) -> CheckState
+// First trial step (wasserstein_solver_bfgs.h:335-341), or 1 in the
+// noise-aware regime.
+inline double first_step(bool first, bool surrogate, double f_prev, double f, double g0) {
+  if (first || surrogate) return 1.0;
+  const double df = std::fmax(f_prev - f, 100.0 * DBL_EPSILON);
+  return std::fmin(1.0, -1.01 * 2.0 * df / g0);
+}
+
+// noise_rel > 0 selects the noise-aware line search with noise floor
+// noise_rel |f|; 0 is the reference algorithm.
 template <class Eval, class Check>
-BfgsResult lbfgs_solve(int64_t n, double *x, Eval &&eval, Check &&check, int history = 5) {
+BfgsResult lbfgs_solve(int64_t n, double *x, Eval &&eval, Check &&check, int history = 5,
+                       double noise_rel = 0.0) {
   BfgsResult res;
   std::vector<double> g(n), p(n), sk(n), yk(n), x0(n);
   std::vector<std::vector<double>> S(history, std::vector<double>(n)),
@@ -139,7 +166,7 @@ BfgsResult lbfgs_solve(int64_t n, double *x, Eval &&eval, Check &&check, int his
   int k = 0;
   double cv = 0.0;
   int st = check(k, g.data(), cv);
-  bool first = true;
+  bool first = true, sur = false;
   double f_prev = 0.0;
   while (st == kIterate) {
     // two-loop recursion, H0 = identity
@@ -164,8 +191,8 @@ BfgsResult lbfgs_solve(int64_t n, double *x, Eval &&eval, Check &&check, int his
 
     const double f0 = f, g0 = bfgs_dot(g.data(), p.data(), n);
     std::memcpy(x0.data(), x, sizeof(double) * n);
-    const double df = std::fmax(f_prev - f, 100.0 * DBL_EPSILON);
-    const double a1 = first ? 1.0 : std::fmin(1.0, -1.01 * 2.0 * df / g0);
+    sur = noise_rel > 0.0 && (sur || std::fabs(g0) < 100.0 * noise_rel * std::fabs(f0));
+    const double a1 = first_step(first, sur, f_prev, f, g0);
     f_prev = f;
     auto phi = [&](double a, double &fo, double &go) {
       for (int64_t t = 0; t < n; ++t) x[t] = x0[t] + a * p[t];
@@ -175,7 +202,7 @@ BfgsResult lbfgs_solve(int64_t n, double *x, Eval &&eval, Check &&check, int his
       go = bfgs_dot(g.data(), p.data(), n);
     };
     double alpha = 0.0;
-    if (!wolfe_line_search(phi, f0, g0, a1, alpha)) {
+    if (!wolfe_line_search(phi, f0, g0, a1, alpha, sur)) {
       ++k;
       res.status = kLineSearchFail;
       res.iterations = k;
@@ -221,15 +248,16 @@ class LineSearchSM {
  public:
   enum { kEval = 0, kAccept = 1, kFail = -1 };
 
-  void start(double f0, double g0, double a1) {
-    f0_ = f0;
+  void start(double f0, double g0, double a1, bool surrogate = false) {
+    sur_ = surrogate;
+    f0_ = surrogate ? 0.0 : f0;  // noise-aware: offsets from f0
     g0_ = g0;
     tol_ = 0.9 * std::fabs(g0);
-    floor_ = f0 + DBL_MAX * 0.001 * g0;
+    floor_ = f0_ + DBL_MAX * 0.001 * g0;
     used_ = 0;
     phase_ = 0;
     prev_a_ = 0.0;
-    prev_f_ = f0;
+    prev_f_ = f0_;
     prev_g_ = g0;
     a_ = a1;
   }
@@ -240,6 +268,9 @@ class LineSearchSM {
     constexpr unsigned max_evals = 20;
     ++used_;
     const double a = a_;
+    if (sur_)
+      f = phase_ == 0 ? prev_f_ + (a - prev_a_) * (0.5 * (prev_g_ + g))
+                      : lo_f_ + (a - lo_a_) * (0.5 * (lo_g_ + g));
     if (phase_ == 0) {
       if (f <= floor_ || a == DBL_MAX) return kAccept;
       if (!armijo(a, f) || (f >= prev_f_ && used_ > 1)) {
@@ -302,6 +333,7 @@ class LineSearchSM {
     }
     return 0.5 * (bl + bh);
   }
+  bool sur_ = false;
   double f0_ = 0, g0_ = 0, tol_ = 0, floor_ = 0;
   unsigned used_ = 0;
   int phase_ = 0;

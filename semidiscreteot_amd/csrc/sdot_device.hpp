@@ -28,6 +28,15 @@ __device__ __forceinline__ bool in_set(const double *y, const double *c, double 
   return s <= smax;
 }
 
+// Records a kernel only reads are addressed through the constant address
+// space: the compiler may then keep wave-uniform record loads in scalar
+// registers (s_load) even where a store of the same kernel could alias them
+// as far as it can tell (the persistent work-list loops store partial sums
+// between the record streams of consecutive items).
+using cdouble = const __attribute__((address_space(4))) double;
+
+__device__ __forceinline__ cdouble *cptr(const double *p) { return (cdouble *)p; }
+
 template <int MODE>
 __device__ __forceinline__ double expo(double d) {
   if constexpr (MODE == kF32Lse) {
@@ -118,7 +127,8 @@ constexpr double kSeedSpread = 2.0;
 template <int DIM>
 struct TRec {
   double a, yl[DIM], y[DIM], nu;
-  __device__ __forceinline__ void load(const double *__restrict__ r) {
+  template <class P>
+  __device__ __forceinline__ void load(P r) {
     a = r[0];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
@@ -303,13 +313,14 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
         S[k] = fma(r.nu, (double)__builtin_amdgcn_exp2f((float)dl[k]), S[k]);
     }
   };
+  cdouble *trec = cptr(a.trec);
   auto body = [&](int j0, int j1) {
     // records are fetched two at a time ahead of their use
     for (int j = j0; j < j1; j += 2) {
       const int j2 = (j + 1 < j1) ? j + 1 : j;
       TRec<DIM> r0, r1;
-      r0.load(a.trec + (int64_t)j * kRec);
-      r1.load(a.trec + (int64_t)j2 * kRec);
+      r0.load(trec + (int64_t)j * kRec);
+      r1.load(trec + (int64_t)j2 * kRec);
       one(r0);
       if (j + 1 < j1) one(r1);
     }
@@ -418,7 +429,9 @@ __device__ __forceinline__ void grad_body(const GArgs &a, int64_t blk, int seg) 
   }
   double acc = 0.0;
   const int nq = a.nq;
-  auto chunk = [&](const double *__restrict__ qp) {
+  cdouble *qrec = cptr(a.qrec_r);
+  cdouble *ccr = cptr(a.cc_r);
+  auto chunk = [&](cdouble *qp) {
     double qx[G][DIM], qm[G], qs[G];
 #pragma unroll
     for (int k = 0; k < G; ++k) {
@@ -445,13 +458,13 @@ __device__ __forceinline__ void grad_body(const GArgs &a, int64_t blk, int seg) 
   auto body = [&](int c0, int c1) {
     for (int c = c0; c < c1; ++c) {
       if (!a.dense) {
-        const double *__restrict__ cp = a.cc_r + (int64_t)c * 4;
+        cdouble *cp = ccr + (int64_t)c * 4;
         double cv[DIM];
 #pragma unroll
         for (int d = 0; d < DIM; ++d) cv[d] = cp[d];
         if (!in_set<DIM>(me.y, cv, a.smax)) continue;
       }
-      const double *qp = a.qrec_r + (int64_t)c * nq * kRec;
+      cdouble *qp = qrec + (int64_t)c * nq * kRec;
       for (int k0 = 0; k0 < nq; k0 += G) chunk(qp + (int64_t)k0 * kRec);
     }
   };

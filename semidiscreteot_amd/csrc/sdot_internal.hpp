@@ -215,6 +215,13 @@ int ctx_source_view(sdot_ctx *c, SourceView &v);
 // Count of serving queues attached to the context (its source stays fixed meanwhile).
 void ctx_attach(sdot_ctx *c, int delta);
 
+// Rounding noise of the assembled functional relative to |F|, for the
+// noise-aware line search: FP64 evaluations scatter by ~2e-16 |F| and FP32_LSE
+// evaluations (FP32 exponentials) by ~4e-10 |F| between neighbouring
+// potentials; the constants keep a factor 5-10 above that.
+constexpr double kNoiseRelF64 = 2e-15;
+constexpr double kNoiseRelF32 = 2e-9;
+
 // Kernel launchers (sdot_kernels.hip).
 hipError_t launch_eval(const Plan &p, Dev &d, const double *d_psi, double *d_grad,
                        double *d_FC, hipStream_t s, Profile *prof);

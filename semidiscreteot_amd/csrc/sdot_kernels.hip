@@ -63,9 +63,14 @@ __global__ __launch_bounds__(256) void k_qfin(QFArgs a) {
   qfin_body<DIM, MODE>(a, blockIdx.x, sh);
 }
 
+// One-dimensional grid, segment index fastest.  Workgroups are dispatched
+// round-robin over the 8 XCDs, so with K2 a multiple of 8 XCD x runs exactly
+// the segments s = x (mod 8): its L2 holds one eighth of the q records instead
+// of streaming all of them.
 template <int DIM, int MODE, int G>
 __global__ __launch_bounds__(64) void k_grad(GArgs a) {
-  grad_body<DIM, MODE, G>(a, blockIdx.x, blockIdx.y);
+  const int64_t blk = blockIdx.x / a.K2;
+  grad_body<DIM, MODE, G>(a, blk, (int)(blockIdx.x - blk * a.K2));
 }
 
 template <int DIM>
@@ -314,7 +319,7 @@ static hipError_t launch_pass1_g(int G, const P1Args &a, int64_t nb, int K1, hip
 
 template <int DIM, int MODE>
 static void launch_grad_g(int G, const GArgs &g, int64_t ntw, int K2, hipStream_t s) {
-  const dim3 gg((unsigned)ntw, (unsigned)K2);
+  const dim3 gg((unsigned)(ntw * K2));
   switch (G) {
     case 1: hipLaunchKernelGGL((k_grad<DIM, MODE, 1>), gg, dim3(64), 0, s, g); break;
     case 2: hipLaunchKernelGGL((k_grad<DIM, MODE, 2>), gg, dim3(64), 0, s, g); break;

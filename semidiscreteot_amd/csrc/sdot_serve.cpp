@@ -165,13 +165,12 @@ SolveOffsets carve(const SourceView &src, const SolveShape &s) {
   return o;
 }
 
-enum Phase { kStart = 0, kNewDir = 1, kTrial = 2 };
-
 struct Request {
   int64_t id = 0;
   int64_t N = 0;
   double eps = 1.0, tol = 0.0, R = INFINITY;
   int control = SDOT_CTRL_L1NORM;
+  int line_search = SDOT_LS_REFERENCE;
   uint32_t max_it = 0;
   std::vector<double> y, nu, psi0;
   TargetLayout L;
@@ -188,17 +187,10 @@ struct Solve {
   SolveOffsets o{};
   SlotDesc desc{};
   SlotVecs vecs{};
-  int64_t items[5] = {0, 0, 0, 0, 0};
-  // host side of the solve (lbfgs_solve's loop, unrolled into steps)
-  Phase phase = kStart;
-  LineSearchSM ls;
-  double f = 0.0, f_prev = 0.0, check = 0.0, relerr = 0.0;
-  bool first = true;
-  bool seeded = false;    // FP32: the next evaluation starts from the history seed
-  bool redo = false;      // repeat the evaluation unseeded
-  bool dir_ready = false; // a redone direction step: g0 / a1 already known
-  double g0 = 0.0, a1 = 0.0;
-  int k = 0, evals = 0;
+  int64_t items[kNumLists] = {};
+  int mode = kF32Lse;   // arithmetic of this solve's evaluations
+  SolveDriver drv;
+  bool seeded = false;  // FP32: the next evaluation starts from the history seed
   double t_admit = 0.0;
 };
 
@@ -213,7 +205,8 @@ struct sdot_server {
   sdot_ctx *ctx = nullptr;
   SourceView src;
   int device = 0;
-  int mode = kF32Lse;
+  int mode = kF32Lse;   // mode a solve starts in
+  double polish = 10.0;
   int max_slots = 256;
   int use_graph = 1;
   int pass_targets = kDefaultPassTargets, grad_qpoints = kDefaultGradQ;
@@ -273,6 +266,10 @@ int fill_shape(sdot_server *s, Request &r) {
   const int64_t k2 = (src.nQ + s->grad_qpoints - 1) / s->grad_qpoints;
   sh.K1 = (int)std::clamp<int64_t>(k1, 1, kMaxSegServe);
   sh.K2 = (int)std::clamp<int64_t>(k2, 1, kMaxSegServe);
+  // work items run segment-fastest and every solve's item count is then a
+  // multiple of 8, so a persistent workgroup (fixed XCD) sees the segments
+  // s = xcd (mod 8) only and its L2 keeps one eighth of the q records
+  sh.K2 = (sh.K2 + 7) / 8 * 8;
   r.bytes = carve(src, sh).total;
   return SDOT_OK;
 }
@@ -288,7 +285,7 @@ void build_desc(sdot_server *s, Solve &v) {
   const SourceView &src = s->src;
   const Request &r = *v.req;
   const SolveShape &sh = r.shape;
-  const bool fp32 = s->mode == kF32Lse;
+  const bool fp32 = v.mode == kF32Lse;
   const double coef = (1.0 / r.eps) * (fp32 ? 1.44269504088896340736 : 1.0);
   const double ln_scale = fp32 ? 0.69314718055994530942 : 1.0;
   double *trec_r = dptr<double>(s, v, v.o.trec_r), *trec_m = dptr<double>(s, v, v.o.trec_m);
@@ -311,7 +308,7 @@ void build_desc(sdot_server *s, Solve &v) {
   pa.trec_r = trec_r;
   pa.trec_m = trec_m;
   pa.PNpart = PNpart;
-  pa.psi_prev = dptr<double>(s, v, v.o.psi_prev);
+  pa.psi_prev = fp32 ? dptr<double>(s, v, v.o.psi_prev) : nullptr;
   pa.DMpart = dptr<double>(s, v, v.o.DMpart);
   pa.Dmpart = dptr<double>(s, v, v.o.Dmpart);
   pa.N = r.N;
@@ -404,11 +401,12 @@ void build_desc(sdot_server *s, Solve &v) {
   w.hist = dptr<SlotHist>(s, v, v.o.hist);
   w.N = r.N;
 
-  v.items[0] = sh.nPpart;
-  v.items[1] = src.n_blocks * sh.K1;
-  v.items[2] = nFpart;
-  v.items[3] = sh.n_twaves * sh.K2;
-  v.items[4] = (r.N + 255) / 256;
+  for (int64_t &it : v.items) it = 0;
+  v.items[kLPrep] = sh.nPpart;
+  v.items[fp32 ? kLPass32 : kLPass64] = src.n_blocks * sh.K1;
+  v.items[fp32 ? kLQfin32 : kLQfin64] = nFpart;
+  v.items[fp32 ? kLGrad32 : kLGrad64] = sh.n_twaves * sh.K2;
+  v.items[kLGfin] = (r.N + 255) / 256;
 }
 
 // Uploads an admitted request into its arena block (offset `off`).
@@ -417,6 +415,11 @@ int admit(sdot_server *s, std::unique_ptr<Request> r, const SolveOffsets &o, siz
   v->o = o;
   v->off = off;
   v->req = std::move(r);
+  v->mode = s->mode;
+  const bool aware = v->req->line_search == SDOT_LS_NOISE_AWARE;
+  const double noise_rel = aware ? (v->mode == kF32Lse ? kNoiseRelF32 : kNoiseRelF64) : 0.0;
+  v->drv.init(v->req->control, v->req->tol, v->req->max_it, noise_rel,
+              v->mode == kF32Lse ? s->polish : 0.0, aware ? kNoiseRelF64 : 0.0);
   v->t_admit = s->now();
   Request &q = *v->req;
   const TargetLayout &L = q.L;
@@ -453,7 +456,7 @@ int upload_batch(sdot_server *s) {
     auto *desc = reinterpret_cast<SlotDesc *>(h + s->off_desc);
     auto *vecs = reinterpret_cast<SlotVecs *>(h + s->off_vecs);
     auto *base = reinterpret_cast<int64_t *>(h + s->off_base);
-    for (int k = 0; k < 5; ++k) base[k * s->bstride] = 0;
+    for (int k = 0; k < kNumLists; ++k) base[k * s->bstride] = 0;
     for (int i = 0; i < n; ++i) {
       Solve &v = *s->active[i];
       v.desc.prep.hist = v.seeded ? 1 : 0;
@@ -462,48 +465,18 @@ int upload_batch(sdot_server *s) {
       v.desc.fin.ucount = &s->d_out[i].lost;
       desc[i] = v.desc;
       vecs[i] = v.vecs;
-      for (int k = 0; k < 5; ++k) base[k * s->bstride + i + 1] = base[k * s->bstride + i] + v.items[k];
+      for (int k = 0; k < kNumLists; ++k)
+        base[k * s->bstride + i + 1] = base[k * s->bstride + i] + v.items[k];
     }
     SV_TRY(hipMemcpyAsync(s->d_block, h, s->off_in, hipMemcpyHostToDevice, s->stream));
     s->dirty = false;
   }
   auto *in = reinterpret_cast<SlotIn *>(h + s->off_in);
-  for (int i = 0; i < n; ++i) {
-    Solve &v = *s->active[i];
-    SlotIn c{};
-    c.action = kActNone;
-    if (!v.redo) {
-      if (v.phase == kNewDir && !v.dir_ready) c.action = kActIter;
-      if (v.phase == kTrial) c.action = kActTrial;
-    }
-    c.first = v.first ? 1 : 0;
-    c.has_prev = v.first ? 0 : 1;
-    c.alpha = v.phase == kTrial ? v.ls.trial() : 0.0;
-    c.f = v.f;
-    c.f_prev = v.f_prev;
-    c.tol = v.req->tol;
-    in[i] = c;
-  }
+  for (int i = 0; i < n; ++i) in[i] = s->active[i]->drv.next_in();
   *reinterpret_cast<int *>(h + s->off_nact) = n;
   SV_TRY(hipMemcpyAsync(s->d_block + s->off_in, h + s->off_in, s->block_bytes - s->off_in,
                         hipMemcpyHostToDevice, s->stream));
   return SDOT_OK;
-}
-
-// Stopping test of SotSolver::solve (SotSolver.cc:156-162, SotSolver.h:382-446).
-int check_state(const Solve &v, const SlotOut &o, double &cv) {
-  const Request &r = *v.req;
-  bool ok;
-  if (r.control == SDOT_CTRL_COMPONENTWISE) {
-    cv = o.cw;
-    ok = cv <= DBL_MIN;
-  } else {
-    cv = o.l1;
-    ok = cv <= r.tol;
-  }
-  if (ok) return kSuccess;
-  if ((unsigned)v.k >= r.max_it || std::isnan(cv)) return kFailure;
-  return kIterate;
 }
 
 int retire(sdot_server *s, size_t i, int status) {
@@ -516,11 +489,11 @@ int retire(sdot_server *s, size_t i, int status) {
   SV_TRY(hipStreamSynchronize(s->stream));
   d.res.id = r.id;
   d.res.status = status;
-  d.res.iterations = v.k;
-  d.res.evaluations = v.evals;
-  d.res.functional = v.f;
-  d.res.check_value = v.check;
-  d.res.rel_marginal_error = v.relerr;
+  d.res.iterations = v.drv.iterations();
+  d.res.evaluations = v.drv.evaluations();
+  d.res.functional = v.drv.functional();
+  d.res.check_value = v.drv.check_value();
+  d.res.rel_marginal_error = v.drv.relerr();
   d.res.t_submit = r.t_submit;
   d.res.t_admit = v.t_admit;
   d.res.t_done = s->now();
@@ -534,82 +507,46 @@ int retire(sdot_server *s, size_t i, int status) {
   return SDOT_OK;
 }
 
-// One solve's reaction to the evaluation of this step; returns a status >= 0
-// when the solve has finished, -1 otherwise.
+// One solve's reaction to the evaluation of this step: a status >= 0 when
+// the solve has finished, -1 otherwise.
 int advance(sdot_server *s, Solve &v, const SlotOut &o) {
   const bool was_seeded = v.seeded;
-  if (s->mode == kF32Lse) {
-    if (!v.seeded) s->dirty = true;
-    v.seeded = true;
+  if (v.mode == kF32Lse && !v.seeded) {
+    v.seeded = true;  // this evaluation left psi and M_q behind
+    s->dirty = true;
   }
-  if (was_seeded && o.lost > 0.0) {
-    // the seeded sweep lost range somewhere: evaluate again without the seed
-    if (v.phase == kNewDir && !v.dir_ready) {
-      v.g0 = o.g0;
-      v.a1 = o.a1;
-      v.dir_ready = true;
-    }
+  // a seeded sweep that lost range is repeated without the seed
+  const bool repeat = was_seeded && o.lost > 0.0;
+  if (repeat) {
     v.seeded = false;
-    v.redo = true;
+    s->dirty = true;
+  }
+  const int st = v.drv.on_result(o, repeat);
+  if (st == SolveDriver::kSwitch) {
+    // finish in FP64: same buffers, FP64 argument blocks and work lists
+    v.mode = kF64Lse;
+    v.seeded = false;
+    build_desc(s, v);
     s->dirty = true;
     return -1;
   }
-  v.redo = false;
-  ++v.evals;
-  v.relerr = o.relerr;
-  int r;
-  switch (v.phase) {
-    case kStart: {
-      v.f = o.F;
-      const int st = check_state(v, o, v.check);
-      if (st == kSuccess) return SDOT_SOLVE_CONVERGED;
-      if (st == kFailure) return std::isnan(v.check) ? SDOT_SOLVE_NONFINITE : SDOT_SOLVE_MAX_ITER;
-      v.phase = kNewDir;
-      return -1;
-    }
-    case kNewDir: {
-      const double g0 = v.dir_ready ? v.g0 : o.g0;
-      const double a1 = v.dir_ready ? v.a1 : o.a1;
-      v.dir_ready = false;
-      v.ls.start(v.f, g0, a1);
-      v.f_prev = v.f;
-      r = v.ls.feed(o.F, o.gp);
-      break;
-    }
-    default:
-      r = v.ls.feed(o.F, o.gp);
-      break;
-  }
-  if (r == LineSearchSM::kEval) {
-    v.phase = kTrial;
-    return -1;
-  }
-  if (r == LineSearchSM::kFail) {
-    ++v.k;
-    v.f = o.F;
-    return SDOT_SOLVE_LINESEARCH_FAIL;
-  }
-  // accepted: the point just evaluated is the new iterate
-  v.f = o.F;
-  v.first = false;
-  ++v.k;
-  const int st = check_state(v, o, v.check);
-  if (st == kSuccess) return SDOT_SOLVE_CONVERGED;
-  if (st == kFailure) return std::isnan(v.check) ? SDOT_SOLVE_NONFINITE : SDOT_SOLVE_MAX_ITER;
-  v.phase = kNewDir;
-  return -1;
+  return st;
 }
 
 int launch_step(sdot_server *s) {
   if (s->exec) {
     SV_TRY(hipGraphLaunch(s->exec, s->stream));
   } else {
-    SV_TRY(launch_serve_step(s->src.dim, s->src.G, s->mode, s->sd, s->stream));
+    SV_TRY(launch_serve_step(s->src.dim, s->src.G, s->sd, s->stream));
   }
   return SDOT_OK;
 }
 
 }  // namespace
+
+namespace {
+int server_init(sdot_server *s, const sdot_server_config &cfg);
+}
 
 // ---------------------------------------------------------------------------
 // C ABI
@@ -622,6 +559,7 @@ extern "C" int sdot_server_config_default(sdot_server_config *cfg) {
   cfg->use_graph = 1;
   cfg->pass_targets = 0;
   cfg->grad_qpoints = 0;
+  cfg->polish_factor = 10.0;
   return SDOT_OK;
 }
 
@@ -640,12 +578,30 @@ extern "C" int sdot_server_create(sdot_ctx *ctx, const sdot_server_config *cfg_i
     return set_error(SDOT_ERR_INVALID, "hbm_budget_bytes must be at least 1 MiB");
   if (cfg.pass_targets < 0 || cfg.grad_qpoints < 0)
     return set_error(SDOT_ERR_INVALID, "work-item sizes must be >= 0");
-  auto s = std::make_unique<sdot_server>();
-  int rc = ctx_source_view(ctx, s->src);
+  if (!(cfg.polish_factor >= 0.0))
+    return set_error(SDOT_ERR_INVALID, "polish_factor must be >= 0");
+  SourceView src;
+  int rc = ctx_source_view(ctx, src);
   if (rc) return rc;
+  sdot_server *s = new sdot_server();
+  s->src = src;
+  rc = server_init(s, cfg);
+  if (rc) {
+    sdot_server_destroy(s);
+    return rc;
+  }
   s->ctx = ctx;
+  ctx_attach(ctx, +1);
+  *out = s;
+  return SDOT_OK;
+}
+
+namespace {
+
+int server_init(sdot_server *s, const sdot_server_config &cfg) {
   s->device = s->src.device;
   s->mode = cfg.precision == SDOT_PREC_FP32_LSE ? kF32Lse : kF64Lse;
+  s->polish = cfg.polish_factor;
   s->max_slots = cfg.max_slots;
   s->use_graph = cfg.use_graph ? 1 : 0;
   if (cfg.pass_targets) s->pass_targets = cfg.pass_targets;
@@ -661,7 +617,7 @@ extern "C" int sdot_server_create(sdot_ctx *ctx, const sdot_server_config *cfg_i
   Carve c;
   s->off_desc = c.take<SlotDesc>(s->max_slots);
   s->off_vecs = c.take<SlotVecs>(s->max_slots);
-  s->off_base = c.take<int64_t>(5 * s->bstride);
+  s->off_base = c.take<int64_t>((size_t)kNumLists * s->bstride);
   s->off_in = c.take<SlotIn>(s->max_slots);
   s->off_nact = c.take<int>(1);
   s->block_bytes = c.off;
@@ -688,18 +644,18 @@ extern "C" int sdot_server_create(sdot_ctx *ctx, const sdot_server_config *cfg_i
   sd.grid_slots = (unsigned)s->max_slots;
   if (s->use_graph) {
     SV_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-    hipError_t le = launch_serve_step(s->src.dim, s->src.G, s->mode, sd, s->stream);
+    hipError_t le = launch_serve_step(s->src.dim, s->src.G, sd, s->stream);
     hipGraph_t g = nullptr;
     hipError_t ce = hipStreamEndCapture(s->stream, &g);
+    s->graph = g;
     if (le != hipSuccess) return set_error(SDOT_ERR_HIP, std::string("step capture: ") + hipGetErrorString(le));
     if (ce != hipSuccess) return set_error(SDOT_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
-    s->graph = g;
     SV_TRY(hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0));
   }
-  ctx_attach(ctx, +1);
-  *out = s.release();
   return SDOT_OK;
 }
+
+}  // namespace
 
 extern "C" int sdot_server_destroy(sdot_server *s) {
   if (!s) return SDOT_OK;
@@ -735,6 +691,9 @@ extern "C" int sdot_server_submit(sdot_server *s, const sdot_request *q, int64_t
   r->tol = q->tolerance;
   r->control = q->solver_control_type;
   r->max_it = q->max_iterations;
+  if (q->line_search != SDOT_LS_REFERENCE && q->line_search != SDOT_LS_NOISE_AWARE)
+    return set_error(SDOT_ERR_INVALID, "unknown line_search");
+  r->line_search = q->line_search;
   r->R = (q->radius <= 0.0 || q->radius >= DBL_MAX || std::isnan(q->radius)) ? INFINITY : q->radius;
   r->y.assign(q->y, q->y + q->n * dim);
   r->nu.assign(q->nu, q->nu + q->n);

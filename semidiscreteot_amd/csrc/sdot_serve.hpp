@@ -3,11 +3,9 @@
 #pragma once
 
 #include "sdot_internal.hpp"
+#include "sdot_serve_driver.hpp"
 
 namespace sdot {
-
-constexpr int kHist = 5;  // L-BFGS history (deal.II SolverBFGS default)
-enum : int { kActNone = 0, kActTrial = 1, kActIter = 2 };
 
 // Argument blocks of one solve's evaluation.
 struct SlotDesc {
@@ -35,22 +33,21 @@ struct SlotVecs {
   int64_t N;
 };
 
-// Written by the host before a step.
-struct SlotIn {
-  int32_t action, first, has_prev, pad;
-  double alpha, f, f_prev, tol;
-};
-
-// Written by the device during a step (F and C first: the final reduction
-// writes them as a pair).  lost > 0: a history-seeded FP32 sweep lost range
-// and the evaluation must be repeated unseeded.
-struct SlotOut {
-  double F, C, gp, l1, cw, relerr, g0, a1, lost;
+// Work lists of one step.  Kernels that do not depend on the arithmetic mode
+// run over every solve; the mode-templated passes run over the solves of their
+// mode (the other solves contribute no items).
+enum : int {
+  kLPrep = 0,
+  kLPass32, kLPass64,
+  kLQfin32, kLQfin64,
+  kLGrad32, kLGrad64,
+  kLGfin,
+  kNumLists
 };
 
 struct ServeDev {
   SlotDesc *desc;
-  int64_t *base;      // [5][bstride] work-list offsets: prep, pass1, qfin, grad, gfin
+  int64_t *base;      // [kNumLists][bstride] item offsets per solve
   int64_t bstride;
   int *nact;
   SlotVecs *vecs;
@@ -61,6 +58,6 @@ struct ServeDev {
   unsigned grid_slots;  // one workgroup per solve (upper bound on the batch)
 };
 
-hipError_t launch_serve_step(int dim, int G, int mode, const ServeDev &sd, hipStream_t s);
+hipError_t launch_serve_step(int dim, int G, const ServeDev &sd, hipStream_t s);
 
 }  // namespace sdot

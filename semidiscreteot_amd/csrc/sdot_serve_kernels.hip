@@ -217,8 +217,11 @@ __global__ __launch_bounds__(kBT) void kl_action(const SlotVecs *__restrict__ ve
     for (int64_t i = threadIdx.x; i < N; i += kBT) v.p[i] = -v.p[i];
     __syncthreads();
     const double g0 = bdot(v.g, v.p, N, sh);
+    // first trial step (wasserstein_solver_bfgs.h:335-341); 1 in the
+    // noise-aware regime (entered before, or |g0| < noise now)
     const double df = fmax(c.f_prev - c.f, 100.0 * 2.220446049250313e-16);
-    const double a1 = c.first ? 1.0 : fmin(1.0, -1.01 * 2.0 * df / g0);
+    const double a1 = (c.first || c.surrogate || fabs(g0) < c.noise)
+                          ? 1.0 : fmin(1.0, -1.01 * 2.0 * df / g0);
     for (int64_t i = threadIdx.x; i < N; i += kBT) {
       v.x0[i] = v.x[i];
       v.gv[i] = v.g[i];
@@ -269,58 +272,53 @@ __global__ __launch_bounds__(kBT) void kl_dots(const SlotVecs *__restrict__ vecs
 // ---------------------------------------------------------------------------
 // launcher: one step for the current batch
 // ---------------------------------------------------------------------------
-template <int DIM, int MODE, int G>
+template <int DIM, int G>
 static void step_t(const ServeDev &sd, hipStream_t s) {
-  const unsigned g1 = sd.grid_w, gb = sd.grid_b;
-  hipLaunchKernelGGL(kl_action, dim3(sd.grid_slots), dim3(kBT), 0, s, sd.vecs, sd.in, sd.out,
-                     sd.nact);
-  hipLaunchKernelGGL((kl_prep<DIM>), dim3(gb), dim3(256), 0, s, sd.desc, sd.base + 0 * sd.bstride,
-                     sd.nact);
-  if (MODE == kF32Lse) {
-    hipLaunchKernelGGL(kl_dred, dim3(sd.grid_slots), dim3(256), 0, s, sd.desc, sd.nact);
-    hipLaunchKernelGGL((kl_pass1<DIM, G, MODE, 2>), dim3(g1), dim3(64), 0, s, sd.desc,
-                       sd.base + 1 * sd.bstride, sd.nact);
-  } else {
-    hipLaunchKernelGGL((kl_pass1<DIM, G, MODE, 0>), dim3(g1), dim3(64), 0, s, sd.desc,
-                       sd.base + 1 * sd.bstride, sd.nact);
-    hipLaunchKernelGGL((kl_pass1<DIM, G, MODE, 1>), dim3(g1), dim3(64), 0, s, sd.desc,
-                       sd.base + 1 * sd.bstride, sd.nact);
-  }
-  hipLaunchKernelGGL((kl_qfin<DIM, MODE>), dim3(gb), dim3(256), 0, s, sd.desc,
-                     sd.base + 2 * sd.bstride, sd.nact);
-  hipLaunchKernelGGL((kl_grad<DIM, MODE, G>), dim3(g1), dim3(64), 0, s, sd.desc,
-                     sd.base + 3 * sd.bstride, sd.nact);
-  hipLaunchKernelGGL((kl_gfin<DIM>), dim3(gb), dim3(256), 0, s, sd.desc, sd.base + 4 * sd.bstride,
-                     sd.nact);
-  hipLaunchKernelGGL(kl_final, dim3(sd.grid_slots), dim3(256), 0, s, sd.desc, sd.nact);
-  hipLaunchKernelGGL(kl_dots, dim3(sd.grid_slots), dim3(kBT), 0, s, sd.vecs, sd.in, sd.out,
-                     sd.nact);
+  const unsigned gw = sd.grid_w, gb = sd.grid_b, gs = sd.grid_slots;
+  const int64_t *b = sd.base;
+  const int64_t L = sd.bstride;
+  hipLaunchKernelGGL(kl_action, dim3(gs), dim3(kBT), 0, s, sd.vecs, sd.in, sd.out, sd.nact);
+  hipLaunchKernelGGL((kl_prep<DIM>), dim3(gb), dim3(256), 0, s, sd.desc, b + kLPrep * L, sd.nact);
+  hipLaunchKernelGGL(kl_dred, dim3(gs), dim3(256), 0, s, sd.desc, sd.nact);
+  hipLaunchKernelGGL((kl_pass1<DIM, G, kF32Lse, 2>), dim3(gw), dim3(64), 0, s, sd.desc,
+                     b + kLPass32 * L, sd.nact);
+  hipLaunchKernelGGL((kl_pass1<DIM, G, kF64Lse, 0>), dim3(gw), dim3(64), 0, s, sd.desc,
+                     b + kLPass64 * L, sd.nact);
+  hipLaunchKernelGGL((kl_pass1<DIM, G, kF64Lse, 1>), dim3(gw), dim3(64), 0, s, sd.desc,
+                     b + kLPass64 * L, sd.nact);
+  hipLaunchKernelGGL((kl_qfin<DIM, kF32Lse>), dim3(gb), dim3(256), 0, s, sd.desc,
+                     b + kLQfin32 * L, sd.nact);
+  hipLaunchKernelGGL((kl_qfin<DIM, kF64Lse>), dim3(gb), dim3(256), 0, s, sd.desc,
+                     b + kLQfin64 * L, sd.nact);
+  hipLaunchKernelGGL((kl_grad<DIM, kF32Lse, G>), dim3(gw), dim3(64), 0, s, sd.desc,
+                     b + kLGrad32 * L, sd.nact);
+  hipLaunchKernelGGL((kl_grad<DIM, kF64Lse, G>), dim3(gw), dim3(64), 0, s, sd.desc,
+                     b + kLGrad64 * L, sd.nact);
+  hipLaunchKernelGGL((kl_gfin<DIM>), dim3(gb), dim3(256), 0, s, sd.desc, b + kLGfin * L, sd.nact);
+  hipLaunchKernelGGL(kl_final, dim3(gs), dim3(256), 0, s, sd.desc, sd.nact);
+  hipLaunchKernelGGL(kl_dots, dim3(gs), dim3(kBT), 0, s, sd.vecs, sd.in, sd.out, sd.nact);
 }
 
-template <int DIM, int MODE>
+template <int DIM>
 static hipError_t step_g(int G, const ServeDev &sd, hipStream_t s) {
   switch (G) {
-    case 1: step_t<DIM, MODE, 1>(sd, s); break;
-    case 2: step_t<DIM, MODE, 2>(sd, s); break;
-    case 3: step_t<DIM, MODE, 3>(sd, s); break;
-    case 4: step_t<DIM, MODE, 4>(sd, s); break;
-    case 5: step_t<DIM, MODE, 5>(sd, s); break;
-    case 6: step_t<DIM, MODE, 6>(sd, s); break;
-    case 7: step_t<DIM, MODE, 7>(sd, s); break;
-    case 8: step_t<DIM, MODE, 8>(sd, s); break;
-    case 9: step_t<DIM, MODE, 9>(sd, s); break;
+    case 1: step_t<DIM, 1>(sd, s); break;
+    case 2: step_t<DIM, 2>(sd, s); break;
+    case 3: step_t<DIM, 3>(sd, s); break;
+    case 4: step_t<DIM, 4>(sd, s); break;
+    case 5: step_t<DIM, 5>(sd, s); break;
+    case 6: step_t<DIM, 6>(sd, s); break;
+    case 7: step_t<DIM, 7>(sd, s); break;
+    case 8: step_t<DIM, 8>(sd, s); break;
+    case 9: step_t<DIM, 9>(sd, s); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_serve_step(int dim, int G, int mode, const ServeDev &sd, hipStream_t s) {
-  if (dim == 2) {
-    if (mode == kF32Lse) return step_g<2, kF32Lse>(G, sd, s);
-    return step_g<2, kF64Lse>(G, sd, s);
-  }
-  if (mode == kF32Lse) return step_g<3, kF32Lse>(G, sd, s);
-  return step_g<3, kF64Lse>(G, sd, s);
+hipError_t launch_serve_step(int dim, int G, const ServeDev &sd, hipStream_t s) {
+  if (dim == 2) return step_g<2>(G, sd, s);
+  return step_g<3>(G, sd, s);
 }
 
 }  // namespace sdot

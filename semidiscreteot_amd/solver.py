@@ -17,6 +17,7 @@ from . import _lib as L
 _THRESH = {"pointwise": 0, "integral": 1, "geometric": 2}
 _CTRL = {"l1norm": 0, "componentwise": 1}
 _PREC = {"fp64": 0, "fp32_lse": 1}
+_LS = {"reference": 0, "noise_aware": 1}
 
 
 def _f64(a):
@@ -30,7 +31,8 @@ def _p(a):
 @dataclass
 class SolverParameters:
     """SotParameterManager::SolverParameters (ParameterManager.h:56-72) plus
-    `precision` ("fp64" | "fp32_lse")."""
+    `precision` ("fp64" | "fp32_lse") and `line_search` ("reference" |
+    "noise_aware", include/sdot.h)."""
     max_iterations: int = 1000
     tolerance: float = 1.0
     solver_control_type: str = "l1norm"
@@ -47,6 +49,7 @@ class SolverParameters:
     epsilon_scaling_steps: int = 5
     use_log_sum_exp_trick: bool = False
     precision: str = "fp64"
+    line_search: str = "reference"
 
     def to_c(self) -> L.Params:
         p = L.Params()
@@ -66,12 +69,14 @@ class SolverParameters:
         p.epsilon_scaling_steps = int(self.epsilon_scaling_steps)
         p.use_log_sum_exp_trick = int(bool(self.use_log_sum_exp_trick))
         p.precision = _PREC[self.precision]
+        p.line_search = _LS[self.line_search]
         return p
 
     @classmethod
     def from_c(cls, p: L.Params) -> "SolverParameters":
         inv_t = {v: k for k, v in _THRESH.items()}
         inv_p = {v: k for k, v in _PREC.items()}
+        inv_l = {v: k for k, v in _LS.items()}
         return cls(max_iterations=p.max_iterations, tolerance=p.tolerance,
                    solver_control_type="componentwise" if p.solver_control_type == 1 else "l1norm",
                    epsilon=p.epsilon, tau=p.tau,
@@ -82,7 +87,7 @@ class SolverParameters:
                    epsilon_scaling_factor=p.epsilon_scaling_factor,
                    epsilon_scaling_steps=p.epsilon_scaling_steps,
                    use_log_sum_exp_trick=bool(p.use_log_sum_exp_trick),
-                   precision=inv_p[p.precision])
+                   precision=inv_p[p.precision], line_search=inv_l[p.line_search])
 
     @classmethod
     def from_prm(cls, text: str | None = None, path: str | None = None) -> "SolverParameters":

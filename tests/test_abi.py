@@ -66,6 +66,7 @@ subsection SotParameterManager
     set epsilon_scaling_factor = 10
     set epsilon_scaling_steps = 3
     set precision = fp32_lse
+    set line_search = noise_aware
   end
   subsection power_diagram_parameters
     set implementation = geogram
@@ -81,6 +82,7 @@ def test_prm_reader():
     assert p.verbose_output is False and p.quadrature_order == 2
     assert p.use_log_sum_exp_trick is True and p.epsilon_scaling_factor == 10.0
     assert p.epsilon_scaling_steps == 3 and p.precision == "fp32_lse"
+    assert p.line_search == "noise_aware"
 
 
 def test_prm_reader_errors():
@@ -157,3 +159,26 @@ def test_lockstep_line_search_matches_loop_form(tmp_path):
                     "-o", str(exe)], check=True)
     r = subprocess.run([str(exe)], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout
+
+
+def test_served_solve_driver_matches_loop_form(tmp_path):
+    """The step-by-step driver of a served solve, fed by a host emulation of
+    the device actions, reproduces lbfgs_solve bitwise on Rosenbrock, also
+    when evaluations are repeated (tests/serve_driver_check.cpp)."""
+    import subprocess
+    exe = tmp_path / "drv"
+    subprocess.run(["g++", "-std=c++17", "-O2", os.path.join(ROOT, "tests", "serve_driver_check.cpp"),
+                    "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout
+
+
+def test_server_config_defaults():
+    lib = L.load()
+    cfg = L.ServerConfig()
+    assert lib.sdot_server_config_default(C.byref(cfg)) == 0
+    assert cfg.precision == 1 and cfg.max_slots == 256 and cfg.use_graph == 1
+    assert cfg.hbm_budget_bytes == 16 << 30 and cfg.polish_factor == 10.0
+    out = C.c_void_p()
+    assert lib.sdot_server_create(None, C.byref(cfg), C.byref(out)) != 0
+    assert not out.value
