@@ -224,6 +224,12 @@ typedef struct {
   double radius;               /* truncation radius R of the candidate sets; <= 0 or
                                   DBL_MAX: every target is a candidate */
   int32_t line_search;         /* sdot_line_search */
+  /* epsilon scaling as SemiDiscreteOT::run_sot applies it (SemiDiscreteOT.cc:877-918):
+   * solves at epsilon * factor^(steps-1-i), i = 0..steps-1, each to the tolerance,
+   * the potential carried over; steps 0 or 1: a single solve at epsilon;
+   * factor <= 1 is replaced by 2 (EpsilonScalingHandler.cc:18-27) */
+  double epsilon_scaling_factor;
+  uint32_t epsilon_scaling_steps;
 } sdot_request;
 
 typedef struct {

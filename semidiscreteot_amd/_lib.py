@@ -68,7 +68,8 @@ class Request(C.Structure):
     _fields_ = [("n", C.c_int64), ("y", _dp), ("nu", _dp), ("psi0", _dp),
                 ("epsilon", C.c_double), ("tolerance", C.c_double),
                 ("solver_control_type", C.c_int32), ("max_iterations", C.c_uint32),
-                ("radius", C.c_double), ("line_search", C.c_int32)]
+                ("radius", C.c_double), ("line_search", C.c_int32),
+                ("epsilon_scaling_factor", C.c_double), ("epsilon_scaling_steps", C.c_uint32)]
 
 
 class Result(C.Structure):

@@ -210,6 +210,7 @@ struct sdot_ctx {
   bool have_source = false, have_target = false;
   int force_mode = -1;  // overrides the arithmetic mode (the Hessian runs FP64 LSE)
   bool hist_valid = false;  // device holds psi and M_q of a previous FP32 evaluation
+  double hist_R = 0.0;      // its truncation radius
   int n_servers = 0;        // serving queues reading this context's source arrays
   Plan plan;
   Dev dev;
@@ -275,7 +276,8 @@ int prepare(sdot_ctx *c, double R) {
   p.smax = smax_for(R);
   p.dense = (R >= diag * (1.0 + 1e-6) + 1e-300) ? 1 : 0;
   p.Rq = R * (1.0 + 1e-9) + 1e-9 * (p.tg.h + p.cg.h + c->coord_scale);
-  p.hist = (p.mode == kF32Lse && c->hist_valid) ? 1 : 0;
+  // the seed needs the previous evaluation's candidate sets: same R
+  p.hist = (p.mode == kF32Lse && c->hist_valid && c->hist_R == R) ? 1 : 0;
   if (!std::isfinite(p.Rq)) p.dense = std::isnan(R) ? 0 : 1;
   if (std::isnan(R)) p.Rq = 0.0;
   // enough waves to fill 256 CUs: the q-major kernel runs 4 waves per SIMD, the
@@ -297,6 +299,7 @@ int prepare(sdot_ctx *c, double R) {
   if (c->cap_M < needM) {
     HIP_TRY(dalloc(d.Mpart, needM));
     HIP_TRY(dalloc(d.Spart, needM));
+    HIP_TRY(dalloc(d.Tpart, needM));
     c->cap_M = needM;
   }
   if (c->cap_G < needG) {
@@ -360,7 +363,10 @@ int run_eval(sdot_ctx *c, double R, const double *d_psi, double *d_grad, double 
   }
   HIP_TRY(launch_eval(c->plan, c->dev, d_psi, d_grad, d_FC, c->stream, pp));
   // an FP32 evaluation leaves psi and M_q behind to seed the next one
-  if (c->plan.mode == kF32Lse) c->hist_valid = true;
+  if (c->plan.mode == kF32Lse) {
+    c->hist_valid = true;
+    c->hist_R = R;
+  }
   return SDOT_OK;
 }
 
@@ -526,7 +532,7 @@ extern "C" int sdot_ctx_destroy(sdot_ctx *c) {
   void *bufs[] = {d.qx_m, d.qw_m, d.qx2_m, d.cc_m, d.blkbox, d.cc_r, d.cstart, d.cell_r2m,
                   d.cell_m2o, d.ty_o, d.tnu_o, d.tpos_r, d.tpos_m, d.tr2o, d.tm2o, d.tstart,
                   d.twbox, d.ty_m, d.trec_r, d.trec_m, d.qrec_r, d.Mpart, d.Spart, d.Gpart,
-                  d.Fpart, d.Cpart, d.PNpart, d.psi, d.grad, d.out, d.scratch, d.i64scratch,
+                  d.Fpart, d.Cpart, d.PNpart, d.psi, d.grad, d.out, d.scratch, d.i64scratch, d.Tpart,
                   d.Upart, d.psi_prev, d.Mhist_m, d.DMpart, d.Dmpart, d.dstat};
   for (void *b : bufs)
     if (b) (void)hipFree(b);

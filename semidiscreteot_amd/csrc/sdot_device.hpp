@@ -112,14 +112,15 @@ __device__ __forceinline__ double block_max_256(double v, double *sh) {
   return r;
 }
 
-// Exponents may exceed the online reference value M by this much (log2
-// units) before M is raised.  The FP32 argument t - M carries an absolute
-// rounding error of about |t - M| 2^-24, i.e. a relative error of
-// |t - M| 2^-24 ln 2 in each term, so the dominant terms must stay within a few
-// units of M: slack 2 and a seed at most 3 units high keep it near 1.2e-7.
-constexpr double kOnlineSlack = 2.0;
-// A history seed sits at most (spread of the psi change) + 1 units above the
-// new maxima; above this spread the sweep starts unseeded.
+// FP32 sweep (PASS 2).  Every exponent t is taken relative to a reference
+// value M that bounds the sweep's largest exponent from above, so that the
+// dominant terms have t - M in [-(spread + 1), 0] and the FP32 argument keeps
+// its precision (the rounding of t - M to FP32 costs |t - M| 2^-25 ln 2 of
+// relative error per term).  M is either the exact maximum from a PASS 0 sweep
+// or, when the previous evaluation of the same candidate sets is available,
+// ceil(previous maximum + largest psi increase * coef): an upper bound on the
+// new maximum, at most (spread of the psi change) * coef + 1 above it.  The
+// seed is used while that spread stays below kSeedSpread log2 units.
 constexpr double kSeedSpread = 2.0;
 
 // Target record (64 bytes):
@@ -225,29 +226,30 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
   const int64_t cell = gg / a.gpc;
   const int64_t q0 = cell * a.nq + (gg - cell * a.gpc) * G;
 
-  double c[DIM], x[G][DIM], M[G], S[G];
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) c[d] = a.cc[cell * DIM + d];
-  // history seed of the online sweep: the previous evaluation's M_q raised by
-  // the largest psi increase bounds every new exponent of the targets that were
-  // candidates before; skipped when the psi change spread could underflow
+  // FP32 history seed (see kSeedSpread): decided per solve from the psi change,
+  // identically in PASS 0 (which is then skipped) and PASS 2
   bool seed = false;
   double shift = 0.0;
-  if (PASS == 2 && a.hist) {
+  if (MODE == kF32Lse && a.hist) {
     const double dmx = a.dstat[0], dmn = a.dstat[1];
     seed = (dmx - dmn) * a.coef < kSeedSpread;
     shift = dmx * a.coef;
   }
+  if (PASS == 0 && seed) return;
+
+  double c[DIM], x[G][DIM], M[G], S[G], T[G];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) c[d] = a.cc[cell * DIM + d];
 #pragma unroll
   for (int k = 0; k < G; ++k) {
 #pragma unroll
     for (int d = 0; d < DIM; ++d) x[k][d] = a.qx[(q0 + k) * DIM + d];
     S[k] = 0.0;
+    T[k] = -INFINITY;
     if (PASS == 0) {
       M[k] = -INFINITY;
-    } else if (PASS == 2) {
-      M[k] = seed ? ceil(a.Mhist[q0 + k] + shift) : -INFINITY;
-      if (!(M[k] < INFINITY)) M[k] = -INFINITY;
+    } else if (PASS == 2 && seed) {
+      M[k] = ceil(a.Mhist[q0 + k] + shift);
     } else if (MODE == kF64Plain) {
       M[k] = a.plain_coef * a.qx2[q0 + k];
     } else {
@@ -281,36 +283,15 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
         S[k] = fma(r.nu, expo<MODE>(t - M[k]), S[k]);
       }
     } else {
-      double dl[G];
-      bool over = false;
+      // exponent relative to the bound M (<= 0 up to rounding), running maximum
 #pragma unroll
       for (int k = 0; k < G; ++k) {
         double t = r.a;
 #pragma unroll
         for (int d = 0; d < DIM; ++d) t = fma(x[k][d], r.yl[d], t);
-        dl[k] = t - M[k];
-        over |= dl[k] > kOnlineSlack;
+        T[k] = fmax(T[k], t);
+        S[k] = fma(r.nu, (double)__builtin_amdgcn_exp2f((float)(t - M[k])), S[k]);
       }
-      // wave-uniform branch, taken rarely: some exponent exceeds its reference
-      // value by more than the slack; raise the value to the next integer above
-      // the exponent, so the rescale of the running sum is an exact power of two
-      if (__any(over)) {
-#pragma unroll
-        for (int k = 0; k < G; ++k) {
-          if (dl[k] > kOnlineSlack) {
-            double t = r.a;  // recomputed: dl + M is NaN while M is still -inf
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) t = fma(x[k][d], r.yl[d], t);
-            const double Mn = ceil(t);
-            S[k] = (M[k] > -INFINITY) ? ldexp(S[k], (int)(M[k] - Mn)) : 0.0;
-            M[k] = Mn;
-            dl[k] = t - Mn;
-          }
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < G; ++k)
-        S[k] = fma(r.nu, (double)__builtin_amdgcn_exp2f((float)dl[k]), S[k]);
     }
   };
   cdouble *trec = cptr(a.trec);
@@ -336,11 +317,12 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
 #pragma unroll
     for (int k = 0; k < G; ++k) {
       const int64_t o = (int64_t)seg * a.nQ + q0 + k;
-      if (PASS == 0) a.out[o] = M[k];
-      if (PASS == 1) a.out[o] = S[k];
+      if (PASS == 0) a.Mout[o] = M[k];
+      if (PASS == 1) a.Sout[o] = S[k];
       if (PASS == 2) {
-        a.out[o] = M[k];
-        a.out2[o] = S[k];
+        a.Mout[o] = M[k];
+        a.Sout[o] = S[k];
+        a.Tout[o] = T[k];
       }
     }
   }
@@ -370,17 +352,17 @@ __device__ __forceinline__ void qfin_body(const QFArgs &a, int64_t blk, double *
       for (int s = 0; s < a.K1; ++s) M = fmax(M, a.Mpart[(int64_t)s * a.nQ + qm]);
       for (int s = 0; s < a.K1; ++s) S += a.Spart[(int64_t)s * a.nQ + qm];
     } else {
-      // segments carry their own reference values: S = sum S_s 2^(M_s - M)
-      M = -INFINITY;
-      for (int s = 0; s < a.K1; ++s) M = fmax(M, a.Mpart[(int64_t)s * a.nQ + qm]);
-      if (M > -INFINITY)
-        for (int s = 0; s < a.K1; ++s) {
-          const double ss = a.Spart[(int64_t)s * a.nQ + qm];
-          if (ss > 0.0) S += ldexp(ss, (int)(a.Mpart[(int64_t)s * a.nQ + qm] - M));
-        }
-      a.Mhist[qm] = M;
-      // every term far below the reference value: the FP32 exponentials lost
-      // their range; the host repeats the evaluation without the seed
+      // every segment of q used the same reference value M
+      M = a.Mpart[qm];
+      double T = -INFINITY;
+      for (int s = 0; s < a.K1; ++s) {
+        S += a.Spart[(int64_t)s * a.nQ + qm];
+        T = fmax(T, a.Tpart[(int64_t)s * a.nQ + qm]);
+      }
+      if (!(T > -INFINITY)) M = -INFINITY;  // no candidates
+      a.Mhist[qm] = T;
+      // safety net: every term far below the reference value (cannot happen
+      // while the seed bound holds); the host repeats the evaluation unseeded
       if (M > -INFINITY && S < 1e-40) Uq = 1.0;
     }
     const double w = a.qw[qm];

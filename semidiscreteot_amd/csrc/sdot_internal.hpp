@@ -72,6 +72,7 @@ struct Dev {
   double *qrec_r = nullptr;    // [nQ*kRec] row cell order
   double *Mpart = nullptr;     // [K1*nQ]
   double *Spart = nullptr;     // [K1*nQ]
+  double *Tpart = nullptr;     // [K1*nQ] FP32: segment maxima of the exponent
   double *Gpart = nullptr;     // [K2*N]
   double *Fpart = nullptr;     // [nFpart]
   double *Cpart = nullptr;     // [nFpart]
@@ -132,11 +133,12 @@ struct P1Args {
   const double *blkbox;  // [n_blocks*2*DIM]
   const double *trec;    // [N*kRec] row order
   const int32_t *tstart;
-  const double *Mpart;   // [K1*nQ]  read by PASS 1
-  const double *Mhist;   // [nQ]     PASS 2: M of the previous evaluation
-  const double *dstat;   // PASS 2: max / min psi change since then
-  double *out;           // [K1*nQ]  M (PASS 0/2) or S (PASS 1)
-  double *out2;          // [K1*nQ]  S (PASS 2)
+  const double *Mpart;   // [K1*nQ]  maxima of PASS 0, read by PASS 1 / 2
+  const double *Mhist;   // [nQ]     FP32: largest exponent of the previous evaluation
+  const double *dstat;   // FP32: max / min psi change since then
+  double *Mout;          // [K1*nQ]  PASS 0: segment maxima; PASS 2: reference value used
+  double *Sout;          // [K1*nQ]  PASS 1 / 2: segment sums
+  double *Tout;          // [K1*nQ]  PASS 2: segment maxima of the exponent
   Grid tg;
   int64_t n_groups, nQ, N;
   int nq, gpc, K1, dense, hist;
@@ -144,7 +146,7 @@ struct P1Args {
 };
 
 struct QFArgs {
-  const double *qx, *qw, *qx2, *Mpart, *Spart;
+  const double *qx, *qw, *qx2, *Mpart, *Spart, *Tpart;
   const int64_t *cell_r2m;
   double *qrec_r, *Fpart, *Cpart, *Upart, *Mhist;
   int64_t nQ;

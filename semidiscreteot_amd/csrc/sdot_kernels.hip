@@ -10,12 +10,12 @@
 //   k_pass1    q-major sweep over each cell's candidates.  Lane = group of G
 //              quadrature points of one cell; wave = 64 Morton-consecutive groups;
 //              grid.y splits the candidate bin rows into K1 segments.
-//                PASS 0: max exponent (FP64 modes),
+//                PASS 0: max exponent (FP64 LSE; FP32 when no history seed),
 //                PASS 1: sum_j nu_j exp(t_qj - M_q) with the global max (FP64 modes),
-//                PASS 2: one online sweep (FP32 mode): per segment an integer
-//                        reference value M and S = sum nu exp2(t - M); M is raised
-//                        (rarely) to ceil(t) when a term would exceed it, which
-//                        rescales S by an exact power of two.
+//                PASS 2: FP32 sweep: S = sum nu exp2(t - M) against an upper
+//                        bound M of the exponents (the exact maximum, or the
+//                        previous evaluation's maximum raised by the largest
+//                        psi increase), and the running maximum of t.
 //   k_qfin     per q: merge segments in fixed order, F_q, w_q/S_q, C_q, the q record
 //              the gradient pass streams; block partials of F and C.
 //   k_grad     target-major: lane = target, loops over the cells near its wave and
@@ -384,21 +384,21 @@ static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_g
   a.Rq = p.Rq;
   a.plain_coef = 0.5 / p.eps;
   a.coef = coef;
+  a.Mout = d.Mpart;
+  a.Sout = d.Spart;
+  a.Tout = d.Tpart;
   hipError_t e;
-  if (MODE == kF64Lse) {
-    a.out = d.Mpart;
+  // exact maxima (FP64 LSE always; FP32 when the history seed is unusable:
+  // the kernel then returns at once)
+  if (MODE != kF64Plain) {
     e = launch_pass1_g<DIM, MODE, 0>(p.G, a, p.n_blocks, p.K1, s);
     if (e != hipSuccess) return e;
   }
   if (timing) (void)hipEventRecord(prof->ev[1], s);
-  if (MODE == kF32Lse) {
-    a.out = d.Mpart;
-    a.out2 = d.Spart;
+  if (MODE == kF32Lse)
     e = launch_pass1_g<DIM, MODE, 2>(p.G, a, p.n_blocks, p.K1, s);
-  } else {
-    a.out = d.Spart;
+  else
     e = launch_pass1_g<DIM, MODE, 1>(p.G, a, p.n_blocks, p.K1, s);
-  }
   if (e != hipSuccess) return e;
   if (timing) (void)hipEventRecord(prof->ev[2], s);
 
@@ -408,6 +408,7 @@ static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_g
   q.qx2 = d.qx2_m;
   q.Mpart = d.Mpart;
   q.Spart = d.Spart;
+  q.Tpart = d.Tpart;
   q.cell_r2m = d.cell_r2m;
   q.qrec_r = d.qrec_r;
   q.Fpart = d.Fpart;

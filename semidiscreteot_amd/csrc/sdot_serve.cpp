@@ -121,7 +121,7 @@ struct SolveShape {
 
 struct SolveOffsets {
   size_t y, nu, pos_r, pos_m, tm2o, tstart, twbox;
-  size_t trec_r, trec_m, qrec_r, Mpart, Spart, Gpart;
+  size_t trec_r, trec_m, qrec_r, Mpart, Spart, Tpart, Gpart;
   size_t Fpart, Cpart, Upart, PNpart, DMpart, Dmpart, dstat, psi_prev, Mhist;
   size_t x, g, x0, gv, p, S[kHist], Y[kHist], hist;
   size_t total;
@@ -143,6 +143,7 @@ SolveOffsets carve(const SourceView &src, const SolveShape &s) {
   o.qrec_r = c.take<double>((size_t)src.nQ * kRec);
   o.Mpart = c.take<double>((size_t)s.K1 * src.nQ);
   o.Spart = c.take<double>((size_t)s.K1 * src.nQ);
+  o.Tpart = s.fp32 ? c.take<double>((size_t)s.K1 * src.nQ) : SIZE_MAX;
   o.Gpart = c.take<double>((size_t)s.K2 * s.N);
   o.Fpart = c.take<double>(nFpart);
   o.Cpart = c.take<double>(nFpart);
@@ -169,6 +170,7 @@ struct Request {
   int64_t id = 0;
   int64_t N = 0;
   double eps = 1.0, tol = 0.0, R = INFINITY;
+  std::vector<double> eps_seq;  // epsilon schedule, largest first (last = eps)
   int control = SDOT_CTRL_L1NORM;
   int line_search = SDOT_LS_REFERENCE;
   uint32_t max_it = 0;
@@ -190,6 +192,8 @@ struct Solve {
   int64_t items[kNumLists] = {};
   int mode = kF32Lse;   // arithmetic of this solve's evaluations
   SolveDriver drv;
+  size_t stage = 0;     // index into the epsilon schedule
+  int it_done = 0, ev_done = 0;  // iterations / evaluations of finished stages
   bool seeded = false;  // FP32: the next evaluation starts from the history seed
   double t_admit = 0.0;
 };
@@ -285,8 +289,9 @@ void build_desc(sdot_server *s, Solve &v) {
   const SourceView &src = s->src;
   const Request &r = *v.req;
   const SolveShape &sh = r.shape;
+  const double eps = r.eps_seq[v.stage];
   const bool fp32 = v.mode == kF32Lse;
-  const double coef = (1.0 / r.eps) * (fp32 ? 1.44269504088896340736 : 1.0);
+  const double coef = (1.0 / eps) * (fp32 ? 1.44269504088896340736 : 1.0);
   const double ln_scale = fp32 ? 0.69314718055994530942 : 1.0;
   double *trec_r = dptr<double>(s, v, v.o.trec_r), *trec_m = dptr<double>(s, v, v.o.trec_m);
   double *qrec_r = dptr<double>(s, v, v.o.qrec_r);
@@ -327,10 +332,9 @@ void build_desc(sdot_server *s, Solve &v) {
   a.Mpart = Mpart;
   a.Mhist = Mhist;
   a.dstat = dstat;
-  // FP32: PASS 2 writes M to out and S to out2; FP64: PASS 0 writes M to
-  // out2, PASS 1 writes S to out
-  a.out = fp32 ? Mpart : Spart;
-  a.out2 = fp32 ? Spart : Mpart;
+  a.Mout = Mpart;
+  a.Sout = Spart;
+  a.Tout = dptr<double>(s, v, v.o.Tpart);
   a.tg = r.L.tg;
   a.n_groups = src.n_groups;
   a.nQ = src.nQ;
@@ -342,7 +346,7 @@ void build_desc(sdot_server *s, Solve &v) {
   a.hist = v.seeded ? 1 : 0;
   a.smax = r.smax;
   a.Rq = r.Rq;
-  a.plain_coef = 0.5 / r.eps;
+  a.plain_coef = 0.5 / eps;
   a.coef = coef;
 
   QFArgs &q = v.desc.qf;
@@ -352,6 +356,7 @@ void build_desc(sdot_server *s, Solve &v) {
   q.qx2 = src.qx2_m;
   q.Mpart = Mpart;
   q.Spart = Spart;
+  q.Tpart = dptr<double>(s, v, v.o.Tpart);
   q.cell_r2m = src.cell_r2m;
   q.qrec_r = qrec_r;
   q.Fpart = Fpart;
@@ -361,8 +366,8 @@ void build_desc(sdot_server *s, Solve &v) {
   q.nQ = src.nQ;
   q.nq = src.nq;
   q.K1 = sh.K1;
-  q.eps = r.eps;
-  q.plain_coef = 0.5 / r.eps;
+  q.eps = eps;
+  q.plain_coef = 0.5 / eps;
   q.ln_scale = ln_scale;
 
   GArgs &gr = v.desc.gr;
@@ -409,17 +414,31 @@ void build_desc(sdot_server *s, Solve &v) {
   v.items[kLGfin] = (r.N + 255) / 256;
 }
 
+const SlotHist kEmptyHist = [] {
+  SlotHist h{};
+  h.count = 0;
+  h.newest = -1;
+  return h;
+}();
+
+void init_driver(sdot_server *s, Solve &v) {
+  const Request &r = *v.req;
+  v.mode = s->mode;
+  v.seeded = false;
+  const bool aware = r.line_search == SDOT_LS_NOISE_AWARE;
+  const double noise_rel = aware ? (v.mode == kF32Lse ? kNoiseRelF32 : kNoiseRelF64) : 0.0;
+  v.drv = SolveDriver();
+  v.drv.init(r.control, r.tol, r.max_it, noise_rel, v.mode == kF32Lse ? s->polish : 0.0,
+             aware ? kNoiseRelF64 : 0.0);
+}
+
 // Uploads an admitted request into its arena block (offset `off`).
 int admit(sdot_server *s, std::unique_ptr<Request> r, const SolveOffsets &o, size_t off) {
   auto v = std::make_unique<Solve>();
   v->o = o;
   v->off = off;
   v->req = std::move(r);
-  v->mode = s->mode;
-  const bool aware = v->req->line_search == SDOT_LS_NOISE_AWARE;
-  const double noise_rel = aware ? (v->mode == kF32Lse ? kNoiseRelF32 : kNoiseRelF64) : 0.0;
-  v->drv.init(v->req->control, v->req->tol, v->req->max_it, noise_rel,
-              v->mode == kF32Lse ? s->polish : 0.0, aware ? kNoiseRelF64 : 0.0);
+  init_driver(s, *v);
   v->t_admit = s->now();
   Request &q = *v->req;
   const TargetLayout &L = q.L;
@@ -435,13 +454,7 @@ int admit(sdot_server *s, std::unique_ptr<Request> r, const SolveOffsets &o, siz
   SV_TRY(up(v->o.tstart, L.tstart.data(), sizeof(int32_t) * L.tstart.size()));
   SV_TRY(up(v->o.twbox, L.twbox.data(), sizeof(double) * L.twbox.size()));
   SV_TRY(up(v->o.x, q.psi0.data(), sizeof(double) * q.N));
-  static const SlotHist h0 = [] {
-    SlotHist h{};
-    h.count = 0;
-    h.newest = -1;
-    return h;
-  }();
-  SV_TRY(up(v->o.hist, &h0, sizeof(SlotHist)));
+  SV_TRY(up(v->o.hist, &kEmptyHist, sizeof(SlotHist)));
   build_desc(s, *v);
   s->active.push_back(std::move(v));
   s->dirty = true;
@@ -489,8 +502,8 @@ int retire(sdot_server *s, size_t i, int status) {
   SV_TRY(hipStreamSynchronize(s->stream));
   d.res.id = r.id;
   d.res.status = status;
-  d.res.iterations = v.drv.iterations();
-  d.res.evaluations = v.drv.evaluations();
+  d.res.iterations = v.it_done + v.drv.iterations();
+  d.res.evaluations = v.ev_done + v.drv.evaluations();
   d.res.functional = v.drv.functional();
   d.res.check_value = v.drv.check_value();
   d.res.rel_marginal_error = v.drv.relerr();
@@ -627,9 +640,6 @@ int server_init(sdot_server *s, const sdot_server_config &cfg) {
   SV_TRY(hipMemcpy(s->d_block, s->h_block, s->block_bytes, hipMemcpyHostToDevice));
   SV_TRY(hipMalloc((void **)&s->d_out, sizeof(SlotOut) * s->max_slots));
   SV_TRY(hipHostMalloc((void **)&s->h_out, sizeof(SlotOut) * s->max_slots, hipHostMallocDefault));
-  int cus = 256;
-  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device);
-  cus = std::max(cus, 1);
   ServeDev &sd = s->sd;
   sd.desc = reinterpret_cast<SlotDesc *>(s->d_block + s->off_desc);
   sd.vecs = reinterpret_cast<SlotVecs *>(s->d_block + s->off_vecs);
@@ -638,9 +648,7 @@ int server_init(sdot_server *s, const sdot_server_config &cfg) {
   sd.in = reinterpret_cast<SlotIn *>(s->d_block + s->off_in);
   sd.nact = reinterpret_cast<int *>(s->d_block + s->off_nact);
   sd.out = s->d_out;
-  // one-wave kernels run up to 16 waves per CU, the 256-thread ones 8 blocks
-  sd.grid_w = (unsigned)cus * 16;
-  sd.grid_b = (unsigned)cus * 8;
+  SV_TRY(serve_grid_sizes(s->src.dim, s->src.G, s->device, sd));
   sd.grid_slots = (unsigned)s->max_slots;
   if (s->use_graph) {
     SV_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
@@ -695,6 +703,16 @@ extern "C" int sdot_server_submit(sdot_server *s, const sdot_request *q, int64_t
     return set_error(SDOT_ERR_INVALID, "unknown line_search");
   r->line_search = q->line_search;
   r->R = (q->radius <= 0.0 || q->radius >= DBL_MAX || std::isnan(q->radius)) ? INFINITY : q->radius;
+  // epsilon schedule of EpsilonScalingHandler::generate_epsilon_sequence
+  // (EpsilonScalingHandler.cc:30-41; factor <= 1 -> 2, steps 0 -> 1 as its
+  // constructor does, :18-27)
+  {
+    uint32_t steps = q->epsilon_scaling_steps == 0 ? 1 : q->epsilon_scaling_steps;
+    const double factor = q->epsilon_scaling_factor > 1.0 ? q->epsilon_scaling_factor : 2.0;
+    if (steps > 64) return set_error(SDOT_ERR_INVALID, "epsilon_scaling_steps must be <= 64");
+    for (uint32_t i = 0; i < steps; ++i)
+      r->eps_seq.push_back(q->epsilon * std::pow(factor, (double)(steps - 1 - i)));
+  }
   r->y.assign(q->y, q->y + q->n * dim);
   r->nu.assign(q->nu, q->nu + q->n);
   for (double v : r->y)
@@ -754,7 +772,24 @@ extern "C" int sdot_server_step(sdot_server *s, int32_t *n_active, int32_t *n_qu
     SV_TRY(hipStreamSynchronize(s->stream));
     SV_TRY(hipEventElapsedTime(&s->last_step_ms, s->ev0, s->ev1));
     std::vector<int> status(n);
-    for (int i = 0; i < n; ++i) status[i] = advance(s, *s->active[i], s->h_out[i]);
+    for (int i = 0; i < n; ++i) {
+      Solve &v = *s->active[i];
+      status[i] = advance(s, v, s->h_out[i]);
+      if (status[i] >= 0 && v.stage + 1 < v.req->eps_seq.size()) {
+        // epsilon scaling: the next stage starts from this potential with a
+        // fresh L-BFGS history, whatever this stage's outcome
+        // (SemiDiscreteOT.cc:888-917)
+        v.it_done += v.drv.iterations();
+        v.ev_done += v.drv.evaluations();
+        ++v.stage;
+        init_driver(s, v);
+        build_desc(s, v);
+        SV_TRY(hipMemcpyAsync(s->arena.at(v.off + v.o.hist), &kEmptyHist, sizeof(SlotHist),
+                              hipMemcpyHostToDevice, s->stream));
+        s->dirty = true;
+        status[i] = -1;
+      }
+    }
     for (int i = n - 1; i >= 0; --i)
       if (status[i] >= 0) {
         rc = retire(s, (size_t)i, status[i]);

@@ -53,11 +53,15 @@ struct ServeDev {
   SlotVecs *vecs;
   SlotIn *in;
   SlotOut *out;
-  unsigned grid_w;      // persistent grid of the one-wave work-list kernels
-  unsigned grid_b;      // persistent grid of the 256-thread work-list kernels
+  // persistent grids: as many workgroups as the device holds at once
+  unsigned grid_pass;   // q-major sweeps (one wave each)
+  unsigned grid_grad;   // target-major sweeps (one wave each)
+  unsigned grid_b;      // 256-thread work-list kernels
   unsigned grid_slots;  // one workgroup per solve (upper bound on the batch)
 };
 
 hipError_t launch_serve_step(int dim, int G, const ServeDev &sd, hipStream_t s);
+// Sets the persistent grid sizes from the occupancy of the step's kernels.
+hipError_t serve_grid_sizes(int dim, int G, int device, ServeDev &sd);
 
 }  // namespace sdot

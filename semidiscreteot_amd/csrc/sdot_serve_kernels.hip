@@ -15,6 +15,8 @@
 // search (sdot_bfgs.hpp LineSearchSM) consumes.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "sdot_device.hpp"
 #include "sdot_serve.hpp"
 
@@ -64,8 +66,7 @@ __global__ __launch_bounds__(64) void kl_pass1(const SlotDesc *__restrict__ desc
   const int64_t total = base[n];
   for (int64_t it = blockIdx.x; it < total; it += gridDim.x) {
     const int s = find_item(base, n, it);
-    P1Args a = desc[s].p1;
-    if (PASS == 0) a.out = a.out2;  // max sweep writes the M partials
+    const P1Args a = desc[s].p1;
     const int64_t loc = it - base[s];
     const int64_t blk = loc / a.K1;
     pass1_body<DIM, G, MODE, PASS>(a, blk, (int)(loc - blk * a.K1));
@@ -274,25 +275,27 @@ __global__ __launch_bounds__(kBT) void kl_dots(const SlotVecs *__restrict__ vecs
 // ---------------------------------------------------------------------------
 template <int DIM, int G>
 static void step_t(const ServeDev &sd, hipStream_t s) {
-  const unsigned gw = sd.grid_w, gb = sd.grid_b, gs = sd.grid_slots;
+  const unsigned gp = sd.grid_pass, gg = sd.grid_grad, gb = sd.grid_b, gs = sd.grid_slots;
   const int64_t *b = sd.base;
   const int64_t L = sd.bstride;
   hipLaunchKernelGGL(kl_action, dim3(gs), dim3(kBT), 0, s, sd.vecs, sd.in, sd.out, sd.nact);
   hipLaunchKernelGGL((kl_prep<DIM>), dim3(gb), dim3(256), 0, s, sd.desc, b + kLPrep * L, sd.nact);
   hipLaunchKernelGGL(kl_dred, dim3(gs), dim3(256), 0, s, sd.desc, sd.nact);
-  hipLaunchKernelGGL((kl_pass1<DIM, G, kF32Lse, 2>), dim3(gw), dim3(64), 0, s, sd.desc,
+  hipLaunchKernelGGL((kl_pass1<DIM, G, kF32Lse, 0>), dim3(gp), dim3(64), 0, s, sd.desc,
                      b + kLPass32 * L, sd.nact);
-  hipLaunchKernelGGL((kl_pass1<DIM, G, kF64Lse, 0>), dim3(gw), dim3(64), 0, s, sd.desc,
+  hipLaunchKernelGGL((kl_pass1<DIM, G, kF32Lse, 2>), dim3(gp), dim3(64), 0, s, sd.desc,
+                     b + kLPass32 * L, sd.nact);
+  hipLaunchKernelGGL((kl_pass1<DIM, G, kF64Lse, 0>), dim3(gp), dim3(64), 0, s, sd.desc,
                      b + kLPass64 * L, sd.nact);
-  hipLaunchKernelGGL((kl_pass1<DIM, G, kF64Lse, 1>), dim3(gw), dim3(64), 0, s, sd.desc,
+  hipLaunchKernelGGL((kl_pass1<DIM, G, kF64Lse, 1>), dim3(gp), dim3(64), 0, s, sd.desc,
                      b + kLPass64 * L, sd.nact);
   hipLaunchKernelGGL((kl_qfin<DIM, kF32Lse>), dim3(gb), dim3(256), 0, s, sd.desc,
                      b + kLQfin32 * L, sd.nact);
   hipLaunchKernelGGL((kl_qfin<DIM, kF64Lse>), dim3(gb), dim3(256), 0, s, sd.desc,
                      b + kLQfin64 * L, sd.nact);
-  hipLaunchKernelGGL((kl_grad<DIM, kF32Lse, G>), dim3(gw), dim3(64), 0, s, sd.desc,
+  hipLaunchKernelGGL((kl_grad<DIM, kF32Lse, G>), dim3(gg), dim3(64), 0, s, sd.desc,
                      b + kLGrad32 * L, sd.nact);
-  hipLaunchKernelGGL((kl_grad<DIM, kF64Lse, G>), dim3(gw), dim3(64), 0, s, sd.desc,
+  hipLaunchKernelGGL((kl_grad<DIM, kF64Lse, G>), dim3(gg), dim3(64), 0, s, sd.desc,
                      b + kLGrad64 * L, sd.nact);
   hipLaunchKernelGGL((kl_gfin<DIM>), dim3(gb), dim3(256), 0, s, sd.desc, b + kLGfin * L, sd.nact);
   hipLaunchKernelGGL(kl_final, dim3(gs), dim3(256), 0, s, sd.desc, sd.nact);
@@ -314,6 +317,46 @@ static hipError_t step_g(int G, const ServeDev &sd, hipStream_t s) {
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+// Workgroups of `kernel` resident per CU (occupancy), at least 1.
+template <class K>
+static int resident(K kernel, int threads) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, threads, 0) != hipSuccess) n = 1;
+  return n > 0 ? n : 1;
+}
+
+template <int DIM, int G>
+static void grids_t(unsigned cus, ServeDev &sd) {
+  // the smallest occupancy of the kernels sharing a grid size, so that every
+  // persistent workgroup is resident from the start
+  int pass = resident(kl_pass1<DIM, G, kF32Lse, 2>, 64);
+  pass = std::min(pass, resident(kl_pass1<DIM, G, kF32Lse, 0>, 64));
+  pass = std::min(pass, resident(kl_pass1<DIM, G, kF64Lse, 0>, 64));
+  pass = std::min(pass, resident(kl_pass1<DIM, G, kF64Lse, 1>, 64));
+  int grad = std::min(resident(kl_grad<DIM, kF32Lse, G>, 64), resident(kl_grad<DIM, kF64Lse, G>, 64));
+  int b = std::min(resident(kl_prep<DIM>, 256), resident(kl_qfin<DIM, kF32Lse>, 256));
+  b = std::min(b, std::min(resident(kl_qfin<DIM, kF64Lse>, 256), resident(kl_gfin<DIM>, 256)));
+  // multiples of 8 keep a persistent workgroup's items on one XCD's segments
+  sd.grid_pass = cus * (unsigned)pass;
+  sd.grid_grad = cus * (unsigned)grad;
+  sd.grid_b = cus * (unsigned)b;
+}
+
+hipError_t serve_grid_sizes(int dim, int G, int device, ServeDev &sd) {
+  int cus = 0;
+  hipError_t e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+  if (e != hipSuccess) return e;
+  cus = std::max(8, cus - cus % 8);
+#define SDOT_GRIDS(D, GG) \
+  if (dim == D && G == GG) { grids_t<D, GG>((unsigned)cus, sd); return hipSuccess; }
+  SDOT_GRIDS(2, 1) SDOT_GRIDS(2, 2) SDOT_GRIDS(2, 3) SDOT_GRIDS(2, 4) SDOT_GRIDS(2, 5)
+  SDOT_GRIDS(2, 6) SDOT_GRIDS(2, 7) SDOT_GRIDS(2, 8) SDOT_GRIDS(2, 9)
+  SDOT_GRIDS(3, 1) SDOT_GRIDS(3, 2) SDOT_GRIDS(3, 3) SDOT_GRIDS(3, 4) SDOT_GRIDS(3, 5)
+  SDOT_GRIDS(3, 6) SDOT_GRIDS(3, 7) SDOT_GRIDS(3, 8) SDOT_GRIDS(3, 9)
+#undef SDOT_GRIDS
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_serve_step(int dim, int G, const ServeDev &sd, hipStream_t s) {

@@ -84,7 +84,8 @@ class SotServer:
     def submit(self, y, nu, epsilon: float, tolerance: float = 1e-6,
                control: str = "componentwise", max_iterations: int = 1000,
                radius: float | None = None, psi0=None,
-               line_search: str = "noise_aware") -> int:
+               line_search: str = "noise_aware", epsilon_scaling_steps: int = 0,
+               epsilon_scaling_factor: float = 2.0) -> int:
         y, nu = _f64(y), _f64(nu)
         if y.size != nu.size * self.dim:
             raise ValueError(f"Dimension mismatch: {y.size // self.dim} points, {nu.size} weights")
@@ -103,6 +104,8 @@ class SotServer:
         q.max_iterations = int(max_iterations)
         q.radius = 0.0 if radius is None else float(radius)
         q.line_search = _LS[line_search]
+        q.epsilon_scaling_steps = int(epsilon_scaling_steps)
+        q.epsilon_scaling_factor = float(epsilon_scaling_factor)
         rid = C.c_int64(-1)
         L.check(self.lib.sdot_server_submit(self._h, C.byref(q), C.byref(rid)))
         self._n[rid.value] = nu.size

@@ -197,6 +197,32 @@ class SotSolver:
             potential[:] = psi
         return psi
 
+    def solve_epsilon_scaling(self, potential=None, params: SolverParameters | None = None):
+        """The epsilon-scaling loop SemiDiscreteOT::run_sot wraps around
+        SotSolver::solve (SemiDiscreteOT.cc:877-918): solves at
+        epsilon * factor^(steps-1-i), i = 0..steps-1 (the schedule of
+        EpsilonScalingHandler.cc:30-41; factor <= 1 -> 2 and steps 0 -> 1 as
+        its constructor, :18-27), each to the tolerance, carrying the potential
+        over; a stage that does not converge does not stop the loop.  Without
+        use_epsilon_scaling it is a single solve.  Returns (psi, [stage info])."""
+        if params is not None:
+            self.configure(params)
+        p = self.current_params
+        steps = max(1, int(p.epsilon_scaling_steps)) if p.use_epsilon_scaling else 1
+        factor = p.epsilon_scaling_factor if p.epsilon_scaling_factor > 1.0 else 2.0
+        eps_final = p.epsilon
+        psi = None if potential is None else _f64(potential).copy()
+        infos = []
+        try:
+            for i in range(steps):
+                stage = SolverParameters(**{f.name: getattr(p, f.name) for f in fields(p)})
+                stage.epsilon = eps_final * factor ** (steps - 1 - i)
+                psi = self.solve(psi, stage)
+                infos.append(dict(self.get_last_solve_info(), epsilon=stage.epsilon))
+        finally:
+            self.configure(p)
+        return psi, infos
+
     # ---- getters (SotSolver.h:240-262) ----
     def _state(self):
         st = L.State()

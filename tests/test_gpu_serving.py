@@ -233,3 +233,32 @@ def test_server_autotune():
         rid = srv.submit(y, nu, 1e-2, 1e-6, "componentwise", 300)
         res, psi = srv.run([rid])[rid]
         assert res.status == "converged"
+
+
+def test_server_epsilon_scaling_matches_run_sot_loop():
+    """Epsilon scaling (SemiDiscreteOT.cc:877-918): the served request runs the
+    same stages as SotSolver.solve_epsilon_scaling; both converge at the final
+    epsilon with the tolerance met in FP64."""
+    say("epsilon scaling")
+    src, s = source()
+    y, nu = request(71, 800)
+    eps, tol, steps, factor = 3e-3, 1e-6, 4, 2.0
+    s.setup_target(y, nu)
+    prm = sd.SolverParameters(epsilon=eps, tau=0.0, tolerance=tol,
+                              solver_control_type="componentwise", max_iterations=3000,
+                              use_log_sum_exp_trick=True, line_search="noise_aware",
+                              use_epsilon_scaling=True, epsilon_scaling_steps=steps,
+                              epsilon_scaling_factor=factor)
+    psi1, infos = s.solve_epsilon_scaling(None, prm)
+    assert [i["epsilon"] for i in infos] == [eps * 8, eps * 4, eps * 2, eps]
+    it1 = sum(i["iterations"] for i in infos)
+    with sd.SotServer(s, precision="fp64", max_slots=4, hbm_budget_bytes=1 << 30) as srv:
+        rid = srv.submit(y, nu, eps, tol, "componentwise", 3000, epsilon_scaling_steps=steps,
+                         epsilon_scaling_factor=factor)
+        res, psi2 = srv.run([rid])[rid]
+    say(f"run_sot loop: {[i['status'] for i in infos]} {it1} its; served {res.status} "
+        f"{res.iterations} its, relerr {res.rel_marginal_error:.3e}")
+    assert infos[-1]["status"] == res.status == "converged"
+    assert abs(it1 - res.iterations) <= 3 * steps
+    assert np.max(np.abs((psi1 - psi1.mean()) - (psi2 - psi2.mean()))) < 1e-6
+    assert true_relerr(src, y, nu, eps, psi2) <= tol * 1.001
