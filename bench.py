@@ -74,6 +74,152 @@ def cpu_baseline(wl, psi, R, target_seconds=15.0):
                       f"{pairs} pairs in {dt:.2f} s"}
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_eval_gmm3d_32cube_N50000.md")
+KERNEL_KEYS = {"lse_max": "k_pass1<3, 8, 2, 0>", "lse_sum": "k_pass1<3, 8, 2, 2>",
+               "grad": "k_grad<3, 2, 8>"}
+
+
+def pmc_summary(fetch_csv, write_csv, copy_bytes):
+    """Per-kernel HBM traffic from two rocprofv3 --pmc runs (FETCH_SIZE, then
+    WRITE_SIZE, KiB per dispatch), each scaled by what the same counter reads
+    for the runtime's device-to-device copy of `copy_bytes` bytes; prints a
+    markdown summary whose `traffic-bytes` lines bench.py reads back."""
+    import csv
+    from collections import defaultdict
+
+    def per_kernel(path):
+        acc, copies = defaultdict(list), []
+        for r in csv.DictReader(open(path)):
+            v = float(r["Counter_Value"]) * 1024.0
+            if "copyBuffer" in r["Kernel_Name"]:
+                copies.append((int(r["Grid_Size"]), v))
+            else:
+                acc[r["Kernel_Name"]].append(v)
+        # the calibration copy is the runtime's copy kernel with the largest grid
+        big = max(g for g, _ in copies)
+        cal = [v for g, v in copies if g == big]
+        return ({k: sum(v) / len(v) for k, v in acc.items()}, {k: len(v) for k, v in acc.items()},
+                sum(cal) / len(cal))
+
+    fetch, nf, copy_f = per_kernel(fetch_csv)
+    write, nw, copy_w = per_kernel(write_csv)
+    kf, kw = copy_bytes / copy_f, copy_bytes / copy_w
+    lines = [f"Calibration: a {copy_bytes}-byte device copy reads FETCH_SIZE = {copy_f / 1024:.1f} KiB "
+             f"and WRITE_SIZE = {copy_w / 1024:.1f} KiB; scale factors {kf:.3f} (fetch) and {kw:.3f} (write).",
+             "", "| kernel | dispatches | fetch bytes | write bytes | traffic bytes/launch |",
+             "|---|---|---|---|---|"]
+    rows = []
+    for k in sorted(fetch, key=lambda k: -fetch[k]):
+        if k not in write:
+            continue
+        t = fetch[k] * kf + write[k] * kw
+        lines.append(f"| `{k.split('(')[0]}` | {nf[k]} | {fetch[k] * kf:.4g} | {write[k] * kw:.4g} | {t:.4g} |")
+        rows.append((k.split("(")[0].replace("void sdot::", "").replace("sdot::", ""), t))
+    lines.append("")
+    lines += [f"traffic-bytes {name} = {t:.6g}" for name, t in rows]
+    print("\n".join(lines))
+
+
+def measured_traffic(kernel_key):
+    """HBM bytes per launch of the named kernel from the committed counter
+    summary, or None."""
+    try:
+        text = open(TRAFFIC_FILE).read()
+    except OSError:
+        return None
+    want = KERNEL_KEYS.get(kernel_key)
+    for line in text.splitlines():
+        if line.startswith("traffic-bytes ") and want and line.split("=")[0].strip().endswith(want):
+            return float(line.split("=")[1])
+    return None
+
+
+def serving_leg(s, args):
+    """Serving workload: the first `serve_requests` requests of the seeded
+    trace (Poisson arrivals at 50/s, N ~ logU[256, 65536], eps in {1e-2, 3e-3,
+    1e-3}, componentwise tolerance 1e-6) against the resident source of `s`.
+      batched:  FP32_LSE serving queue (escalation to FP64 for the final
+                iterations), requests submitted at their arrival times;
+      one-at-a-time: the FP64 single-solve path (sdot_solve), requests in
+                arrival order, each started when it has arrived and the
+                previous one has finished.
+    Both use the noise-aware line search; every batched result is re-checked
+    with an FP64 evaluation at the returned potential."""
+    import itertools
+    import semidiscreteot_amd as sd
+    from semidiscreteot_amd import workloads
+    dmax = np.finfo(float).max
+    reqs = list(itertools.islice(workloads.serving_trace(), args.serve_requests))
+    kw = dict(control="componentwise", max_iterations=args.serve_max_it,
+              epsilon_scaling_steps=args.serve_eps_steps, epsilon_scaling_factor=2.0)
+
+    srv = sd.SotServer(s, precision="fp32_lse", max_slots=256, hbm_budget_bytes=96 << 30)
+    t0 = srv.clock()
+    pending = sorted(reqs, key=lambda r: r.arrival)
+    inflight, done = {}, {}
+    while pending or inflight:
+        now = srv.clock() - t0
+        while pending and pending[0].arrival <= now:
+            r = pending.pop(0)
+            inflight[srv.submit(r.y, r.nu, r.epsilon, r.tolerance, **kw)] = r
+        if not inflight:
+            time.sleep(max(0.0, pending[0].arrival - now))
+            continue
+        srv.step()
+        for rid in list(inflight):
+            out = srv.poll(rid)
+            if out is not None:
+                done[inflight.pop(rid).index] = out
+    srv.close()
+    lat_b = np.array([done[r.index][0].t_done - (t0 + r.arrival) for r in reqs])
+    span_b = max(done[r.index][0].t_done for r in reqs) - (t0 + min(r.arrival for r in reqs))
+    # FP64 check of every batched result
+    errs = []
+    for r in reqs:
+        s.setup_target(r.y, r.nu)
+        s.configure(sd.SolverParameters(epsilon=r.epsilon, tau=0.0, use_log_sum_exp_trick=True))
+        _, g, _ = s.evaluate_functional_R(dmax, done[r.index][1])
+        errs.append(float(np.max(np.abs(g) / r.nu)))
+    ok_b = sum(e <= r.tolerance * 1.001 for e, r in zip(errs, reqs))
+    log(f"serving batched fp32: {len(reqs)} solves in {span_b:.2f}s; iterations "
+        f"{[done[r.index][0].iterations for r in reqs]}; fp64 relerr max {max(errs):.3e}")
+
+    # one-at-a-time FP64 single-solve path
+    end, lat_s, ok_s, its = 0.0, [], 0, []
+    for r in sorted(reqs, key=lambda r: r.arrival):
+        ta = time.perf_counter()
+        s.setup_target(r.y, r.nu)
+        prm = sd.SolverParameters(epsilon=r.epsilon, tau=0.0, tolerance=r.tolerance,
+                                  solver_control_type="componentwise",
+                                  max_iterations=args.serve_max_it, use_log_sum_exp_trick=True,
+                                  precision="fp64", line_search="noise_aware",
+                                  use_epsilon_scaling=args.serve_eps_steps > 1,
+                                  epsilon_scaling_steps=max(1, args.serve_eps_steps),
+                                  epsilon_scaling_factor=2.0)
+        psi, infos = s.solve_epsilon_scaling(None, prm)
+        dt = time.perf_counter() - ta
+        end = max(end, r.arrival) + dt
+        lat_s.append(end - r.arrival)
+        ok_s += infos[-1]["status"] == "converged"
+        its.append(sum(i["iterations"] for i in infos))
+    span_s = end - min(r.arrival for r in reqs)
+    lat_s = np.array(lat_s)
+    log(f"serving one-at-a-time fp64: {len(reqs)} solves in {span_s:.2f}s; iterations {its}")
+    return {
+        "workload": f"first {len(reqs)} requests of the seeded serving trace (seed 4, Poisson 50/s, "
+                    "N logU[256,65536], eps {1e-2,3e-3,1e-3}, tol 1e-6 componentwise) on the "
+                    "resident 32^3 Q1 / QGauss(2) mesh",
+        "max_iterations": args.serve_max_it, "epsilon_scaling_steps": args.serve_eps_steps,
+        "batched_fp32": {"solves_per_s": len(reqs) / span_b, "p50_s": float(np.percentile(lat_b, 50)),
+                         "p99_s": float(np.percentile(lat_b, 99)), "within_tol": int(ok_b),
+                         "max_fp64_rel_marginal_err": max(errs)},
+        "one_at_a_time_fp64": {"solves_per_s": len(reqs) / span_s,
+                               "p50_s": float(np.percentile(lat_s, 50)),
+                               "p99_s": float(np.percentile(lat_s, 99)), "within_tol": int(ok_s)},
+        "speedup": span_s / span_b,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,9 +230,19 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--modes", default="fp64,fp32_lse",
                     help="comma list of precisions to time (fp32_lse is the reported one)")
+    ap.add_argument("--serve-requests", type=int, default=8,
+                    help="requests of the serving trace to replay (0: skip the serving leg)")
+    ap.add_argument("--serve-max-it", type=int, default=1000)
+    ap.add_argument("--serve-eps-steps", type=int, default=1,
+                    help="epsilon-scaling stages per request (1: none)")
+    ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_BYTES"),
+                    help="summarise two rocprofv3 --pmc runs into per-kernel HBM traffic and exit")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="also run device-to-device copies of this size (counter calibration)")
     args = ap.parse_args()
+    if args.pmc_summary:
+        pmc_summary(args.pmc_summary[0], args.pmc_summary[1], int(args.pmc_summary[2]))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
@@ -174,6 +330,10 @@ def main():
     achieved = pairs / (kern[dom] * 1e-3)
     peak = pass_peak_pairs(3)
     eval_frac = (pairs / (r["ms"] * 1e-3)) / (peak / 2.0)
+    serving = None
+    if args.serve_requests > 0:
+        log("serving leg ...")
+        serving = serving_leg(s, args)
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -198,7 +358,7 @@ def main():
                    "precision": "fp32_lse", "parallelism": "single-gpu"},
         "roofline": {"bound": "valu", "kernel": f"k_{dom}", "achieved": achieved / 1e9,
                      "peak": peak / 1e9, "unit": "Gpair/s", "frac": achieved / peak,
-                     "traffic": None,
+                     "traffic": measured_traffic(dom),
                      "eval_frac": eval_frac,
                      "peak_basis": "per pair sweep 1/max(1/R_exp,(d+3)/R_fma) with R_fma = 3.93e13 FMA/s "
                                    "(published 78.6 TFLOP/s FP64/FP32 vector) and R_exp >= R_fma/4; "
@@ -208,6 +368,7 @@ def main():
             "value": pairs / (results["fp64"]["ms"] * 1e-3),
             "F_rel_diff_vs_fp32": abs(results["fp64"]["F"] - r["F"]) / abs(results["fp64"]["F"])},
         "kernel_ms": kern,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     if args.copy_bytes > 0:

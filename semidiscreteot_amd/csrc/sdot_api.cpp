@@ -220,7 +220,6 @@ struct sdot_ctx {
   double tgt_lo[3] = {0, 0, 0}, tgt_hi[3] = {0, 0, 0};
   double coord_scale = 1.0;
   size_t cap_trec = 0, cap_qrec = 0, cap_M = 0, cap_G = 0, cap_F = 0, cap_P = 0, cap_N = 0;
-  size_t cap_Mh = 0;
   // profiling: a pool of event sets resolved at synchronisation points
   bool prof_on = false;
   std::vector<std::array<hipEvent_t, 6>> ev_pool;
@@ -301,6 +300,8 @@ int prepare(sdot_ctx *c, double R) {
     HIP_TRY(dalloc(d.Spart, needM));
     HIP_TRY(dalloc(d.Tpart, needM));
     c->cap_M = needM;
+    c->hist_valid = false;  // the seed reads the previous segment maxima
+    p.hist = 0;
   }
   if (c->cap_G < needG) {
     HIP_TRY(dalloc(d.Gpart, needG));
@@ -323,26 +324,19 @@ int prepare(sdot_ctx *c, double R) {
   }
   if (c->cap_P < (size_t)p.nPpart) {
     HIP_TRY(dalloc(d.PNpart, p.nPpart));
-    HIP_TRY(dalloc(d.DMpart, p.nPpart));
-    HIP_TRY(dalloc(d.Dmpart, p.nPpart));
     c->cap_P = p.nPpart;
   }
   if (c->cap_N < (size_t)p.N) {
     HIP_TRY(dalloc(d.psi, p.N));
     HIP_TRY(dalloc(d.grad, p.N));
     HIP_TRY(dalloc(d.psi_prev, p.N));
+    HIP_TRY(dalloc(d.dpsi_r, p.N));
     c->cap_N = p.N;
     c->hist_valid = false;
     p.hist = 0;
   }
-  if (c->cap_Mh < (size_t)p.nQ) {
-    HIP_TRY(dalloc(d.Mhist_m, p.nQ));
-    c->cap_Mh = p.nQ;
-    c->hist_valid = false;
-    p.hist = 0;
-  }
   if (!d.out) HIP_TRY(dalloc(d.out, 4));
-  if (!d.dstat) HIP_TRY(dalloc(d.dstat, 4));
+  if (!d.ucount) HIP_TRY(dalloc(d.ucount, 1));
   return SDOT_OK;
 }
 
@@ -384,7 +378,7 @@ int host_eval(sdot_ctx *c, double R, const double *psi, double *grad, double *F,
   if (seeded) {
     // a seed far above the new maxima would have flushed the FP32 terms: redo unseeded
     double lost = 0.0;
-    HIP_TRY(hipMemcpyAsync(&lost, c->dev.dstat + 2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(&lost, c->dev.ucount, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (lost > 0.0) {
       c->hist_valid = false;
@@ -533,7 +527,7 @@ extern "C" int sdot_ctx_destroy(sdot_ctx *c) {
                   d.cell_m2o, d.ty_o, d.tnu_o, d.tpos_r, d.tpos_m, d.tr2o, d.tm2o, d.tstart,
                   d.twbox, d.ty_m, d.trec_r, d.trec_m, d.qrec_r, d.Mpart, d.Spart, d.Gpart,
                   d.Fpart, d.Cpart, d.PNpart, d.psi, d.grad, d.out, d.scratch, d.i64scratch, d.Tpart,
-                  d.Upart, d.psi_prev, d.Mhist_m, d.DMpart, d.Dmpart, d.dstat};
+                  d.Upart, d.psi_prev, d.dpsi_r, d.ucount};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto &set : c->ev_pool)
@@ -676,7 +670,7 @@ extern "C" int sdot_set_source(sdot_ctx *c, int dim, int64_t n_cells, int nq, co
   HIP_TRY(upload(dv.cell_m2o, m2o));
   c->h_cell_r2o.resize(n_cells);
   for (int64_t r = 0; r < n_cells; ++r) c->h_cell_r2o[r] = m2o[r2m[r]];
-  c->cap_M = c->cap_qrec = c->cap_F = c->cap_Mh = 0;
+  c->cap_M = c->cap_qrec = c->cap_F = 0;
   c->hist_valid = false;
   c->have_source = true;
   return SDOT_OK;

@@ -112,15 +112,18 @@ __device__ __forceinline__ double block_max_256(double v, double *sh) {
   return r;
 }
 
-// FP32 sweep (PASS 2).  Every exponent t is taken relative to a reference
-// value M that bounds the sweep's largest exponent from above, so that the
-// dominant terms have t - M in [-(spread + 1), 0] and the FP32 argument keeps
-// its precision (the rounding of t - M to FP32 costs |t - M| 2^-25 ln 2 of
-// relative error per term).  M is either the exact maximum from a PASS 0 sweep
-// or, when the previous evaluation of the same candidate sets is available,
-// ceil(previous maximum + largest psi increase * coef): an upper bound on the
-// new maximum, at most (spread of the psi change) * coef + 1 above it.  The
-// seed is used while that spread stays below kSeedSpread log2 units.
+// FP32 sweep (PASS 2).  Every exponent t of a work item (a block of
+// quadrature points x a segment of candidate targets) is taken relative to an
+// integer reference value M that bounds the item's exponents from above, so
+// that the dominant terms have t - M in [-(spread + 1), 0] and the FP32
+// argument keeps its precision (the rounding of t - M to FP32 costs
+// |t - M| 2^-25 ln 2 of relative error per term).  M is ceil of either the
+// item's exact maximum (a max loop over its targets first) or, when the
+// previous evaluation of the same item is available, of its previous maximum
+// raised by the largest exponent change among its targets: an upper bound of
+// the new maximum, at most (spread of the changes) + 1 above it.  The seed is
+// used while that spread, over the item's own targets, stays below
+// kSeedSpread log2 units.
 constexpr double kSeedSpread = 2.0;
 
 // Target record (64 bytes):
@@ -147,7 +150,7 @@ struct TRec {
 template <int DIM>
 __device__ __forceinline__ void prep_body(const PrepArgs &a, int64_t blk, double *sh) {
   const int64_t j = blk * 256 + threadIdx.x;
-  double pn = 0.0, dmx = -INFINITY, dmn = INFINITY;
+  double pn = 0.0;
   if (j < a.N) {
     double yr[DIM], yp[DIM];
     double y2 = 0.0;
@@ -177,40 +180,14 @@ __device__ __forceinline__ void prep_body(const PrepArgs &a, int64_t blk, double
     }
     pn = psi * nu;
     if (a.psi_prev) {
-      if (a.hist) {
-        const double dd = psi - a.psi_prev[j];
-        dmx = dd;
-        dmn = dd;
-      }
+      // change of the exponent coefficient since the previous evaluation, in
+      // the row order the q-major sweep streams
+      if (a.hist) a.dpsi_r[a.pos_r[j]] = (psi - a.psi_prev[j]) * a.coef;
       a.psi_prev[j] = psi;
     }
   }
   const double s = block_sum_256(pn, sh);
   if (threadIdx.x == 0) a.PNpart[blk] = s;
-  if (a.psi_prev) {
-    const double m1 = block_max_256(dmx, sh);
-    const double m2 = -block_max_256(-dmn, sh);
-    if (threadIdx.x == 0) {
-      a.DMpart[blk] = m1;
-      a.Dmpart[blk] = m2;
-    }
-  }
-}
-
-// Global max / min of the psi change (history mode): dstat[0] = max, [1] = min.
-__device__ __forceinline__ void dred_body(const double *DMpart, const double *Dmpart, int64_t n,
-                                          double *dstat, double *sh) {
-  double mx = -INFINITY, mn = INFINITY;
-  for (int64_t i = threadIdx.x; i < n; i += 256) {
-    mx = fmax(mx, DMpart[i]);
-    mn = fmin(mn, Dmpart[i]);
-  }
-  const double a = block_max_256(mx, sh);
-  const double b = -block_max_256(-mn, sh);
-  if (threadIdx.x == 0) {
-    dstat[0] = a;
-    dstat[1] = b;
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -226,17 +203,6 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
   const int64_t cell = gg / a.gpc;
   const int64_t q0 = cell * a.nq + (gg - cell * a.gpc) * G;
 
-  // FP32 history seed (see kSeedSpread): decided per solve from the psi change,
-  // identically in PASS 0 (which is then skipped) and PASS 2
-  bool seed = false;
-  double shift = 0.0;
-  if (MODE == kF32Lse && a.hist) {
-    const double dmx = a.dstat[0], dmn = a.dstat[1];
-    seed = (dmx - dmn) * a.coef < kSeedSpread;
-    shift = dmx * a.coef;
-  }
-  if (PASS == 0 && seed) return;
-
   double c[DIM], x[G][DIM], M[G], S[G], T[G];
 #pragma unroll
   for (int d = 0; d < DIM; ++d) c[d] = a.cc[cell * DIM + d];
@@ -246,10 +212,8 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
     for (int d = 0; d < DIM; ++d) x[k][d] = a.qx[(q0 + k) * DIM + d];
     S[k] = 0.0;
     T[k] = -INFINITY;
-    if (PASS == 0) {
+    if (PASS == 0 || PASS == 2) {
       M[k] = -INFINITY;
-    } else if (PASS == 2 && seed) {
-      M[k] = ceil(a.Mhist[q0 + k] + shift);
     } else if (MODE == kF64Plain) {
       M[k] = a.plain_coef * a.qx2[q0 + k];
     } else {
@@ -306,13 +270,57 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
       if (j + 1 < j1) one(r1);
     }
   };
+  auto sweep = [&](auto &&f) {
+    if (a.dense) {
+      const int64_t j0 = a.N * seg / a.K1, j1 = a.N * (seg + 1) / a.K1;
+      f((int)j0, (int)j1);
+    } else {
+      for_rows<DIM>(a.tg, a.tstart, lo, hi, a.Rq, seg, a.K1, f);
+    }
+  };
 
-  if (a.dense) {
-    const int64_t j0 = a.N * seg / a.K1, j1 = a.N * (seg + 1) / a.K1;
-    body((int)j0, (int)j1);
-  } else {
-    for_rows<DIM>(a.tg, a.tstart, lo, hi, a.Rq, seg, a.K1, body);
+  if (PASS == 2) {
+    // reference value of this item (see kSeedSpread)
+    bool seed = false;
+    double dmx = -INFINITY;
+    if (a.hist) {
+      cdouble *dp = cptr(a.dpsi);
+      double dmn = INFINITY;
+      sweep([&](int j0, int j1) {
+        for (int j = j0; j < j1; ++j) {
+          dmx = fmax(dmx, dp[j]);
+          dmn = fmin(dmn, dp[j]);
+        }
+      });
+      seed = dmx - dmn < kSeedSpread;
+    }
+    if (seed) {
+#pragma unroll
+      for (int k = 0; k < G; ++k) M[k] = ceil(a.Tout[(int64_t)seg * a.nQ + q0 + k] + dmx);
+    } else {
+      // exact maximum over the item's candidates first
+      sweep([&](int j0, int j1) {
+        for (int j = j0; j < j1; ++j) {
+          TRec<DIM> r;
+          r.load(trec + (int64_t)j * kRec);
+          if (!a.dense && !in_set<DIM>(r.y, c, a.smax)) continue;
+#pragma unroll
+          for (int k = 0; k < G; ++k) {
+            double t = r.a;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) t = fma(x[k][d], r.yl[d], t);
+            T[k] = fmax(T[k], t);
+          }
+        }
+      });
+#pragma unroll
+      for (int k = 0; k < G; ++k) {
+        M[k] = ceil(T[k]);
+        T[k] = -INFINITY;
+      }
+    }
   }
+  sweep(body);
   if (active) {
 #pragma unroll
     for (int k = 0; k < G; ++k) {
@@ -352,15 +360,25 @@ __device__ __forceinline__ void qfin_body(const QFArgs &a, int64_t blk, double *
       for (int s = 0; s < a.K1; ++s) M = fmax(M, a.Mpart[(int64_t)s * a.nQ + qm]);
       for (int s = 0; s < a.K1; ++s) S += a.Spart[(int64_t)s * a.nQ + qm];
     } else {
-      // every segment of q used the same reference value M
-      M = a.Mpart[qm];
+      // segments carry integer reference values: S = sum_s S_s 2^(M_s - M),
+      // then renormalised to M = ceil(largest exponent) for the gradient pass
+      M = -INFINITY;
       double T = -INFINITY;
       for (int s = 0; s < a.K1; ++s) {
-        S += a.Spart[(int64_t)s * a.nQ + qm];
+        M = fmax(M, a.Mpart[(int64_t)s * a.nQ + qm]);
         T = fmax(T, a.Tpart[(int64_t)s * a.nQ + qm]);
       }
-      if (!(T > -INFINITY)) M = -INFINITY;  // no candidates
-      a.Mhist[qm] = T;
+      if (M > -INFINITY && T > -INFINITY) {
+        for (int s = 0; s < a.K1; ++s) {
+          const double ss = a.Spart[(int64_t)s * a.nQ + qm];
+          if (ss > 0.0) S += ldexp(ss, (int)(a.Mpart[(int64_t)s * a.nQ + qm] - M));
+        }
+        const double Mt = ceil(T);
+        S = ldexp(S, (int)(M - Mt));
+        M = Mt;
+      } else {
+        M = -INFINITY;  // no candidates
+      }
       // safety net: every term far below the reference value (cannot happen
       // while the seed bound holds); the host repeats the evaluation unseeded
       if (M > -INFINITY && S < 1e-40) Uq = 1.0;

@@ -82,10 +82,8 @@ struct Dev {
   double *out = nullptr;       // [4] F, C, count of q whose FP32 sum lost range
   double *Upart = nullptr;     // [nFpart]
   double *psi_prev = nullptr;  // [N] psi of the previous FP32 evaluation
-  double *Mhist_m = nullptr;   // [nQ] M_q of the previous FP32 evaluation (Morton order)
-  double *DMpart = nullptr;    // [nPpart] block max of the psi change
-  double *Dmpart = nullptr;    // [nPpart] block min of the psi change
-  double *dstat = nullptr;     // [2] global max / min of the psi change
+  double *dpsi_r = nullptr;    // [N] exponent change since then, row order
+  double *ucount = nullptr;    // [1] q-points whose FP32 sum lost its range
   double *scratch = nullptr;   // misc
   size_t scratch_bytes = 0;
   int64_t *i64scratch = nullptr;
@@ -119,7 +117,7 @@ struct PrepArgs {
   double *trec_r, *trec_m;
   double *PNpart;
   double *psi_prev;    // [N] psi of the previous FP32 evaluation (history mode)
-  double *DMpart, *Dmpart;  // block max / min of psi - psi_prev
+  double *dpsi_r;      // [N] (psi - psi_prev) * coef, row order (history mode)
   int64_t N;
   int hist;
   double o[3];
@@ -133,12 +131,12 @@ struct P1Args {
   const double *blkbox;  // [n_blocks*2*DIM]
   const double *trec;    // [N*kRec] row order
   const int32_t *tstart;
-  const double *Mpart;   // [K1*nQ]  maxima of PASS 0, read by PASS 1 / 2
-  const double *Mhist;   // [nQ]     FP32: largest exponent of the previous evaluation
-  const double *dstat;   // FP32: max / min psi change since then
+  const double *Mpart;   // [K1*nQ]  maxima of PASS 0, read by PASS 1
+  const double *dpsi;    // [N]      FP32 history: exponent change per target, row order
   double *Mout;          // [K1*nQ]  PASS 0: segment maxima; PASS 2: reference value used
   double *Sout;          // [K1*nQ]  PASS 1 / 2: segment sums
-  double *Tout;          // [K1*nQ]  PASS 2: segment maxima of the exponent
+  double *Tout;          // [K1*nQ]  PASS 2: segment maxima of the exponent (read first:
+                         //          the previous evaluation's, for the history seed)
   Grid tg;
   int64_t n_groups, nQ, N;
   int nq, gpc, K1, dense, hist;
@@ -148,7 +146,7 @@ struct P1Args {
 struct QFArgs {
   const double *qx, *qw, *qx2, *Mpart, *Spart, *Tpart;
   const int64_t *cell_r2m;
-  double *qrec_r, *Fpart, *Cpart, *Upart, *Mhist;
+  double *qrec_r, *Fpart, *Cpart, *Upart;
   int64_t nQ;
   int nq, K1;
   double eps, plain_coef, ln_scale;

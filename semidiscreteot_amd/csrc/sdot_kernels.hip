@@ -10,12 +10,13 @@
 //   k_pass1    q-major sweep over each cell's candidates.  Lane = group of G
 //              quadrature points of one cell; wave = 64 Morton-consecutive groups;
 //              grid.y splits the candidate bin rows into K1 segments.
-//                PASS 0: max exponent (FP64 LSE; FP32 when no history seed),
+//                PASS 0: max exponent (FP64 LSE),
 //                PASS 1: sum_j nu_j exp(t_qj - M_q) with the global max (FP64 modes),
-//                PASS 2: FP32 sweep: S = sum nu exp2(t - M) against an upper
-//                        bound M of the exponents (the exact maximum, or the
-//                        previous evaluation's maximum raised by the largest
-//                        psi increase), and the running maximum of t.
+//                PASS 2: FP32 sweep: S = sum nu exp2(t - M) against an integer
+//                        upper bound M of the item's exponents (its exact
+//                        maximum, found by a max loop first, or the previous
+//                        evaluation's maximum raised by the largest exponent
+//                        change among its targets), and the running maximum of t.
 //   k_qfin     per q: merge segments in fixed order, F_q, w_q/S_q, C_q, the q record
 //              the gradient pass streams; block partials of F and C.
 //   k_grad     target-major: lane = target, loops over the cells near its wave and
@@ -44,12 +45,6 @@ template <int DIM>
 __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
   __shared__ double sh[256];
   prep_body<DIM>(a, blockIdx.x, sh);
-}
-
-__global__ __launch_bounds__(256) void k_dred(const double *DMpart, const double *Dmpart,
-                                              int64_t n, double *dstat) {
-  __shared__ double sh[256];
-  dred_body(DMpart, Dmpart, n, dstat, sh);
 }
 
 template <int DIM, int G, int MODE, int PASS>
@@ -351,15 +346,12 @@ static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_g
   pa.trec_m = d.trec_m;
   pa.PNpart = d.PNpart;
   pa.psi_prev = (MODE == kF32Lse) ? d.psi_prev : nullptr;
-  pa.DMpart = d.DMpart;
-  pa.Dmpart = d.Dmpart;
+  pa.dpsi_r = d.dpsi_r;
   pa.N = p.N;
   pa.hist = p.hist;
   for (int k = 0; k < 3; ++k) pa.o[k] = p.origin[k];
   pa.coef = coef;
   hipLaunchKernelGGL((k_prep<DIM>), dim3((unsigned)p.nPpart), dim3(256), 0, s, pa);
-  if (MODE == kF32Lse && p.hist)
-    hipLaunchKernelGGL(k_dred, dim3(1), dim3(256), 0, s, d.DMpart, d.Dmpart, p.nPpart, d.dstat);
 
   P1Args a{};
   a.qx = d.qx_m;
@@ -369,8 +361,7 @@ static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_g
   a.trec = d.trec_r;
   a.tstart = d.tstart;
   a.Mpart = d.Mpart;
-  a.Mhist = d.Mhist_m;
-  a.dstat = d.dstat;
+  a.dpsi = d.dpsi_r;
   a.tg = p.tg;
   a.n_groups = p.n_groups;
   a.nQ = p.nQ;
@@ -388,9 +379,8 @@ static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_g
   a.Sout = d.Spart;
   a.Tout = d.Tpart;
   hipError_t e;
-  // exact maxima (FP64 LSE always; FP32 when the history seed is unusable:
-  // the kernel then returns at once)
-  if (MODE != kF64Plain) {
+  // exact maxima first (FP64 LSE; the FP32 sweep finds its own bounds)
+  if (MODE == kF64Lse) {
     e = launch_pass1_g<DIM, MODE, 0>(p.G, a, p.n_blocks, p.K1, s);
     if (e != hipSuccess) return e;
   }
@@ -414,7 +404,6 @@ static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_g
   q.Fpart = d.Fpart;
   q.Cpart = d.Cpart;
   q.Upart = d.Upart;
-  q.Mhist = d.Mhist_m;
   q.nQ = p.nQ;
   q.nq = p.nq;
   q.K1 = p.K1;
@@ -443,7 +432,7 @@ static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_g
   if (timing) (void)hipEventRecord(prof->ev[4], s);
   GFArgs gf{d.Gpart, d.trec_m, d.tm2o, d_grad, p.N, p.K2};
   hipLaunchKernelGGL((k_gfin<DIM>), dim3((unsigned)((p.N + 255) / 256)), dim3(256), 0, s, gf);
-  FinArgs fa{d.Fpart, d.Cpart, d.Upart, d.PNpart, p.nFpart, p.nPpart, d_FC, d.dstat + 2};
+  FinArgs fa{d.Fpart, d.Cpart, d.Upart, d.PNpart, p.nFpart, p.nPpart, d_FC, d.ucount};
   hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, s, fa);
   if (timing) (void)hipEventRecord(prof->ev[5], s);
   return hipGetLastError();

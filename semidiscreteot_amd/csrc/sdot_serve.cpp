@@ -122,7 +122,7 @@ struct SolveShape {
 struct SolveOffsets {
   size_t y, nu, pos_r, pos_m, tm2o, tstart, twbox;
   size_t trec_r, trec_m, qrec_r, Mpart, Spart, Tpart, Gpart;
-  size_t Fpart, Cpart, Upart, PNpart, DMpart, Dmpart, dstat, psi_prev, Mhist;
+  size_t Fpart, Cpart, Upart, PNpart, psi_prev, dpsi;
   size_t x, g, x0, gv, p, S[kHist], Y[kHist], hist;
   size_t total;
 };
@@ -149,11 +149,8 @@ SolveOffsets carve(const SourceView &src, const SolveShape &s) {
   o.Cpart = c.take<double>(nFpart);
   o.Upart = c.take<double>(nFpart);
   o.PNpart = c.take<double>(s.nPpart);
-  o.DMpart = c.take<double>(s.nPpart);
-  o.Dmpart = c.take<double>(s.nPpart);
-  o.dstat = c.take<double>(4);
   o.psi_prev = s.fp32 ? c.take<double>(s.N) : SIZE_MAX;
-  o.Mhist = s.fp32 ? c.take<double>(src.nQ) : SIZE_MAX;
+  o.dpsi = s.fp32 ? c.take<double>(s.N) : SIZE_MAX;
   o.x = c.take<double>(s.N);
   o.g = c.take<double>(s.N);
   o.x0 = c.take<double>(s.N);
@@ -225,6 +222,7 @@ struct sdot_server {
   size_t off_desc = 0, off_vecs = 0, off_base = 0, off_in = 0, off_nact = 0, block_bytes = 0;
   char *d_block = nullptr, *h_block = nullptr;
   SlotOut *d_out = nullptr, *h_out = nullptr;
+  int *d_ctr = nullptr;
   int64_t bstride = 0;
   ServeDev sd{};
   std::vector<std::unique_ptr<Solve>> active;
@@ -299,7 +297,7 @@ void build_desc(sdot_server *s, Solve &v) {
   double *Gpart = dptr<double>(s, v, v.o.Gpart);
   double *Fpart = dptr<double>(s, v, v.o.Fpart), *Cpart = dptr<double>(s, v, v.o.Cpart);
   double *Upart = dptr<double>(s, v, v.o.Upart), *PNpart = dptr<double>(s, v, v.o.PNpart);
-  double *dstat = dptr<double>(s, v, v.o.dstat), *Mhist = dptr<double>(s, v, v.o.Mhist);
+  double *dpsi = dptr<double>(s, v, v.o.dpsi);
   double *x = dptr<double>(s, v, v.o.x), *g = dptr<double>(s, v, v.o.g);
   const int64_t nFpart = (src.nQ + 255) / 256;
 
@@ -314,8 +312,7 @@ void build_desc(sdot_server *s, Solve &v) {
   pa.trec_m = trec_m;
   pa.PNpart = PNpart;
   pa.psi_prev = fp32 ? dptr<double>(s, v, v.o.psi_prev) : nullptr;
-  pa.DMpart = dptr<double>(s, v, v.o.DMpart);
-  pa.Dmpart = dptr<double>(s, v, v.o.Dmpart);
+  pa.dpsi_r = dpsi;
   pa.N = r.N;
   pa.hist = v.seeded ? 1 : 0;
   for (int k = 0; k < 3; ++k) pa.o[k] = src.origin[k];
@@ -330,8 +327,7 @@ void build_desc(sdot_server *s, Solve &v) {
   a.trec = trec_r;
   a.tstart = dptr<int32_t>(s, v, v.o.tstart);
   a.Mpart = Mpart;
-  a.Mhist = Mhist;
-  a.dstat = dstat;
+  a.dpsi = dpsi;
   a.Mout = Mpart;
   a.Sout = Spart;
   a.Tout = dptr<double>(s, v, v.o.Tpart);
@@ -362,7 +358,6 @@ void build_desc(sdot_server *s, Solve &v) {
   q.Fpart = Fpart;
   q.Cpart = Cpart;
   q.Upart = Upart;
-  q.Mhist = Mhist;
   q.nQ = src.nQ;
   q.nq = src.nq;
   q.K1 = sh.K1;
@@ -639,6 +634,8 @@ int server_init(sdot_server *s, const sdot_server_config &cfg) {
   std::memset(s->h_block, 0, s->block_bytes);
   SV_TRY(hipMemcpy(s->d_block, s->h_block, s->block_bytes, hipMemcpyHostToDevice));
   SV_TRY(hipMalloc((void **)&s->d_out, sizeof(SlotOut) * s->max_slots));
+  SV_TRY(hipMalloc((void **)&s->d_ctr, sizeof(int) * kNumQueues * 8));
+  SV_TRY(hipMemset(s->d_ctr, 0, sizeof(int) * kNumQueues * 8));
   SV_TRY(hipHostMalloc((void **)&s->h_out, sizeof(SlotOut) * s->max_slots, hipHostMallocDefault));
   ServeDev &sd = s->sd;
   sd.desc = reinterpret_cast<SlotDesc *>(s->d_block + s->off_desc);
@@ -648,6 +645,7 @@ int server_init(sdot_server *s, const sdot_server_config &cfg) {
   sd.in = reinterpret_cast<SlotIn *>(s->d_block + s->off_in);
   sd.nact = reinterpret_cast<int *>(s->d_block + s->off_nact);
   sd.out = s->d_out;
+  sd.ctr = s->d_ctr;
   SV_TRY(serve_grid_sizes(s->src.dim, s->src.G, s->device, sd));
   sd.grid_slots = (unsigned)s->max_slots;
   if (s->use_graph) {
@@ -676,6 +674,7 @@ extern "C" int sdot_server_destroy(sdot_server *s) {
   if (s->d_block) (void)hipFree(s->d_block);
   if (s->h_block) (void)hipHostFree(s->h_block);
   if (s->d_out) (void)hipFree(s->d_out);
+  if (s->d_ctr) (void)hipFree(s->d_ctr);
   if (s->h_out) (void)hipHostFree(s->h_out);
   s->arena.release();
   if (s->stream) (void)hipStreamDestroy(s->stream);

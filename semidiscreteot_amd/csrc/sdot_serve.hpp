@@ -45,8 +45,12 @@ enum : int {
   kNumLists
 };
 
+// Item counters of the dynamically scheduled sweeps (one per kernel and XCD).
+enum : int { kQPass2F32 = 0, kQPass0F64, kQPass1F64, kQGradF32, kQGradF64, kNumQueues };
+
 struct ServeDev {
   SlotDesc *desc;
+  int *ctr;           // [kNumQueues][8], zeroed by the first kernel of every step
   int64_t *base;      // [kNumLists][bstride] item offsets per solve
   int64_t bstride;
   int *nact;

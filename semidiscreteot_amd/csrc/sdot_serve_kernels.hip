@@ -31,6 +31,28 @@ __device__ __forceinline__ int find_item(const int64_t *__restrict__ base, int n
   return lo;
 }
 
+// Dynamic item scheduling of the one-wave sweeps.  Workgroups of a
+// persistent grid are dispatched round-robin over the 8 XCDs, so workgroup b
+// sits on XCD b mod 8; it takes the items i = b (mod 8), in order, from that
+// XCD's counter.  Items of equal residue share segments (item counts of every
+// solve are multiples of 8), so an XCD keeps streaming its own eighth of the
+// records, and a workgroup that finishes early takes more items instead of
+// idling at the end of a static share.  Which workgroup runs an item does not
+// change what the item computes.
+template <class F>
+__device__ __forceinline__ void for_items_xcd(int *__restrict__ ctr, int64_t total, F &&f) {
+  const int x = blockIdx.x & 7;
+  if (x >= total) return;  // (also: an empty list costs no atomics)
+  for (;;) {
+    int k = 0;
+    if (threadIdx.x == 0) k = atomicAdd(ctr + x, 1);
+    k = __builtin_amdgcn_readfirstlane(k);
+    const int64_t it = x + 8 * (int64_t)k;
+    if (it >= total) break;
+    f(it);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // evaluation work lists
 // ---------------------------------------------------------------------------
@@ -48,29 +70,19 @@ __global__ __launch_bounds__(256) void kl_prep(const SlotDesc *__restrict__ desc
   }
 }
 
-__global__ __launch_bounds__(256) void kl_dred(const SlotDesc *__restrict__ desc,
-                                               const int *__restrict__ nact) {
-  __shared__ double sh[256];
-  const int n = *nact;
-  for (int s = blockIdx.x; s < n; s += gridDim.x) {
-    const SlotDesc &d = desc[s];
-    if (d.p1.hist) dred_body(d.prep.DMpart, d.prep.Dmpart, d.nPpart, (double *)d.p1.dstat, sh);
-  }
-}
-
 template <int DIM, int G, int MODE, int PASS>
 __global__ __launch_bounds__(64) void kl_pass1(const SlotDesc *__restrict__ desc,
                                                const int64_t *__restrict__ base,
-                                               const int *__restrict__ nact) {
+                                               const int *__restrict__ nact,
+                                               int *__restrict__ ctr) {
   const int n = *nact;
-  const int64_t total = base[n];
-  for (int64_t it = blockIdx.x; it < total; it += gridDim.x) {
+  for_items_xcd(ctr, base[n], [&](int64_t it) {
     const int s = find_item(base, n, it);
     const P1Args a = desc[s].p1;
     const int64_t loc = it - base[s];
     const int64_t blk = loc / a.K1;
     pass1_body<DIM, G, MODE, PASS>(a, blk, (int)(loc - blk * a.K1));
-  }
+  });
 }
 
 template <int DIM, int MODE>
@@ -90,16 +102,16 @@ __global__ __launch_bounds__(256) void kl_qfin(const SlotDesc *__restrict__ desc
 template <int DIM, int MODE, int G>
 __global__ __launch_bounds__(64) void kl_grad(const SlotDesc *__restrict__ desc,
                                               const int64_t *__restrict__ base,
-                                              const int *__restrict__ nact) {
+                                              const int *__restrict__ nact,
+                                              int *__restrict__ ctr) {
   const int n = *nact;
-  const int64_t total = base[n];
-  for (int64_t it = blockIdx.x; it < total; it += gridDim.x) {
+  for_items_xcd(ctr, base[n], [&](int64_t it) {
     const int s = find_item(base, n, it);
     const GArgs a = desc[s].gr;
     const int64_t loc = it - base[s];
     const int64_t blk = loc / a.K2;
     grad_body<DIM, MODE, G>(a, blk, (int)(loc - blk * a.K2));
-  }
+  });
 }
 
 template <int DIM>
@@ -167,8 +179,12 @@ __device__ __forceinline__ double bdot(const double *a, const double *b, int64_t
 __global__ __launch_bounds__(kBT) void kl_action(const SlotVecs *__restrict__ vecs,
                                                  const SlotIn *__restrict__ in,
                                                  SlotOut *__restrict__ out,
-                                                 const int *__restrict__ nact) {
+                                                 const int *__restrict__ nact,
+                                                 int *__restrict__ ctr) {
   __shared__ double sh[kBT];
+  // the step's item counters start at zero (a kernel, not a memset node, so
+  // the step can be replayed from a captured graph)
+  if (blockIdx.x == 0 && threadIdx.x < kNumQueues * 8) ctr[threadIdx.x] = 0;
   const int n = *nact;
   for (int s = blockIdx.x; s < n; s += gridDim.x) {
     const SlotVecs v = vecs[s];
@@ -278,25 +294,22 @@ static void step_t(const ServeDev &sd, hipStream_t s) {
   const unsigned gp = sd.grid_pass, gg = sd.grid_grad, gb = sd.grid_b, gs = sd.grid_slots;
   const int64_t *b = sd.base;
   const int64_t L = sd.bstride;
-  hipLaunchKernelGGL(kl_action, dim3(gs), dim3(kBT), 0, s, sd.vecs, sd.in, sd.out, sd.nact);
+  hipLaunchKernelGGL(kl_action, dim3(gs), dim3(kBT), 0, s, sd.vecs, sd.in, sd.out, sd.nact, sd.ctr);
   hipLaunchKernelGGL((kl_prep<DIM>), dim3(gb), dim3(256), 0, s, sd.desc, b + kLPrep * L, sd.nact);
-  hipLaunchKernelGGL(kl_dred, dim3(gs), dim3(256), 0, s, sd.desc, sd.nact);
-  hipLaunchKernelGGL((kl_pass1<DIM, G, kF32Lse, 0>), dim3(gp), dim3(64), 0, s, sd.desc,
-                     b + kLPass32 * L, sd.nact);
   hipLaunchKernelGGL((kl_pass1<DIM, G, kF32Lse, 2>), dim3(gp), dim3(64), 0, s, sd.desc,
-                     b + kLPass32 * L, sd.nact);
+                     b + kLPass32 * L, sd.nact, sd.ctr + kQPass2F32 * 8);
   hipLaunchKernelGGL((kl_pass1<DIM, G, kF64Lse, 0>), dim3(gp), dim3(64), 0, s, sd.desc,
-                     b + kLPass64 * L, sd.nact);
+                     b + kLPass64 * L, sd.nact, sd.ctr + kQPass0F64 * 8);
   hipLaunchKernelGGL((kl_pass1<DIM, G, kF64Lse, 1>), dim3(gp), dim3(64), 0, s, sd.desc,
-                     b + kLPass64 * L, sd.nact);
+                     b + kLPass64 * L, sd.nact, sd.ctr + kQPass1F64 * 8);
   hipLaunchKernelGGL((kl_qfin<DIM, kF32Lse>), dim3(gb), dim3(256), 0, s, sd.desc,
                      b + kLQfin32 * L, sd.nact);
   hipLaunchKernelGGL((kl_qfin<DIM, kF64Lse>), dim3(gb), dim3(256), 0, s, sd.desc,
                      b + kLQfin64 * L, sd.nact);
   hipLaunchKernelGGL((kl_grad<DIM, kF32Lse, G>), dim3(gg), dim3(64), 0, s, sd.desc,
-                     b + kLGrad32 * L, sd.nact);
+                     b + kLGrad32 * L, sd.nact, sd.ctr + kQGradF32 * 8);
   hipLaunchKernelGGL((kl_grad<DIM, kF64Lse, G>), dim3(gg), dim3(64), 0, s, sd.desc,
-                     b + kLGrad64 * L, sd.nact);
+                     b + kLGrad64 * L, sd.nact, sd.ctr + kQGradF64 * 8);
   hipLaunchKernelGGL((kl_gfin<DIM>), dim3(gb), dim3(256), 0, s, sd.desc, b + kLGfin * L, sd.nact);
   hipLaunchKernelGGL(kl_final, dim3(gs), dim3(256), 0, s, sd.desc, sd.nact);
   hipLaunchKernelGGL(kl_dots, dim3(gs), dim3(kBT), 0, s, sd.vecs, sd.in, sd.out, sd.nact);
@@ -332,7 +345,6 @@ static void grids_t(unsigned cus, ServeDev &sd) {
   // the smallest occupancy of the kernels sharing a grid size, so that every
   // persistent workgroup is resident from the start
   int pass = resident(kl_pass1<DIM, G, kF32Lse, 2>, 64);
-  pass = std::min(pass, resident(kl_pass1<DIM, G, kF32Lse, 0>, 64));
   pass = std::min(pass, resident(kl_pass1<DIM, G, kF64Lse, 0>, 64));
   pass = std::min(pass, resident(kl_pass1<DIM, G, kF64Lse, 1>, 64));
   int grad = std::min(resident(kl_grad<DIM, kF32Lse, G>, 64), resident(kl_grad<DIM, kF64Lse, G>, 64));
