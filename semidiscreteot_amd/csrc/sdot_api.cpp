@@ -133,6 +133,13 @@ double sdot::smax_for(double R) {
 
 int sdot::set_error(int code, const std::string &msg) { return fail(code, msg); }
 
+// Squared radius below which a pair surely passes s <= smax: a relative margin
+// of 1e-12, far above the rounding of the FP64 squared distance (a few ulps).
+double sdot::inner_s(double smax) {
+  if (!(smax > 0.0) || !std::isfinite(smax)) return 0.0;
+  return smax * (1.0 - 1e-12);
+}
+
 // Orders of the target points (Morton for the gradient waves, row-major bins
 // for the q-major sweeps), the bin grid and the per-wave bounding boxes.
 void sdot::build_target_layout(int dim, int64_t n, const double *y, TargetLayout &L) {
@@ -273,6 +280,7 @@ int prepare(sdot_ctx *c, double R) {
   }
   const double diag = std::sqrt(diag2);
   p.smax = smax_for(R);
+  p.s_in = inner_s(p.smax);
   p.dense = (R >= diag * (1.0 + 1e-6) + 1e-300) ? 1 : 0;
   p.Rq = R * (1.0 + 1e-9) + 1e-9 * (p.tg.h + p.cg.h + c->coord_scale);
   // the seed needs the previous evaluation's candidate sets: same R

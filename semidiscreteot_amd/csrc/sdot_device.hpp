@@ -7,6 +7,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "sdot_internal.hpp"
 
 namespace sdot {
@@ -46,13 +48,20 @@ __device__ __forceinline__ double expo(double d) {
   }
 }
 
+using Tested = std::true_type;    // run whose members need the candidate test
+using Inside = std::false_type;   // run of candidates of every query point
+
 // Visit the bin rows (x-runs of bins, contiguous in row-major bin order) whose
 // bins can hold a point within Rq of the box [lo,hi]; every nseg-th visited row
-// (phase seg) is handed to body(first_item, end_item).
+// (phase seg) is handed to body(first_item, end_item, tag).  With s_in > 0 a row
+// is split into runs: bins every point of which lies within sqrt(s_in) of every
+// point of the box (tag Inside: candidates of every query point in the box,
+// s_in being the candidate bound with a relative margin far above the rounding
+// of the squared distance) and the rest (tag Tested).
 template <int DIM, class Body>
 __device__ __forceinline__ void for_rows(const Grid &g, const int32_t *__restrict__ start,
                                          const double *lo, const double *hi, double Rq,
-                                         int seg, int nseg, Body &&body) {
+                                         double s_in, int seg, int nseg, Body &&body) {
   int b0[3] = {0, 0, 0}, b1[3] = {0, 0, 0};
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
@@ -60,12 +69,14 @@ __device__ __forceinline__ void for_rows(const Grid &g, const int32_t *__restric
     b1[d] = bin_of(hi[d] + Rq, g.o[d], g.inv_h, g.n[d]);
   }
   const double R2 = Rq * Rq;
+  const double pad = 1e-9 * g.h;
   int phase = 0;
   for (int bz = b0[2]; bz <= b1[2]; ++bz) {
-    double dz = 0.0;
+    double dz = 0.0, dzf = 0.0;
     if (DIM == 3) {
       const double zl = g.o[2] + bz * g.h, zh = zl + g.h;
       dz = fmax(0.0, fmax(zl - hi[2], lo[2] - zh));
+      dzf = fmax(zh - lo[2], hi[2] - zl);
     }
     for (int by = b0[1]; by <= b1[1]; ++by) {
       const double yl = g.o[1] + by * g.h, yh = yl + g.h;
@@ -79,9 +90,34 @@ __device__ __forceinline__ void for_rows(const Grid &g, const int32_t *__restric
       const int x0 = bin_of(lo[0] - wx, g.o[0], g.inv_h, g.n[0]);
       const int x1 = bin_of(hi[0] + wx, g.o[0], g.inv_h, g.n[0]);
       const int base = (bz * g.n[1] + by) * g.n[0];
+      // inner bins [i0, i1]: farthest y / z distance to the box, then the x
+      // interval [hi - w, lo + w] every point of which is within sqrt(s_in) of
+      // every point of the box; only bins entirely inside it (less a pad that
+      // covers the rounding of the binning rule)
+      int i0 = x1 + 1, i1 = x1;
+      if (s_in > 0.0) {
+        const double dyf = fmax(yh - lo[1], hi[1] - yl);
+        const double r2x = s_in - dyf * dyf - dzf * dzf;
+        if (r2x > 0.0) {
+          const double w = sqrt(r2x);
+          const double xa = hi[0] - w + pad, xb = lo[0] + w - pad;
+          if (xa < xb) {
+            i0 = max(x0, (int)ceil((xa - g.o[0]) * g.inv_h));
+            i1 = min(x1, (int)floor((xb - g.o[0]) * g.inv_h) - 1);
+          }
+        }
+      }
       const int s = __builtin_amdgcn_readfirstlane(start[base + x0]);
       const int e = __builtin_amdgcn_readfirstlane(start[base + x1 + 1]);
-      if (s < e) body(s, e);
+      if (i0 <= i1) {
+        const int m0 = __builtin_amdgcn_readfirstlane(start[base + i0]);
+        const int m1 = __builtin_amdgcn_readfirstlane(start[base + i1 + 1]);
+        if (s < m0) body(s, m0, Tested{});
+        if (m0 < m1) body(m0, m1, Inside{});
+        if (m1 < e) body(m1, e, Tested{});
+      } else if (s < e) {
+        body(s, e, Tested{});
+      }
     }
   }
 }
@@ -228,8 +264,10 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
     lo[d] = a.blkbox[blk * 2 * DIM + d];
     hi[d] = a.blkbox[blk * 2 * DIM + DIM + d];
   }
-  auto one = [&](const TRec<DIM> &r) {
-    if (!a.dense && !in_set<DIM>(r.y, c, a.smax)) return;
+  auto one = [&](const TRec<DIM> &r, auto tag) {
+    if constexpr (decltype(tag)::value) {
+      if (!in_set<DIM>(r.y, c, a.smax)) return;
+    }
     if (PASS == 0) {
 #pragma unroll
       for (int k = 0; k < G; ++k) {
@@ -259,23 +297,23 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
     }
   };
   cdouble *trec = cptr(a.trec);
-  auto body = [&](int j0, int j1) {
+  auto body = [&](int j0, int j1, auto tag) {
     // records are fetched two at a time ahead of their use
     for (int j = j0; j < j1; j += 2) {
       const int j2 = (j + 1 < j1) ? j + 1 : j;
       TRec<DIM> r0, r1;
       r0.load(trec + (int64_t)j * kRec);
       r1.load(trec + (int64_t)j2 * kRec);
-      one(r0);
-      if (j + 1 < j1) one(r1);
+      one(r0, tag);
+      if (j + 1 < j1) one(r1, tag);
     }
   };
   auto sweep = [&](auto &&f) {
     if (a.dense) {
       const int64_t j0 = a.N * seg / a.K1, j1 = a.N * (seg + 1) / a.K1;
-      f((int)j0, (int)j1);
+      f((int)j0, (int)j1, Inside{});
     } else {
-      for_rows<DIM>(a.tg, a.tstart, lo, hi, a.Rq, seg, a.K1, f);
+      for_rows<DIM>(a.tg, a.tstart, lo, hi, a.Rq, a.s_in, seg, a.K1, f);
     }
   };
 
@@ -286,7 +324,7 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
     if (a.hist) {
       cdouble *dp = cptr(a.dpsi);
       double dmn = INFINITY;
-      sweep([&](int j0, int j1) {
+      sweep([&](int j0, int j1, auto) {
         for (int j = j0; j < j1; ++j) {
           dmx = fmax(dmx, dp[j]);
           dmn = fmin(dmn, dp[j]);
@@ -299,11 +337,13 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
       for (int k = 0; k < G; ++k) M[k] = ceil(a.Tout[(int64_t)seg * a.nQ + q0 + k] + dmx);
     } else {
       // exact maximum over the item's candidates first
-      sweep([&](int j0, int j1) {
+      sweep([&](int j0, int j1, auto tag) {
         for (int j = j0; j < j1; ++j) {
           TRec<DIM> r;
           r.load(trec + (int64_t)j * kRec);
-          if (!a.dense && !in_set<DIM>(r.y, c, a.smax)) continue;
+          if constexpr (decltype(tag)::value) {
+            if (!in_set<DIM>(r.y, c, a.smax)) continue;
+          }
 #pragma unroll
           for (int k = 0; k < G; ++k) {
             double t = r.a;
@@ -455,9 +495,9 @@ __device__ __forceinline__ void grad_body(const GArgs &a, int64_t blk, int seg) 
       }
     }
   };
-  auto body = [&](int c0, int c1) {
+  auto body = [&](int c0, int c1, auto tag) {
     for (int c = c0; c < c1; ++c) {
-      if (!a.dense) {
+      if constexpr (decltype(tag)::value) {
         cdouble *cp = ccr + (int64_t)c * 4;
         double cv[DIM];
 #pragma unroll
@@ -470,9 +510,9 @@ __device__ __forceinline__ void grad_body(const GArgs &a, int64_t blk, int seg) 
   };
   if (a.dense) {
     const int64_t c0 = a.n_cells * seg / a.K2, c1 = a.n_cells * (seg + 1) / a.K2;
-    body((int)c0, (int)c1);
+    body((int)c0, (int)c1, Inside{});
   } else {
-    for_rows<DIM>(a.cg, a.cstart, lo, hi, a.Rq, seg, a.K2, body);
+    for_rows<DIM>(a.cg, a.cstart, lo, hi, a.Rq, a.s_in, seg, a.K2, body);
   }
   if (active) a.Gpart[(int64_t)seg * a.N + j] = acc;
 }

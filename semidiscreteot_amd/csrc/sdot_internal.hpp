@@ -102,6 +102,7 @@ struct Plan {
   double eps = 1.0;
   double smax = 0.0;   // candidate rule: s <= smax  <=>  sqrt(s) <= R
   double Rq = 0.0;     // conservative enumeration radius
+  double s_in = 0.0;    // inner runs: every pair closer than sqrt(s_in) passes the rule
   int hist = 0;        // FP32 mode: seed the online sweep with the previous evaluation
   int dense = 0;       // every pair is a candidate (no enumeration, no test)
 };
@@ -141,6 +142,7 @@ struct P1Args {
   int64_t n_groups, nQ, N;
   int nq, gpc, K1, dense, hist;
   double smax, Rq, plain_coef, coef;
+  double s_in;            // squared radius of the untested inner runs (0: none)
 };
 
 struct QFArgs {
@@ -163,6 +165,7 @@ struct GArgs {
   int64_t N, n_cells;
   int nq, K2, dense;
   double smax, Rq;
+  double s_in;            // squared radius of the untested inner runs (0: none)
 };
 
 struct GFArgs {
@@ -197,6 +200,8 @@ void build_target_layout(int dim, int64_t n, const double *y, TargetLayout &L);
 
 // Largest double s with sqrt(s) <= R (candidate test s <= smax).
 double smax_for(double R);
+// Squared radius of the untested inner runs for the bound smax (0: none).
+double inner_s(double smax);
 // Sets the thread's sdot_last_error message; returns code.
 int set_error(int code, const std::string &msg);
 

@@ -109,7 +109,8 @@ __global__ __launch_bounds__(64) void k_cstats(CSArgs a) {
   }
   int64_t cnt = 0;
   uint64_t h = 0;
-  auto body = [&](int j0, int j1) {
+  // the audit applies the exact rule to every enumerated target (no inner runs)
+  auto body = [&](int j0, int j1, auto) {
     for (int j = j0; j < j1; ++j) {
       const double *r = a.trec + (int64_t)j * kRec;
       double yr[DIM];
@@ -124,9 +125,9 @@ __global__ __launch_bounds__(64) void k_cstats(CSArgs a) {
     }
   };
   if (a.dense)
-    body(0, (int)a.N);
+    body(0, (int)a.N, Inside{});
   else
-    for_rows<DIM>(a.tg, a.tstart, lo, hi, a.Rq, 0, 1, body);
+    for_rows<DIM>(a.tg, a.tstart, lo, hi, a.Rq, 0.0, 0, 1, body);
   if (owner) {
     const int64_t oc = a.cell_m2o[cell];
     a.count[oc] = cnt;
@@ -373,6 +374,7 @@ static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_g
   a.hist = p.hist;
   a.smax = p.smax;
   a.Rq = p.Rq;
+  a.s_in = p.s_in;
   a.plain_coef = 0.5 / p.eps;
   a.coef = coef;
   a.Mout = d.Mpart;
@@ -428,6 +430,7 @@ static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_g
   g.dense = p.dense;
   g.smax = p.smax;
   g.Rq = p.Rq;
+  g.s_in = p.s_in;
   launch_grad_g<DIM, MODE>(p.G, g, p.n_twaves, p.K2, s);
   if (timing) (void)hipEventRecord(prof->ev[4], s);
   GFArgs gf{d.Gpart, d.trec_m, d.tm2o, d_grad, p.N, p.K2};

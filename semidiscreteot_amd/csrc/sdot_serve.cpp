@@ -175,7 +175,7 @@ struct Request {
   TargetLayout L;
   SolveShape shape;
   int dense = 0;
-  double smax = 0.0, Rq = 0.0;
+  double smax = 0.0, Rq = 0.0, s_in = 0.0;
   size_t bytes = 0;
   double t_submit = 0.0;
 };
@@ -259,6 +259,7 @@ int fill_shape(sdot_server *s, Request &r) {
   const double diag = std::sqrt(diag2);
   const double R = r.R;
   r.smax = smax_for(R);
+  r.s_in = inner_s(r.smax);
   r.dense = (R >= diag * (1.0 + 1e-6) + 1e-300) ? 1 : 0;
   r.Rq = R * (1.0 + 1e-9) + 1e-9 * (r.L.tg.h + src.cg.h + std::max(src.coord_scale, r.L.scale));
   if (!std::isfinite(r.Rq)) r.dense = 1;
@@ -342,6 +343,7 @@ void build_desc(sdot_server *s, Solve &v) {
   a.hist = v.seeded ? 1 : 0;
   a.smax = r.smax;
   a.Rq = r.Rq;
+  a.s_in = r.s_in;
   a.plain_coef = 0.5 / eps;
   a.coef = coef;
 
@@ -381,6 +383,7 @@ void build_desc(sdot_server *s, Solve &v) {
   gr.dense = r.dense;
   gr.smax = r.smax;
   gr.Rq = r.Rq;
+  gr.s_in = r.s_in;
 
   v.desc.gf = GFArgs{Gpart, trec_m, dptr<int32_t>(s, v, v.o.tm2o), g, r.N, sh.K2};
   // out / ucount are set per batch position in upload_batch
