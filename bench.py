@@ -169,7 +169,11 @@ def serving_leg(s, args):
         for rid in list(inflight):
             out = srv.poll(rid)
             if out is not None:
-                done[inflight.pop(rid).index] = out
+                r = inflight.pop(rid)
+                done[r.index] = out
+                log(f"  batched: request {r.index} (N={r.nu.size}, eps={r.epsilon:g}) "
+                    f"{out[0].status} after {out[0].iterations} iterations, "
+                    f"{len(done)}/{len(reqs)} done")
     srv.close()
     lat_b = np.array([done[r.index][0].t_done - (t0 + r.arrival) for r in reqs])
     span_b = max(done[r.index][0].t_done for r in reqs) - (t0 + min(r.arrival for r in reqs))
@@ -202,6 +206,8 @@ def serving_leg(s, args):
         lat_s.append(end - r.arrival)
         ok_s += infos[-1]["status"] == "converged"
         its.append(sum(i["iterations"] for i in infos))
+        log(f"  one-at-a-time: request {r.index} (N={r.nu.size}) {infos[-1]['status']} after "
+            f"{its[-1]} iterations, {dt:.2f}s")
     span_s = end - min(r.arrival for r in reqs)
     lat_s = np.array(lat_s)
     log(f"serving one-at-a-time fp64: {len(reqs)} solves in {span_s:.2f}s; iterations {its}")
