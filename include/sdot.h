@@ -158,6 +158,33 @@ int sdot_hessian(sdot_ctx *ctx, const double *psi, double *H, int64_t ld);
  * the reference's line search and stopping test; psi in/out (size N). */
 int sdot_solve(sdot_ctx *ctx, double *psi, sdot_solve_info *info);
 
+/* ---- multilevel potential transfer ---------------------------------------- */
+/* SoftmaxRefinement<dim>::compute_refinement (SoftmaxRefinement.h, SoftmaxRefinement.cc:
+ * 200-280, with local_assemble :61-198) over the context's source measure (the
+ * source the reference constructs it with, SemiDiscreteOT.cc:776-784): the
+ * potential of a fine target level from the coarse level's,
+ *   psi_f = -lambda log sum_q (w_q / Z_q) exp(-|x_q - y_f|^2 / (2 lambda)),
+ *   Z_q   = sum_c nu_c exp((psi_c - |x_q - y_c|^2 / 2) / lambda),
+ * c over the coarse points within distance_threshold of the center of the cell
+ * of q and of x_q, f over the children of those coarse points within
+ * distance_threshold of x_q; fine points that receive nothing get 0.
+ * use_log_sum_exp_trick selects the max-shifted form of the reference.
+ *   y_fine [n_fine*dim]; y_coarse [n_coarse*dim], nu_coarse, psi_coarse [n_coarse]
+ *   child lists of the level (child_indices[level] of the reference) as CSR:
+ *   the children of coarse point c < n_parents are
+ *   children[child_offsets[c] .. child_offsets[c+1]) (child_offsets[0] = 0);
+ *   coarse points c >= n_parents have none and children outside [0, n_fine)
+ *   are skipped, as the reference does after reporting them.
+ * The fine target density of the reference's signature is not read by its
+ * algorithm and is not taken.  The context's target measure is left as it is.
+ * FP64; psi_fine [n_fine] out. */
+int sdot_softmax_refinement(sdot_ctx *ctx, int64_t n_fine, const double *y_fine,
+                            int64_t n_coarse, const double *y_coarse, const double *nu_coarse,
+                            const double *psi_coarse, int64_t n_parents,
+                            const int64_t *child_offsets, const int64_t *children,
+                            double lambda, double distance_threshold,
+                            int use_log_sum_exp_trick, double *psi_fine);
+
 /* ---- thresholds and audit ------------------------------------------------ */
 /* compute_covering_radius (SotSolver.cc:431-473), raw (without the 1.1). */
 int sdot_covering_radius(sdot_ctx *ctx, double *r0);

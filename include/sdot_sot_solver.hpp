@@ -1,12 +1,14 @@
 // sdot_sot_solver.hpp -- header-only C++ facade with the method set of the
 // reference's SotSolver<dim, spacedim> (include/SemiDiscreteOT/solvers/SotSolver.h:
 // 176-348) over the C ABI of sdot.h.  No deal.II types: setup_source takes the
-// flat arrays a deal.II caller pulls once from FEValues (see INTEGRATION.md for
-// the loop that fills them).  Errors throw std::runtime_error as the reference
-// does for invalid measures (SotSolver.cc:110-112); non-convergence does not
-// throw (SotSolver.cc:207-211) and is reported by get_convergence_status().
+// flat arrays a deal.II caller pulls once from FEValues (one loop over the
+// locally owned cells: quadrature points, density times JxW, cell centers).
+// Errors throw std::runtime_error as the reference does for invalid measures
+// (SotSolver.cc:110-112); non-convergence does not throw (SotSolver.cc:207-211)
+// and is reported by get_convergence_status().
 #pragma once
 
+#include <cstdint>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -121,6 +123,50 @@ class SotSolverGPU {
   Params params_{};
   sdot_solve_info info_{};
   int64_t n_ = 0;
+};
+
+// SoftmaxRefinement<dim, spacedim> (SoftmaxRefinement.h, SoftmaxRefinement.cc:4-27,
+// 200-280) over the source measure of a SotSolverGPU: the reference's constructor
+// takes the source DoFHandler/mapping/FE/density/quadrature order, which the
+// solver's context already holds as quadrature data.
+template <int dim, int spacedim = dim>
+class SoftmaxRefinementGPU {
+ public:
+  SoftmaxRefinementGPU(const SotSolverGPU<dim, spacedim> &solver, double distance_threshold,
+                       bool use_log_sum_exp_trick)
+      : ctx_(solver.handle()), threshold_(distance_threshold), lse_(use_log_sum_exp_trick) {}
+
+  // Points flat (n*spacedim); child_indices[level][c] = fine children of coarse c.
+  // target_density_fine is not read by the reference's algorithm.
+  std::vector<double> compute_refinement(
+      const std::vector<double> &target_points_fine, const std::vector<double> &target_density_fine,
+      const std::vector<double> &target_points_coarse,
+      const std::vector<double> &target_density_coarse, const std::vector<double> &potential_coarse,
+      double regularization_param, int level,
+      const std::vector<std::vector<std::vector<size_t>>> &child_indices) const {
+    (void)target_density_fine;
+    const int64_t n_fine = (int64_t)(target_points_fine.size() / spacedim);
+    std::vector<double> out(n_fine, 0.0);
+    if (level < 0 || level >= (int)child_indices.size()) return out;  // reference: error, zeros
+    const auto &lists = child_indices[level];
+    std::vector<int64_t> off(lists.size() + 1, 0), flat;
+    for (size_t c = 0; c < lists.size(); ++c) {
+      for (size_t f : lists[c]) flat.push_back(f > (size_t)INT64_MAX ? -1 : (int64_t)f);
+      off[c + 1] = (int64_t)flat.size();
+    }
+    check(sdot_softmax_refinement(ctx_, n_fine, target_points_fine.data(),
+                                  (int64_t)target_density_coarse.size(),
+                                  target_points_coarse.data(), target_density_coarse.data(),
+                                  potential_coarse.data(), (int64_t)lists.size(), off.data(),
+                                  flat.empty() ? nullptr : flat.data(), regularization_param,
+                                  threshold_, lse_ ? 1 : 0, out.empty() ? nullptr : out.data()));
+    return out;
+  }
+
+ private:
+  sdot_ctx *ctx_;
+  double threshold_;
+  bool lse_;
 };
 
 }  // namespace sdot

@@ -93,6 +93,9 @@ def lib():
                                 C.POINTER(_State), _dp, C.POINTER(_Info)]
         L.orc_bfgs_generic.argtypes = [_FUN, C.c_void_p, C.c_int64, _dp, C.c_int, C.c_double,
                                        C.POINTER(_Info)]
+        L.orc_softmax_refinement.argtypes = [C.POINTER(_Source), C.c_int64, _dp,
+                                             C.POINTER(_Target), _dp, C.c_int64, _i64p, _i64p,
+                                             C.c_double, C.c_double, C.c_int, C.c_int, _dp]
         _lib = L
     return _lib
 
@@ -284,3 +287,31 @@ def bfgs_generic(fun, x0, max_iterations=1000, tol=1e-10):
     lib().orc_bfgs_generic(cfun, None, n, _p(x), int(max_iterations), float(tol), C.byref(info))
     return x, {"status": STATUS.get(info.status, info.status), "iterations": info.iterations,
                "evaluations": info.evaluations, "functional": info.functional}
+
+
+def children_csr(children):
+    """List of child-index lists (one per coarse point) -> (offsets, flat)."""
+    off = np.zeros(len(children) + 1, dtype=np.int64)
+    off[1:] = np.cumsum([len(c) for c in children])
+    flat = np.ascontiguousarray(np.concatenate([np.asarray(c, dtype=np.int64) for c in children])
+                                if len(children) else np.zeros(0, dtype=np.int64), dtype=np.int64)
+    return off, flat
+
+
+def softmax_refinement(src, y_fine, coarse, psi_coarse, children, lam, threshold, use_lse=True,
+                       n_threads=0):
+    """SoftmaxRefinement::compute_refinement (SoftmaxRefinement.cc:200-280); `children`
+    is the level's list of child-index lists."""
+    y_fine = _f64(y_fine)
+    n_fine = y_fine.size // src.dim
+    off, flat = children_csr(children)
+    out = np.zeros(n_fine)
+    rc = lib().orc_softmax_refinement(C.byref(src._s), n_fine, _p(y_fine), C.byref(coarse._t),
+                                      _p(_f64(psi_coarse)), len(children),
+                                      off.ctypes.data_as(_i64p),
+                                      (flat if flat.size else np.zeros(1, dtype=np.int64)
+                                       ).ctypes.data_as(_i64p),
+                                      float(lam), float(threshold), int(bool(use_lse)),
+                                      int(n_threads), _p(out))
+    assert rc == 0
+    return out

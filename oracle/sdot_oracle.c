@@ -22,6 +22,12 @@
  *   orc_hessian               FD of the gradient, H*1 = 0, symmetry, PSD.
  *   orc_bfgs_generic/solve    quadratic (-> A^-1 b) and Rosenbrock minima,
  *                             converged SOT gradients below tolerance.
+ *   orc_softmax_refinement    identity hierarchy (fine = coarse, one child
+ *                             each): psi_f = psi_c - lambda log(1 + g_c/nu_c)
+ *                             with g from orc_evaluate_R at the same psi;
+ *                             a child listed under two parents gets
+ *                             -lambda log 2 more; 50-digit brute force with
+ *                             truncation on tiny inputs (tests/golden).
  * Only the last bits of summation order (R-tree traversal order, WorkStream
  * copier order, deal.II's blocked dot products) are not reproduced; they are
  * far below the 1e-10 parity bar.
@@ -609,6 +615,126 @@ int orc_hessian(const orc_source_t *s, const orc_target_t *t,
     H[k] = v;
   }
   free(tH);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* SoftmaxRefinement::compute_refinement (SoftmaxRefinement.cc:200-280) and    */
+/* its local_assemble (SoftmaxRefinement.cc:61-198): the potential of a fine   */
+/* target level from the coarse level's potential,                           */
+/*   psi_f = -lambda log sum_q (w_q / Z_q) e^{-M_q} exp(-|x_q - y_f|^2/(2 lambda)),*/
+/*   Z_q e^{M_q} = sum_c nu_c exp((psi_c - |x_q - y_c|^2/2)/lambda),             */
+/* where c runs over the coarse candidates of the cell of q (cell rule by the  */
+/* cell center, :81, :43-59) that pass the per-q test d^2 <= R^2 (:152,:162),  */
+/* and f over the children of those candidates that pass the same test (:99-  */
+/* 128, :192).  Fine points that receive nothing keep psi_f = 0 (:270).       */
+/* ------------------------------------------------------------------------ */
+
+int orc_softmax_refinement(const orc_source_t *s, int64_t n_fine,
+                           const double *y_fine, const orc_target_t *coarse,
+                           const double *psi_coarse, int64_t n_parents,
+                           const int64_t *child_off, const int64_t *child_idx,
+                           double lambda, double threshold, int use_lse,
+                           int n_threads, double *psi_fine) {
+  const int dim = s->dim;
+  const int64_t Nc = coarse->n;
+  const int nq = s->nq;
+  const double lambda_inv = 1.0 / lambda;
+  const double threshold_sq = threshold * threshold;
+  orc_params_t tp;
+  memset(&tp, 0, sizeof(tp));
+  tp.n_threads = n_threads;
+  const int nthr = orc_threads(&tp);
+  const int64_t nf_alloc = n_fine > 0 ? n_fine : 1;
+  double *tpot = (double *)calloc((size_t)nthr * (size_t)nf_alloc, sizeof(double));
+  if (!tpot) return -1;
+
+#pragma omp parallel num_threads(nthr)
+  {
+#ifdef _OPENMP
+    const int tid = omp_get_thread_num();
+#else
+    const int tid = 0;
+#endif
+    double *pot = tpot + (size_t)tid * nf_alloc;
+    int64_t *cidx = (int64_t *)malloc(sizeof(int64_t) * (Nc > 0 ? Nc : 1));
+    int64_t cap = 64, nfl = 0;
+    int64_t *fidx = (int64_t *)malloc(sizeof(int64_t) * cap);
+#pragma omp for schedule(static)
+    for (int64_t c = 0; c < s->n_cells; ++c) {
+      /* coarse candidates of the cell (:81) */
+      int64_t nt = orc_candidates(coarse, s->center + c * dim, dim, threshold, cidx);
+      if (nt == 0) continue;
+      /* children of the candidates, in candidate order (:109-128) */
+      nfl = 0;
+      for (int64_t i = 0; i < nt; ++i) {
+        const int64_t ci = cidx[i];
+        if (ci >= n_parents) continue;
+        for (int64_t k = child_off[ci]; k < child_off[ci + 1]; ++k) {
+          const int64_t f = child_idx[k];
+          if (f < 0 || f >= n_fine) continue;
+          if (nfl == cap) {
+            cap *= 2;
+            fidx = (int64_t *)realloc(fidx, sizeof(int64_t) * cap);
+          }
+          fidx[nfl++] = f;
+        }
+      }
+      if (nfl == 0) continue;
+      for (int q = 0; q < nq; ++q) {
+        const int64_t g = c * nq + q;
+        const double *x = s->x + g * dim;
+        const double density_value = s->rho[g];
+        const double JxW = s->jxw[g];
+        double total_sum_exp = 0.0;
+        double max_exponent = -DBL_MAX;
+        if (use_lse) {
+          for (int64_t i = 0; i < nt; ++i) {
+            const int64_t j = cidx[i];
+            const double local_dist2 = pow(orc_dist(x, coarse->y + j * dim, dim), 2);
+            if (local_dist2 <= threshold_sq) {
+              const double exponent = (psi_coarse[j] - 0.5 * local_dist2) * lambda_inv;
+              max_exponent = exponent > max_exponent ? exponent : max_exponent;
+            }
+          }
+          for (int64_t i = 0; i < nt; ++i) {
+            const int64_t j = cidx[i];
+            const double local_dist2 = pow(orc_dist(x, coarse->y + j * dim, dim), 2);
+            if (local_dist2 <= threshold_sq) {
+              const double shifted_exp =
+                  exp((psi_coarse[j] - 0.5 * local_dist2) * lambda_inv - max_exponent);
+              total_sum_exp += coarse->nu[j] * shifted_exp;
+            }
+          }
+        } else {
+          for (int64_t i = 0; i < nt; ++i) {
+            const int64_t j = cidx[i];
+            const double local_dist2 = pow(orc_dist(x, coarse->y + j * dim, dim), 2);
+            if (local_dist2 <= threshold_sq)
+              total_sum_exp += coarse->nu[j] * exp((psi_coarse[j] - 0.5 * local_dist2) * lambda_inv);
+          }
+        }
+        if (total_sum_exp <= 0.0) continue;
+        double scale = density_value * JxW / total_sum_exp;
+        if (use_lse) scale *= exp(-max_exponent);
+        for (int64_t i = 0; i < nfl; ++i) {
+          const int64_t f = fidx[i];
+          const double local_dist2_fine = pow(orc_dist(x, y_fine + f * dim, dim), 2);
+          if (local_dist2_fine <= threshold_sq)
+            pot[f] += scale * exp((-0.5 * local_dist2_fine) * lambda_inv);
+        }
+      }
+    }
+    free(cidx);
+    free(fidx);
+  }
+  /* copier sum, then psi_f = -lambda log(pot_f) where pot_f > 0 (:264-274) */
+  for (int64_t f = 0; f < n_fine; ++f) {
+    double v = 0.0;
+    for (int k = 0; k < nthr; ++k) v += tpot[(size_t)k * nf_alloc + f];
+    psi_fine[f] = v > 0.0 ? -lambda * log(v) : v;
+  }
+  free(tpot);
   return 0;
 }
 

@@ -109,6 +109,18 @@ int orc_discarded_fraction(const orc_source_t *s, const orc_target_t *t,
 int orc_hessian(const orc_source_t *s, const orc_target_t *t,
                 const orc_params_t *p, double R, const double *psi, double *H);
 
+/* ---- multilevel potential transfer ---- */
+/* SoftmaxRefinement::compute_refinement (SoftmaxRefinement.cc:200-280) with
+ * its local_assemble (:61-198).  The coarse-to-fine child lists of the level
+ * are CSR: children of coarse point c are child_idx[child_off[c] ..
+ * child_off[c+1]) for c < n_parents. */
+int orc_softmax_refinement(const orc_source_t *s, int64_t n_fine,
+                           const double *y_fine, const orc_target_t *coarse,
+                           const double *psi_coarse, int64_t n_parents,
+                           const int64_t *child_off, const int64_t *child_idx,
+                           double lambda, double threshold, int use_lse,
+                           int n_threads, double *psi_fine);
+
 /* ---- BFGS driver ---- */
 int orc_solve(const orc_source_t *s, const orc_target_t *t,
               const orc_params_t *p, orc_state_t *st, double *psi,

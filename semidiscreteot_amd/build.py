@@ -17,7 +17,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libsdot.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
-HIP_SOURCES = ["sdot_kernels.hip", "sdot_serve_kernels.hip"]
+HIP_SOURCES = ["sdot_kernels.hip", "sdot_serve_kernels.hip", "sdot_refine.hip"]
 CXX_SOURCES = ["sdot_api.cpp", "sdot_mesh.cpp", "sdot_prm.cpp", "sdot_serve.cpp"]
 
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",

@@ -13,6 +13,7 @@
 #include <cstring>
 #include <numeric>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "sdot_bfgs.hpp"
@@ -138,6 +139,27 @@ int sdot::set_error(int code, const std::string &msg) { return fail(code, msg); 
 double sdot::inner_s(double smax) {
   if (!(smax > 0.0) || !std::isfinite(smax)) return 0.0;
   return smax * (1.0 - 1e-12);
+}
+
+// Largest double s with pow(sqrt(s), 2) <= R*R, the per-q test of the softmax
+// refinement (pow(d, 2) is d*d, one rounding).  s -> fl(fl(sqrt(s))^2) is
+// monotone, so the test is s <= s2max.
+double sdot::s2max_for(double R) {
+  if (std::isnan(R)) return -1.0;
+  const double T = R * R;
+  if (std::isinf(T)) return INFINITY;
+  auto ok = [T](double s) {
+    const double d = std::sqrt(s);
+    return d * d <= T;
+  };
+  double s = T;
+  while (s > 0 && !ok(s)) s = std::nextafter(s, 0.0);
+  for (;;) {
+    const double n = std::nextafter(s, INFINITY);
+    if (std::isinf(n) || !ok(n)) break;
+    s = n;
+  }
+  return s;
 }
 
 // Orders of the target points (Morton for the gradient waves, row-major bins
@@ -531,7 +553,7 @@ extern "C" int sdot_ctx_destroy(sdot_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   Dev &d = c->dev;
-  void *bufs[] = {d.qx_m, d.qw_m, d.qx2_m, d.cc_m, d.blkbox, d.cc_r, d.cstart, d.cell_r2m,
+  void *bufs[] = {d.qx_m, d.qxr_m, d.qw_m, d.qx2_m, d.cc_m, d.blkbox, d.cc_r, d.cstart, d.cell_r2m,
                   d.cell_m2o, d.ty_o, d.tnu_o, d.tpos_r, d.tpos_m, d.tr2o, d.tm2o, d.tstart,
                   d.twbox, d.ty_m, d.trec_r, d.trec_m, d.qrec_r, d.Mpart, d.Spart, d.Gpart,
                   d.Fpart, d.Cpart, d.PNpart, d.psi, d.grad, d.out, d.scratch, d.i64scratch, d.Tpart,
@@ -619,7 +641,8 @@ extern "C" int sdot_set_source(sdot_ctx *c, int dim, int64_t n_cells, int nq, co
   std::vector<int64_t> m2o(n_cells);
   std::iota(m2o.begin(), m2o.end(), 0);
   std::stable_sort(m2o.begin(), m2o.end(), [&](int64_t a, int64_t b) { return key[a] < key[b]; });
-  std::vector<double> qx((size_t)nQ * dim), qw(nQ), qx2(nQ), ccm((size_t)n_cells * dim);
+  std::vector<double> qx((size_t)nQ * dim), qxr((size_t)nQ * dim), qw(nQ), qx2(nQ),
+      ccm((size_t)n_cells * dim);
   for (int64_t m = 0; m < n_cells; ++m) {
     const int64_t o = m2o[m];
     for (int d = 0; d < dim; ++d) ccm[m * dim + d] = centers[o * dim + d];
@@ -628,6 +651,7 @@ extern "C" int sdot_set_source(sdot_ctx *c, int dim, int64_t n_cells, int nq, co
       for (int d = 0; d < dim; ++d) {
         const double v = x[(o * nq + k) * dim + d] - p.origin[d];
         qx[(m * nq + k) * dim + d] = v;
+        qxr[(m * nq + k) * dim + d] = x[(o * nq + k) * dim + d];
         s2 += v * v;
       }
       qx2[m * nq + k] = s2;
@@ -668,6 +692,7 @@ extern "C" int sdot_set_source(sdot_ctx *c, int dim, int64_t n_cells, int nq, co
 
   Dev &dv = c->dev;
   HIP_TRY(upload(dv.qx_m, qx));
+  HIP_TRY(upload(dv.qxr_m, qxr));
   HIP_TRY(upload(dv.qw_m, qw));
   HIP_TRY(upload(dv.qx2_m, qx2));
   HIP_TRY(upload(dv.cc_m, ccm));
@@ -796,6 +821,160 @@ extern "C" int sdot_hessian(sdot_ctx *c, const double *psi, double *H, int64_t l
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   cleanup();
   if (e != hipSuccess) return fail(SDOT_ERR_HIP, std::string("hessian: ") + hipGetErrorString(e));
+  return SDOT_OK;
+}
+
+extern "C" int sdot_softmax_refinement(sdot_ctx *c, int64_t n_fine, const double *y_fine,
+                                       int64_t n_coarse, const double *y_coarse,
+                                       const double *nu_coarse, const double *psi_coarse,
+                                       int64_t n_parents, const int64_t *child_offsets,
+                                       const int64_t *children, double lambda,
+                                       double distance_threshold, int use_log_sum_exp_trick,
+                                       double *psi_fine) {
+  if (ctx_check(c)) return SDOT_ERR_INVALID;
+  if (!c->have_source) return fail(SDOT_ERR_STATE, "call sdot_set_source before the refinement");
+  if (n_fine < 0 || n_fine >= (1ll << 31) - 1) return fail(SDOT_ERR_INVALID, "n_fine out of range");
+  if (n_coarse < 1 || n_coarse >= (1ll << 31) - 1)
+    return fail(SDOT_ERR_INVALID, "n_coarse out of range");
+  if (n_parents < 0) return fail(SDOT_ERR_INVALID, "n_parents < 0");
+  if ((n_fine > 0 && (!y_fine || !psi_fine)) || !y_coarse || !nu_coarse || !psi_coarse ||
+      !child_offsets)
+    return fail(SDOT_ERR_INVALID, "null argument");
+  if (!(lambda > 0) || !std::isfinite(lambda))
+    return fail(SDOT_ERR_INVALID, "regularization parameter must be finite and > 0");
+  const int dim = c->plan.dim;
+  if (!all_finite(y_coarse, n_coarse * dim) || !all_finite(nu_coarse, n_coarse) ||
+      !all_finite(psi_coarse, n_coarse) || (n_fine > 0 && !all_finite(y_fine, n_fine * dim)))
+    return fail(SDOT_ERR_INVALID, "non-finite coarse or fine values");
+  if (child_offsets[0] != 0) return fail(SDOT_ERR_INVALID, "child_offsets[0] must be 0");
+  for (int64_t i = 0; i < n_parents; ++i)
+    if (child_offsets[i + 1] < child_offsets[i])
+      return fail(SDOT_ERR_INVALID, "child_offsets must be non-decreasing");
+  if (child_offsets[n_parents] > 0 && !children) return fail(SDOT_ERR_INVALID, "children is null");
+  HIP_TRY(hipSetDevice(c->device));
+  const Plan &p = c->plan;
+
+  // coarse points in bin-row order (the q-major pass streams bin runs)
+  TargetLayout L;
+  build_target_layout(dim, n_coarse, y_coarse, L);
+  std::vector<double> crec((size_t)n_coarse * kRec, 0.0);
+  for (int64_t r = 0; r < n_coarse; ++r) {
+    const int64_t j = L.r2o[r];
+    for (int d = 0; d < dim; ++d) crec[r * kRec + d] = y_coarse[j * dim + d];
+    crec[r * kRec + 3] = psi_coarse[j];
+    crec[r * kRec + 4] = nu_coarse[j];
+  }
+  // (parent, child) entries of the child lists; coarse indices without a list
+  // and child indices outside the fine level are skipped, as the reference
+  // does after reporting them (SoftmaxRefinement.cc:109-128)
+  std::vector<int64_t> ent_c, ent_f;
+  for (int64_t ci = 0; ci < std::min(n_coarse, n_parents); ++ci)
+    for (int64_t k = child_offsets[ci]; k < child_offsets[ci + 1]; ++k) {
+      const int64_t f = children[k];
+      if (f < 0 || f >= n_fine) continue;
+      ent_c.push_back(ci);
+      ent_f.push_back(f);
+    }
+  const int64_t E = (int64_t)ent_c.size();
+  std::vector<int64_t> order(E);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+    return L.pos_m[ent_c[a]] < L.pos_m[ent_c[b]];
+  });
+  std::vector<double> erec((size_t)std::max<int64_t>(E, 1) * kRec, 0.0);
+  std::vector<int64_t> foff(n_fine + 1, 0), fent(std::max<int64_t>(E, 1), 0);
+  for (int64_t k = 0; k < E; ++k) {
+    const int64_t e = order[k];
+    for (int d = 0; d < dim; ++d) {
+      erec[k * kRec + d] = y_coarse[ent_c[e] * dim + d];
+      erec[k * kRec + 3 + d] = y_fine[ent_f[e] * dim + d];
+    }
+    foff[ent_f[e] + 1]++;
+  }
+  for (int64_t f = 0; f < n_fine; ++f) foff[f + 1] += foff[f];
+  {
+    std::vector<int64_t> fill(foff.begin(), foff.end() - 1);
+    for (int64_t k = 0; k < E; ++k) fent[fill[ent_f[order[k]]]++] = k;
+  }
+
+  RefArgs a{};
+  a.tg = L.tg;
+  a.cg = p.cg;
+  a.nQ = p.nQ;
+  a.n_cells = p.n_cells;
+  a.Nc = n_coarse;
+  a.E = E;
+  a.n_fine = n_fine;
+  a.nq = p.nq;
+  a.lse = use_log_sum_exp_trick ? 1 : 0;
+  a.smax = smax_for(distance_threshold);
+  a.s2max = s2max_for(distance_threshold);
+  const double scale = std::max(c->coord_scale, L.scale);
+  a.Rq = std::isnan(distance_threshold)
+             ? 0.0
+             : distance_threshold * (1.0 + 1e-9) + 1e-9 * (L.tg.h + p.cg.h + scale);
+  a.lam = lambda;
+  a.lam_inv = 1.0 / lambda;
+  const int64_t nwq = (p.nQ + kWave - 1) / kWave, nwe = (E + kWave - 1) / kWave;
+  a.K1 = (int)std::min<int64_t>(64, std::max<int64_t>(1, 8192 / std::max<int64_t>(1, nwq)));
+  a.K2 = (int)std::min<int64_t>(64, std::max<int64_t>(1, 16384 / std::max<int64_t>(1, nwe)));
+  if (a.K2 >= 8) a.K2 -= a.K2 % 8;
+  a.qxr = c->dev.qxr_m;
+  a.qw = c->dev.qw_m;
+  a.cc = c->dev.cc_m;
+  a.cc_r = c->dev.cc_r;
+  a.cstart = c->dev.cstart;
+  a.cell_r2m = c->dev.cell_r2m;
+
+  std::vector<void *> tmp;
+  auto cleanup = [&]() {
+    for (void *b : tmp) (void)hipFree(b);
+  };
+  auto alloc = [&](auto *&ptr, size_t n) {
+    void *b = nullptr;
+    hipError_t e = hipMalloc(&b, std::max<size_t>(n, 1) * sizeof(*ptr));
+    if (e == hipSuccess) tmp.push_back(b);
+    ptr = (std::remove_reference_t<decltype(ptr)>)b;
+    return e;
+  };
+  double *d_crec = nullptr, *d_erec = nullptr, *d_M = nullptr, *d_S = nullptr, *d_rq = nullptr,
+         *d_E = nullptr, *d_psi = nullptr;
+  int32_t *d_tstart = nullptr;
+  int64_t *d_foff = nullptr, *d_fent = nullptr;
+  hipError_t e = alloc(d_crec, crec.size());
+  if (e == hipSuccess) e = alloc(d_tstart, L.tstart.size());
+  if (e == hipSuccess) e = alloc(d_erec, erec.size());
+  if (e == hipSuccess) e = alloc(d_foff, foff.size());
+  if (e == hipSuccess) e = alloc(d_fent, fent.size());
+  if (e == hipSuccess) e = alloc(d_M, (size_t)a.K1 * p.nQ);
+  if (e == hipSuccess) e = alloc(d_S, (size_t)a.K1 * p.nQ);
+  if (e == hipSuccess) e = alloc(d_rq, (size_t)p.nQ * 4);
+  if (e == hipSuccess) e = alloc(d_E, (size_t)a.K2 * std::max<int64_t>(E, 1));
+  if (e == hipSuccess) e = alloc(d_psi, (size_t)n_fine);
+  auto h2d = [&](void *dst, const void *src, size_t bytes) {
+    if (e == hipSuccess) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream);
+  };
+  h2d(d_crec, crec.data(), crec.size() * sizeof(double));
+  h2d(d_tstart, L.tstart.data(), L.tstart.size() * sizeof(int32_t));
+  h2d(d_erec, erec.data(), erec.size() * sizeof(double));
+  h2d(d_foff, foff.data(), foff.size() * sizeof(int64_t));
+  h2d(d_fent, fent.data(), fent.size() * sizeof(int64_t));
+  a.crec = d_crec;
+  a.tstart = d_tstart;
+  a.erec = d_erec;
+  a.foff = d_foff;
+  a.fent = d_fent;
+  a.Mpart = d_M;
+  a.Spart = d_S;
+  a.rq = d_rq;
+  a.Epart = d_E;
+  a.psi_fine = d_psi;
+  if (e == hipSuccess) e = launch_refinement(dim, a, c->stream);
+  if (e == hipSuccess && n_fine > 0)
+    e = hipMemcpyAsync(psi_fine, d_psi, sizeof(double) * n_fine, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  cleanup();
+  if (e != hipSuccess) return fail(SDOT_ERR_HIP, std::string("softmax refinement: ") + hipGetErrorString(e));
   return SDOT_OK;
 }
 

@@ -48,6 +48,7 @@ SDOT_HD inline int bin_of(double v, double o, double inv_h, int n) {
 struct Dev {
   // ---- source (static after sdot_set_source) ----
   double *qx_m = nullptr;      // [nQ*dim] shifted quadrature points, Morton cell order
+  double *qxr_m = nullptr;     // [nQ*dim] raw quadrature points, Morton cell order
   double *qw_m = nullptr;      // [nQ] rho*JxW
   double *qx2_m = nullptr;     // [nQ] |x'|^2 (plain-mode shift)
   double *cc_m = nullptr;      // [n_cells*dim] raw centers, Morton order
@@ -226,6 +227,38 @@ void ctx_attach(sdot_ctx *c, int delta);
 // potentials; the constants keep a factor 5-10 above that.
 constexpr double kNoiseRelF64 = 2e-15;
 constexpr double kNoiseRelF32 = 2e-9;
+
+// Softmax potential transfer from a coarse to a fine target level
+// (SoftmaxRefinement::compute_refinement, SoftmaxRefinement.cc:61-280), FP64.
+// Entries are (coarse parent, fine child) pairs of the level's child lists,
+// sorted by the Morton position of the parent.
+struct RefArgs {
+  const double *qxr;      // [nQ*DIM] raw quadrature points, Morton cell order
+  const double *qw;       // [nQ] rho*JxW, Morton cell order
+  const double *cc;       // [n_cells*DIM] raw centers, Morton order
+  const double *cc_r;     // [n_cells*4] raw centers, row (bin) order
+  const int32_t *cstart;  // row-order cell offsets per cell bin
+  const int64_t *cell_r2m;
+  const double *crec;     // [Nc*kRec] coarse points, row (bin) order: y[DIM] at 0, psi at 3, nu at 4
+  const int32_t *tstart;  // coarse bin offsets
+  const double *erec;     // [E*kRec] entries: parent y[DIM] at 0, child y[DIM] at 3
+  const int64_t *foff;    // [n_fine+1] fine point -> its entries (CSR, caller order)
+  const int64_t *fent;    // [E]
+  double *Mpart, *Spart;  // [K1*nQ] Morton cell order
+  double *rq;             // [nQ*4] row cell order: x[DIM] at 0, w e^{-M} / S at 3 (0: skipped q)
+  double *Epart;          // [K2*E]
+  double *psi_fine;       // [n_fine]
+  Grid tg, cg;
+  int64_t nQ, n_cells, Nc, E, n_fine;
+  int nq, K1, K2, lse;
+  double smax;   // cell rule:  |y_c - center| <= R   <=>  s <= smax
+  double s2max;  // per-q rule: pow(|x_q - y|, 2) <= R*R  <=>  s <= s2max
+  double Rq, lam, lam_inv;
+};
+// Largest double s with pow(sqrt(s), 2) <= R*R (the per-q test of
+// SoftmaxRefinement.cc:152,162,173,192).
+double s2max_for(double R);
+hipError_t launch_refinement(int dim, const RefArgs &a, hipStream_t s);
 
 // Kernel launchers (sdot_kernels.hip).
 hipError_t launch_eval(const Plan &p, Dev &d, const double *d_psi, double *d_grad,

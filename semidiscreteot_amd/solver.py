@@ -13,6 +13,7 @@ from dataclasses import dataclass, fields
 import numpy as np
 
 from . import _lib as L
+from .epsilon_scaling import EpsilonScalingHandler
 
 _THRESH = {"pointwise": 0, "integral": 1, "geometric": 2}
 _CTRL = {"l1norm": 0, "componentwise": 1}
@@ -208,15 +209,18 @@ class SotSolver:
         if params is not None:
             self.configure(params)
         p = self.current_params
-        steps = max(1, int(p.epsilon_scaling_steps)) if p.use_epsilon_scaling else 1
-        factor = p.epsilon_scaling_factor if p.epsilon_scaling_factor > 1.0 else 2.0
-        eps_final = p.epsilon
+        if p.use_epsilon_scaling:
+            seq = EpsilonScalingHandler(p.epsilon, p.epsilon_scaling_factor,
+                                        p.epsilon_scaling_steps).generate_epsilon_sequence()
+            seq = seq or [p.epsilon]
+        else:
+            seq = [p.epsilon]
         psi = None if potential is None else _f64(potential).copy()
         infos = []
         try:
-            for i in range(steps):
+            for eps in seq:
                 stage = SolverParameters(**{f.name: getattr(p, f.name) for f in fields(p)})
-                stage.epsilon = eps_final * factor ** (steps - 1 - i)
+                stage.epsilon = eps
                 psi = self.solve(psi, stage)
                 infos.append(dict(self.get_last_solve_info(), epsilon=stage.epsilon))
         finally:

@@ -15,6 +15,9 @@ SotSolver.cc:854).  Inputs are stored as the exact doubles used, so the
 values are the mathematical ones for those inputs; quadrature points come
 from textbook Gauss-Legendre nodes, not from the oracle.
 
+refine_cases.json holds the same kind of values for the softmax potential
+transfer between target levels (refine() below).
+
 Run: python tests/golden/make_golden.py   (needs mpmath and numpy only)
 """
 import json
@@ -138,6 +141,75 @@ def case(name, dim, n, N, seed, eps, R, lse, cluster=None):
     }
 
 
+def refine(dim, nq, x, rho, jxw, cen, yc, nuc, psic, yf, children, lam, R):
+    """SoftmaxRefinement::compute_refinement (SoftmaxRefinement.cc:61-280):
+        pot_f = sum_q w_q exp(-|x_q - y_f|^2 / (2 lam)) / Z_q,
+        Z_q   = sum_c nu_c exp((psi_c - |x_q - y_c|^2 / 2) / lam),
+    c over the coarse points within R of the center of the cell of q that are
+    within R of x_q, f over the children of those coarse points (a child
+    listed k times counts k times) within R of x_q; psi_f = -lam log pot_f,
+    0 where nothing arrives."""
+    lam = mp.mpf(lam)
+    Rm = mp.mpf(R)
+    pot = [mp.mpf(0)] * len(yf)
+    for c in range(len(cen)):
+        cand = [j for j in range(len(yc))
+                if mp.sqrt(sum((mp.mpf(yc[j][k]) - mp.mpf(cen[c][k])) ** 2 for k in range(dim))) <= Rm]
+        fine = [f for j in cand if j < len(children) for f in children[j] if 0 <= f < len(yf)]
+        for q in range(nq):
+            i = c * nq + q
+            Z = mp.mpf(0)
+            for j in cand:
+                d2 = sum((mp.mpf(x[i][k]) - mp.mpf(yc[j][k])) ** 2 for k in range(dim))
+                if mp.sqrt(d2) <= Rm:
+                    Z += mp.mpf(nuc[j]) * mp.exp((mp.mpf(psic[j]) - d2 / 2) / lam)
+            if Z == 0:
+                continue
+            w = mp.mpf(rho[i]) * mp.mpf(jxw[i])
+            for f in fine:
+                d2 = sum((mp.mpf(x[i][k]) - mp.mpf(yf[f][k])) ** 2 for k in range(dim))
+                if mp.sqrt(d2) <= Rm:
+                    pot[f] += w * mp.exp(-d2 / (2 * lam)) / Z
+    return [(-lam * mp.log(v)) if v > 0 else mp.mpf(0) for v in pot]
+
+
+def refine_case(name, dim, n, Nc, Nf, seed, lam, R, lse):
+    x, rho, jxw, cen, nq = box_source(dim, n, seed)
+    rng = np.random.default_rng(seed + 200)
+    yc = rng.random((Nc, dim))
+    nuc = rng.random(Nc) + 0.2
+    nuc = nuc / nuc.sum()
+    psic = 0.05 * rng.standard_normal(Nc)
+    # fine points scattered around their parents, every coarse point a parent
+    parent = np.concatenate([np.arange(Nc), rng.integers(0, Nc, Nf - Nc)])
+    yf = yc[parent] + 0.08 * (rng.random((Nf, dim)) - 0.5)
+    children = [[int(f) for f in np.flatnonzero(parent == c)] for c in range(Nc)]
+    # one fine point listed under a second parent, one invalid index
+    children[0].append(int(np.flatnonzero(parent == 1)[0]))
+    children[-1].append(Nf + 3)
+    psif = refine(dim, nq, x, rho, jxw, cen, yc, nuc, psic, yf, children, lam, R)
+    return {
+        "name": name, "dim": dim, "nq": nq, "lambda": lam, "R": R, "use_lse": lse,
+        "x": x.ravel().tolist(), "rho": rho.tolist(), "jxw": jxw.tolist(),
+        "center": cen.ravel().tolist(), "y_coarse": yc.ravel().tolist(), "nu_coarse": nuc.tolist(),
+        "psi_coarse": psic.tolist(), "y_fine": yf.ravel().tolist(), "children": children,
+        "psi_fine": [mp.nstr(v, 40) for v in psif],
+    }
+
+
+def main_refine():
+    cases = [
+        refine_case("2d_truncated_lse", 2, 3, 6, 20, 11, 0.02, 0.35, True),
+        refine_case("2d_untruncated_plain", 2, 2, 5, 12, 12, 0.05, 1.7976931348623157e308, False),
+        refine_case("3d_truncated_lse", 3, 2, 5, 14, 13, 0.03, 0.45, True),
+        refine_case("2d_tight_lse", 2, 3, 7, 18, 14, 0.01, 0.12, True),
+    ]
+    out = os.path.join(HERE, "refine_cases.json")
+    with open(out, "w") as f:
+        json.dump(cases, f)
+    print("wrote", out)
+
+
 def main():
     cases = [
         case("2d_untruncated_lse", 2, 2, 5, 1, 0.05, 1.7976931348623157e308, True),
@@ -155,3 +227,4 @@ def main():
 
 if __name__ == "__main__":
     main()
+    main_refine()

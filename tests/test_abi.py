@@ -140,6 +140,10 @@ def test_cpp_facade_compiles_and_links(tmp_path):
     import subprocess
     src = tmp_path / "facade.cpp"
     src.write_text('#include "sdot_sot_solver.hpp"\n'
+                   "std::vector<double> refine(const sdot::SotSolverGPU<3> &s,\n"
+                   "    const std::vector<std::vector<std::vector<size_t>>> &ch) {\n"
+                   "  sdot::SoftmaxRefinementGPU<3> r(s, 0.1, true);\n"
+                   "  return r.compute_refinement({}, {}, {}, {}, {}, 0.1, 0, ch); }\n"
                    "int main() { auto p = sdot::SotSolverGPU<3>::default_parameters();\n"
                    "  return (p.max_iterations == 1000 && p.tau == 1e-8) ? 0 : 1; }\n")
     exe = tmp_path / "facade"

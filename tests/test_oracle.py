@@ -190,3 +190,55 @@ def test_tet_source_mass():
     src, tgt, _ = tet_case(300, 20, seed=3)
     assert abs(src.w.sum() - 1.0) < 1e-12
     assert src.nq == 4
+
+
+# ---- softmax potential transfer (SoftmaxRefinement.cc:61-280) ----
+
+REFINE = json.load(open(os.path.join(HERE, "golden", "refine_cases.json")))
+
+
+@pytest.mark.parametrize("case", REFINE, ids=[c["name"] for c in REFINE])
+def test_refinement_matches_50_digit_golden(case):
+    """Truncated and untruncated transfers with a child listed under two
+    parents, an out-of-range child index, and (tight case) fine points that
+    receive nothing and keep 0."""
+    src = O.Source(case["dim"], case["nq"], case["x"], case["rho"], case["jxw"], case["center"])
+    coarse = O.Target(case["y_coarse"], case["nu_coarse"])
+    psi = O.softmax_refinement(src, case["y_fine"], coarse, case["psi_coarse"], case["children"],
+                               case["lambda"], case["R"], case["use_lse"], n_threads=2)
+    ref = np.array([float(v) for v in case["psi_fine"]])
+    assert np.array_equal(psi == 0.0, ref == 0.0)
+    assert np.max(np.abs(psi - ref)) <= 1e-15
+
+
+@pytest.mark.parametrize("lse", [True, False])
+def test_refinement_identity_hierarchy(lse):
+    """Fine level = coarse level, each point its own child, no truncation:
+    sum_q w_q exp(-d^2/2l)/Z_q = (g_c + nu_c) / nu_c * exp(-psi_c/l), so
+    psi_f = psi_c - l log(1 + g_c / nu_c) with g the gradient the evaluation
+    assembles at the same potential and epsilon = l."""
+    src, tgt = structured_case(2, 10, 2, 30, 7, gmm_source=True, dirichlet=3.0)
+    rng = np.random.default_rng(8)
+    psi = 0.02 * rng.standard_normal(tgt.n)
+    lam = 0.015
+    prm = O.params(epsilon=lam, use_lse=lse, n_threads=2)
+    _, g, _ = O.evaluate_R(src, tgt, prm, np.finfo(float).max, psi)
+    children = [[i] for i in range(tgt.n)]
+    pf = O.softmax_refinement(src, tgt.y, tgt, psi, children, lam, np.finfo(float).max, lse)
+    assert np.max(np.abs(pf - (psi - lam * np.log1p(g / tgt.nu)))) <= 1e-14
+    # a child listed under a second parent receives its sum twice: -l log 2 more
+    children[3].append(5)
+    pf2 = O.softmax_refinement(src, tgt.y, tgt, psi, children, lam, np.finfo(float).max, lse)
+    assert abs(pf2[5] - (pf[5] - lam * np.log(2.0))) <= 1e-14
+    assert np.array_equal(np.delete(pf2, 5), np.delete(pf, 5))
+
+
+def test_refinement_threshold_below_every_distance():
+    """No coarse point within the threshold of any cell center: nothing is
+    transferred and every fine potential stays 0."""
+    src, tgt = structured_case(2, 6, 2, 9, 3)
+    y = np.full(tgt.n * 2, 5.0)
+    far = O.Target(y, tgt.nu)
+    pf = O.softmax_refinement(src, tgt.y, far, np.zeros(tgt.n), [[i] for i in range(tgt.n)],
+                              0.01, 1.0, True)
+    assert np.all(pf == 0.0)
