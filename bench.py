@@ -194,6 +194,7 @@ def serving_leg(s, args):
         ta = time.perf_counter()
         s.setup_target(r.y, r.nu)
         prm = sd.SolverParameters(epsilon=r.epsilon, tau=0.0, tolerance=r.tolerance,
+                                  verbose_output=False,
                                   solver_control_type="componentwise",
                                   max_iterations=args.serve_max_it, use_log_sum_exp_trick=True,
                                   precision="fp64", line_search="noise_aware",
@@ -226,6 +227,66 @@ def serving_leg(s, args):
     }
 
 
+def multilevel_leg(args):
+    """The combined multilevel workload end to end (16^3 -> 32^3 -> 64^3 hexes,
+    1,000 -> 10,000 -> 100,000 targets, epsilon 1e-1 -> 1e-2 -> 1e-3, FP32_LSE
+    solves finished in FP64, softmax transfer between target levels): wall
+    time per combined level, iterations, transfer time, and the FP64 relative
+    marginal error of the final potential.  The solver's iteration lines go
+    to stderr."""
+    import tempfile
+    import semidiscreteot_amd as sd
+    from semidiscreteot_amd import workloads
+    t0 = time.perf_counter()
+    wl = workloads.multilevel_3d()
+    log(f"multilevel workload {wl.name} built in {time.perf_counter() - t0:.1f}s")
+    pts, dens, par = wl.hierarchy
+    hdir = tempfile.mkdtemp(prefix="sdot_hier_")
+    from semidiscreteot_amd import formats
+    formats.write_target_hierarchy(hdir, pts, dens, par)
+    s = sd.SotSolver(3)
+    prm = sd.SolverParameters(epsilon=wl.epsilon, tau=wl.tau, tolerance=1e-6,
+                              solver_control_type="componentwise",
+                              max_iterations=args.ml_max_it, use_log_sum_exp_trick=True,
+                              precision="fp32_lse", line_search="noise_aware",
+                              use_epsilon_scaling=True, epsilon_scaling_factor=wl.epsilon_factor,
+                              epsilon_scaling_steps=wl.epsilon_steps, quadrature_order=2,
+                              verbose_output=True)
+    ml = sd.MultilevelParameters(target_hierarchy_dir=hdir)
+    saved = os.dup(1)
+    sys.stdout.flush()
+    os.dup2(2, 1)
+    try:
+        ta = time.perf_counter()
+        psi, recs = sd.run_combined_multilevel(s, wl.sources, prm, ml, log=log)
+        total = time.perf_counter() - ta
+    finally:
+        sys.stdout.flush()
+        os.dup2(saved, 1)
+        os.close(saved)
+    s.setup_target(pts[0].ravel(), dens[0])
+    s.configure(sd.SolverParameters(epsilon=wl.epsilon, tau=0.0, use_log_sum_exp_trick=True,
+                                    verbose_output=False))
+    # FP64 relative marginal error at the final potential: with the last
+    # solve's truncation radius (the problem the solve stops on) and without
+    # truncation (what the reference's pointwise rule leaves out)
+    R_last = recs[-1].threshold
+    _, g, _ = s.evaluate_functional_R(R_last, psi)
+    err = float(np.max(np.abs(g) / dens[0]))
+    _, g, _ = s.evaluate_functional_R(np.finfo(float).max, psi)
+    err_full = float(np.max(np.abs(g) / dens[0]))
+    dm = s.discarded_mass(R_last, psi, wl.sources[-1].w)
+    log(f"multilevel: {total:.2f}s total, final FP64 relative marginal error {err:.3e} at R = "
+        f"{R_last:.4f} (untruncated {err_full:.3e}, discarded mass {dm:.2e})")
+    return {"workload": wl.name, "seconds": total, "fp64_rel_marginal_err": err,
+            "final_R": R_last, "fp64_rel_marginal_err_untruncated": err_full,
+            "discarded_mass": dm,
+            "levels": [{"target_level": r.level, "n_targets": r.n_targets, "n_cells": r.n_cells,
+                        "epsilons": r.epsilons, "iterations": r.iterations, "status": r.status,
+                        "seconds": r.seconds, "transfer_seconds": r.transfer_seconds,
+                        "threshold": r.threshold} for r in recs]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -241,6 +302,10 @@ def main():
     ap.add_argument("--serve-max-it", type=int, default=1000)
     ap.add_argument("--serve-eps-steps", type=int, default=1,
                     help="epsilon-scaling stages per request (1: none)")
+    ap.add_argument("--multilevel", action="store_true",
+                    help="also run the combined multilevel workload end to end")
+    ap.add_argument("--ml-max-it", type=int, default=3000,
+                    help="iteration budget of each multilevel solve")
     ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_BYTES"),
                     help="summarise two rocprofv3 --pmc runs into per-kernel HBM traffic and exit")
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -266,6 +331,7 @@ def main():
     # psi: an untruncated FP64 L-BFGS run from 0 (tau = 0 disables truncation;
     # FP64, so that the FP32 kernels of a profiled run are the timed ones only)
     params = sd.SolverParameters(epsilon=wl.epsilon, tau=0.0, use_log_sum_exp_trick=True,
+                                 verbose_output=False,
                                  solver_control_type="componentwise", tolerance=1e-6,
                                  max_iterations=args.psi_iters, precision="fp64")
     s.configure(params)
@@ -341,6 +407,10 @@ def main():
     if args.serve_requests > 0:
         log("serving leg ...")
         serving = serving_leg(s, args)
+    multilevel = None
+    if args.multilevel:
+        log("multilevel leg ...")
+        multilevel = multilevel_leg(args)
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -376,6 +446,7 @@ def main():
             "F_rel_diff_vs_fp32": abs(results["fp64"]["F"] - r["F"]) / abs(results["fp64"]["F"])},
         "kernel_ms": kern,
         "serving": serving,
+        "multilevel": multilevel,
         "cpu_baseline": cpu,
     }
     if args.copy_bytes > 0:

@@ -155,7 +155,11 @@ int sdot_synchronize(sdot_ctx *ctx);
 int sdot_hessian(sdot_ctx *ctx, const double *psi, double *H, int64_t ld);
 
 /* SotSolver::solve (SotSolver.h:211-212, SotSolver.cc:105-215): L-BFGS with
- * the reference's line search and stopping test; psi in/out (size N). */
+ * the reference's line search and stopping test; psi in/out (size N).  In
+ * FP32_LSE precision the iteration runs on FP32 evaluations until its stopping
+ * value is within 10x the tolerance (or a line search fails), then continues
+ * from there on FP64 evaluations within the same iteration budget, so that
+ * the stopping decision is taken on an FP64 gradient. */
 int sdot_solve(sdot_ctx *ctx, double *psi, sdot_solve_info *info);
 
 /* ---- multilevel potential transfer ---------------------------------------- */

@@ -1007,30 +1007,93 @@ extern "C" int sdot_solve(sdot_ctx *c, double *psi, sdot_solve_info *info) {
   const double tol = comp ? DBL_MIN : c->params.tolerance;
   const double utol = c->params.tolerance;
   const unsigned max_it = c->params.max_iterations;
-  auto check = [&](int step, const double *g, double &cv) -> int {
+  // stopping value at tolerance scale `mult` (SotSolver.h:382-446)
+  auto check_value = [&](const double *g, double mult) {
+    double cv;
     if (comp) {
       cv = -INFINITY;
-      for (int64_t j = 0; j < N; ++j) cv = std::max(cv, std::fabs(g[j]) - c->h_nu[j] * utol);
+      for (int64_t j = 0; j < N; ++j) cv = std::max(cv, std::fabs(g[j]) - c->h_nu[j] * utol * mult);
     } else {
       cv = 0.0;
       for (int64_t j = 0; j < N; ++j) cv += std::fabs(g[j]);
     }
+    return cv;
+  };
+  // the iteration line VerboseSolverControl::check prints (SotSolver.h:408-440)
+  double l1_first = 0.0;
+  auto report = [&](int step, const double *g, double cv) {
+    if (!c->params.verbose_output) return;
+    double l2 = 0.0;
+    for (int64_t j = 0; j < N; ++j) l2 += g[j] * g[j];
+    l2 = std::sqrt(l2);
+    if (comp) {
+      std::printf("Iteration %d - L-2 gradient norm: %g - Max Scaled Residual (max |g_i| - T_i*tol): %g\n",
+                  step, l2, cv);
+    } else {
+      const double rel = (step == 0 || l1_first == 0.0) ? cv : cv / l1_first;
+      if (step == 0) l1_first = cv;
+      std::printf("Iteration %d - L-2 gradient norm: %g - L-1 gradient norm: %g - Relative L-1 residual: %g\n",
+                  step, l2, cv, rel);
+    }
+    std::fflush(stdout);
+  };
+  auto check = [&](int step, const double *g, double &cv) -> int {
+    cv = check_value(g, 1.0);
+    report(step, g, cv);
     if (cv <= tol) return kSuccess;
     if ((unsigned)step >= max_it || std::isnan(cv)) return kFailure;
     return kIterate;
   };
-  double noise_rel = 0.0;
-  if (c->params.line_search == SDOT_LS_NOISE_AWARE)
-    noise_rel = (c->params.use_log_sum_exp_trick && c->params.precision == SDOT_PREC_FP32_LSE)
-                    ? kNoiseRelF32 : kNoiseRelF64;
-  BfgsResult r = lbfgs_solve(N, psi, eval, check, 5, noise_rel);
-  if (eval_rc) return eval_rc;
-  c->last_iterations = r.iterations;
+  const bool noise_aware = c->params.line_search == SDOT_LS_NOISE_AWARE;
+  const bool low = c->params.use_log_sum_exp_trick && c->params.precision == SDOT_PREC_FP32_LSE;
+  BfgsResult r;
+  int its = 0, evals = 0;
+  if (low) {
+    // FP32_LSE: iterate on FP32 evaluations until the stopping value is within
+    // kPolishFactor of the tolerance (or the budget ends, or a line search
+    // fails), then finish on FP64 evaluations, so that the stopping decision
+    // is taken on an FP64 gradient
+    auto check_low = [&](int step, const double *g, double &cv) -> int {
+      report(step, g, check_value(g, 1.0));
+      cv = check_value(g, kPolishFactor);
+      if (cv <= (comp ? DBL_MIN : utol * kPolishFactor)) return kSuccess;
+      if ((unsigned)step >= max_it || std::isnan(cv)) return kFailure;
+      return kIterate;
+    };
+    r = lbfgs_solve(N, psi, eval, check_low, 5, noise_aware ? kNoiseRelF32 : 0.0);
+    if (eval_rc) return eval_rc;
+    its = r.iterations;
+    evals = r.evaluations;
+    if (r.status != kNonFinite) {
+      const unsigned left = max_it > (unsigned)its ? max_it - (unsigned)its : 0u;
+      const int offset = its;
+      auto check_hi = [&](int step, const double *g, double &cv) -> int {
+        cv = check_value(g, 1.0);
+        report(offset + step, g, cv);
+        if (cv <= tol) return kSuccess;
+        if ((unsigned)step >= left || std::isnan(cv)) return kFailure;
+        return kIterate;
+      };
+      c->force_mode = kF64Lse;
+      c->hist_valid = false;
+      r = lbfgs_solve(N, psi, eval, check_hi, 5, noise_aware ? kNoiseRelF64 : 0.0);
+      c->force_mode = -1;
+      if (eval_rc) return eval_rc;
+      its += r.iterations;
+      evals += r.evaluations;
+    }
+  } else {
+    r = lbfgs_solve(N, psi, eval, check, 5, noise_aware ? kNoiseRelF64 : 0.0);
+    if (eval_rc) return eval_rc;
+    its = r.iterations;
+    evals = r.evaluations;
+  }
+  c->last_iterations = its;
   c->converged = r.status == kConverged;
   if (info) {
     info->status = r.status;
-    info->iterations = r.iterations;
-    info->evaluations = r.evaluations;
+    info->iterations = its;
+    info->evaluations = evals;
     info->functional = r.f;
     info->check_value = r.check;
     info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -118,15 +118,31 @@ def kmeans(points: np.ndarray, k: int, rng: np.random.Generator, max_iter: int =
     Returns (centers [k, dim], assignment [n])."""
     from scipy.spatial import cKDTree
 
-    n = len(points)
-    centers = np.empty((k, points.shape[1]))
+    n, dim = points.shape
+    cols = [np.ascontiguousarray(points[:, d]) for d in range(dim)]
+    tmp = np.empty(n)
+
+    def dist2(c, out):
+        np.subtract(cols[0], c[0], out=out)
+        np.square(out, out=out)
+        for d in range(1, dim):
+            np.subtract(cols[d], c[d], out=tmp)
+            np.square(tmp, out=tmp)
+            out += tmp
+        return out
+
+    centers = np.empty((k, dim))
     centers[0] = points[rng.integers(n)]
-    d2 = np.sum((points - centers[0]) ** 2, axis=1)
+    d2 = dist2(centers[0], np.empty(n))
+    cand = np.empty(n)
     for i in range(1, k):
-        tot = d2.sum()
-        j = int(rng.choice(n, p=d2 / tot)) if tot > 0 else int(rng.integers(n))
-        centers[i] = points[j]
-        np.minimum(d2, np.sum((points - centers[i]) ** 2, axis=1), out=d2)
+        # D^2 sampling: the next center with probability proportional to the
+        # squared distance to the nearest chosen center
+        cum = np.cumsum(d2)
+        j = int(np.searchsorted(cum, rng.random() * cum[-1], side="right")) if cum[-1] > 0 \
+            else int(rng.integers(n))
+        centers[i] = points[min(j, n - 1)]
+        np.minimum(d2, dist2(centers[i], cand), out=d2)
     assign = None
     for _ in range(max_iter):
         _, a = cKDTree(centers).query(points)

@@ -10,6 +10,9 @@ mesh extraction so that the measured path never touches the oracle.
              U[0.2,0.8]^3, sigma U[0.05,0.15], seed 1); 50,000 targets from a
              6-component mixture (seed 101), nu ~ Dirichlet(5); eps = 1e-3 with
              pointwise truncation.
+  multilevel_3d  the gmm_3d density on 16^3 -> 32^3 -> 64^3 hexes, targets
+             1,000 -> 10,000 -> 100,000 (k-means levels of the 100,000 set,
+             seed 2), epsilon 1e-1 -> 1e-2 -> 1e-3.
 """
 from __future__ import annotations
 
@@ -78,6 +81,44 @@ def gmm_3d(n_cells=32, n_targets=50000, seed=1, eps=1e-3, tau=1e-16) -> Workload
     nu = trng.dirichlet(np.full(n_targets, 5.0))
     nu = nu / nu.sum()
     return Workload(f"gmm_3d_{n_cells}cube_q2_N{n_targets}", src, y.ravel(), nu, eps, tau)
+
+
+@dataclass
+class MultilevelWorkload:
+    name: str
+    sources: list          # SourceArrays per source level, coarsest first
+    hierarchy: tuple       # (points, densities, parents) per target level, level 0 finest
+    epsilon: float         # finest epsilon
+    epsilon_factor: float
+    epsilon_steps: int
+    tau: float
+
+
+def multilevel_3d(cells=(16, 32, 64), n_targets=100000, sizes=(100000, 10000, 1000),
+                  seed=1, target_seed=101, kmeans_seed=2, eps=1e-3, eps_factor=10.0,
+                  tau=1e-12) -> MultilevelWorkload:
+    """Combined multilevel workload: the gmm_3d source density on 16^3, 32^3
+    and 64^3 Q1 hexes with QGauss(2) (32,768 / 262,144 / 2,097,152 q), the
+    gmm_3d target mixture with 100,000 points (nu ~ Dirichlet(5)) and its
+    coarse levels of 10,000 and 1,000 points from seeded k-means (seed 2)
+    with the weights summed per cluster; epsilon annealed 1e-1 -> 1e-2 -> 1e-3
+    (one value per combined level)."""
+    from .multilevel import generate_target_hierarchy
+    rng = np.random.default_rng(seed)
+    means = rng.uniform(0.2, 0.8, size=(4, 3))
+    sig = rng.uniform(0.05, 0.15, size=4)
+    sources = []
+    for n in cells:
+        rho = gmm_pdf(box_nodes(3, n), means, sig)
+        sources.append(structured_source(3, [n] * 3, [0.0] * 3, [1.0] * 3, 2, rho))
+    trng = np.random.default_rng(target_seed)
+    y = gmm_points(trng, n_targets, 3, 6)
+    nu = trng.dirichlet(np.full(n_targets, 5.0))
+    nu = nu / nu.sum()
+    hier = generate_target_hierarchy(y, nu, None, sizes=list(sizes), seed=kmeans_seed)
+    name = "multilevel_3d_" + "-".join(f"{n}cube" for n in cells) + "_N" + \
+        "-".join(str(s) for s in reversed(sizes))
+    return MultilevelWorkload(name, sources, hier, eps, eps_factor, len(cells), tau)
 
 
 @dataclass

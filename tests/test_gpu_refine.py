@@ -177,3 +177,37 @@ def test_target_multilevel_driver(tmp_path):
     s.configure(prm)
     _, g, _ = s.evaluate_functional_R(DMAX, psi)
     assert np.max(np.abs(g) / nl) <= 1e-7 * 1.001
+
+
+def test_combined_multilevel_driver(tmp_path):
+    """run_combined_multilevel over two source levels (8^3, 16^3 hexes) and
+    three target levels, epsilon 0.09 -> 0.03 -> 0.01 spread over the
+    combined levels, FP32_LSE solves: the source stays at its coarsest level
+    until the target hierarchy has as many levels left, every level reaches
+    the tolerance, and the final potential solves the finest problem in FP64
+    within 1e-6 relative marginal error."""
+    say("combined multilevel driver")
+    rng = np.random.default_rng(61)
+    srcs = []
+    for n in (8, 16):
+        src_o, _ = structured_case(3, n, 2, 4, 61, gmm_source=True)
+        srcs.append(sd.SourceArrays(3, src_o.nq, src_o.x, src_o.w, src_o.center))
+    y = gmm_sample(rng, 1500, 3, 5)
+    nu = rng.dirichlet(np.full(1500, 5.0))
+    hdir = str(tmp_path / "hier")
+    sd.generate_target_hierarchy(y, nu, hdir, sizes=[1500, 300, 60], seed=2)
+    s = sd.SotSolver(3)
+    prm = sd.SolverParameters(epsilon=0.01, tau=0.0, tolerance=1e-6,
+                              solver_control_type="componentwise", max_iterations=3000,
+                              use_log_sum_exp_trick=True, precision="fp32_lse",
+                              line_search="noise_aware", use_epsilon_scaling=True,
+                              epsilon_scaling_factor=3.0, epsilon_scaling_steps=3)
+    ml = sd.MultilevelParameters(target_hierarchy_dir=hdir)
+    psi, recs = sd.run_combined_multilevel(s, srcs, prm, ml)
+    assert [(r.level, r.n_cells) for r in recs] == [(2, 512), (1, 512), (0, 4096)]
+    assert [r.epsilons for r in recs] == [[0.09], [0.03], [0.01]]
+    assert all(r.status == "converged" for r in recs), [(r.level, r.status) for r in recs]
+    s.setup_target(y.ravel(), nu)
+    s.configure(sd.SolverParameters(epsilon=0.01, tau=0.0, use_log_sum_exp_trick=True))
+    _, g, _ = s.evaluate_functional_R(DMAX, psi)
+    assert np.max(np.abs(g) / nu) <= 1e-6 * 1.001
