@@ -309,6 +309,16 @@ int prepare(sdot_ctx *c, double R) {
   p.hist = (p.mode == kF32Lse && c->hist_valid && c->hist_R == R) ? 1 : 0;
   if (!std::isfinite(p.Rq)) p.dense = std::isnan(R) ? 0 : 1;
   if (std::isnan(R)) p.Rq = 0.0;
+  // truncated sweeps run the small grouping (same R, same grouping: the FP32
+  // history stays valid)
+  {
+    const Grouping &gr = (!p.dense && p.grp_small.G != p.grp_big.G) ? p.grp_small : p.grp_big;
+    p.small = &gr == &p.grp_small ? 1 : 0;
+    p.G = gr.G;
+    p.gpc = gr.gpc;
+    p.n_groups = gr.n_groups;
+    p.n_blocks = gr.n_blocks;
+  }
   // enough waves to fill 256 CUs: the q-major kernel runs 4 waves per SIMD, the
   // target-major one up to 12
   int64_t K1 = std::max<int64_t>(1, 8192 / std::max<int64_t>(1, p.n_blocks));
@@ -361,6 +371,8 @@ int prepare(sdot_ctx *c, double R) {
     HIP_TRY(dalloc(d.grad, p.N));
     HIP_TRY(dalloc(d.psi_prev, p.N));
     HIP_TRY(dalloc(d.dpsi_r, p.N));
+    HIP_TRY(dalloc(d.dpsi_cx, (p.N + 63) / 64));
+    HIP_TRY(dalloc(d.dpsi_cn, (p.N + 63) / 64));
     c->cap_N = p.N;
     c->hist_valid = false;
     p.hist = 0;
@@ -553,11 +565,11 @@ extern "C" int sdot_ctx_destroy(sdot_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   Dev &d = c->dev;
-  void *bufs[] = {d.qx_m, d.qxr_m, d.qw_m, d.qx2_m, d.cc_m, d.blkbox, d.cc_r, d.cstart, d.cell_r2m,
+  void *bufs[] = {d.qx_m, d.qxr_m, d.qw_m, d.qx2_m, d.cc_m, d.blkbox, d.blkbox_s, d.cc_r, d.cstart, d.cell_r2m,
                   d.cell_m2o, d.ty_o, d.tnu_o, d.tpos_r, d.tpos_m, d.tr2o, d.tm2o, d.tstart,
                   d.twbox, d.ty_m, d.trec_r, d.trec_m, d.qrec_r, d.Mpart, d.Spart, d.Gpart,
                   d.Fpart, d.Cpart, d.PNpart, d.psi, d.grad, d.out, d.scratch, d.i64scratch, d.Tpart,
-                  d.Upart, d.psi_prev, d.dpsi_r, d.ucount};
+                  d.Upart, d.psi_prev, d.dpsi_r, d.dpsi_cx, d.dpsi_cn, d.ucount};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto &set : c->ev_pool)
@@ -599,16 +611,25 @@ extern "C" int sdot_set_source(sdot_ctx *c, int dim, int64_t n_cells, int nq, co
   p.n_cells = n_cells;
   p.nq = nq;
   p.nQ = nQ;
-  int G = 1;
-  for (int g = 9; g >= 1; --g)
-    if (nq % g == 0) {
-      G = g;
-      break;
-    }
-  p.G = G;
-  p.gpc = nq / G;
-  p.n_groups = n_cells * p.gpc;
-  p.n_blocks = (p.n_groups + kWave - 1) / kWave;
+  auto grouping = [&](int gmax) {
+    Grouping gr;
+    for (int g = gmax; g >= 1; --g)
+      if (nq % g == 0) {
+        gr.G = g;
+        break;
+      }
+    gr.gpc = nq / gr.G;
+    gr.n_groups = n_cells * gr.gpc;
+    gr.n_blocks = (gr.n_groups + kWave - 1) / kWave;
+    return gr;
+  };
+  p.grp_big = grouping(9);
+  p.grp_small = grouping(4);
+  p.small = 0;
+  p.G = p.grp_big.G;
+  p.gpc = p.grp_big.gpc;
+  p.n_groups = p.grp_big.n_groups;
+  p.n_blocks = p.grp_big.n_blocks;
 
   // bounding boxes; the shift origin is the middle of the quadrature-point box
   double qlo[3], qhi[3], clo[3], chi[3];
@@ -658,23 +679,27 @@ extern "C" int sdot_set_source(sdot_ctx *c, int dim, int64_t n_cells, int nq, co
       qw[m * nq + k] = w[o * nq + k];
     }
   }
-  // AABB of the cell centers of each 64-group block
-  std::vector<double> box((size_t)p.n_blocks * 2 * dim);
-  for (int64_t b = 0; b < p.n_blocks; ++b) {
-    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    const int64_t g0 = b * kWave, g1 = std::min<int64_t>(p.n_groups, g0 + kWave);
-    for (int64_t g = g0; g < g1; ++g) {
-      const int64_t cm = g / p.gpc;
+  // AABB of the cell centers of each 64-group block, for both groupings
+  auto boxes = [&](const Grouping &gr) {
+    std::vector<double> box((size_t)gr.n_blocks * 2 * dim);
+    for (int64_t b = 0; b < gr.n_blocks; ++b) {
+      double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      const int64_t g0 = b * kWave, g1 = std::min<int64_t>(gr.n_groups, g0 + kWave);
+      for (int64_t g = g0; g < g1; ++g) {
+        const int64_t cm = g / gr.gpc;
+        for (int d = 0; d < dim; ++d) {
+          lo[d] = std::min(lo[d], ccm[cm * dim + d]);
+          hi[d] = std::max(hi[d], ccm[cm * dim + d]);
+        }
+      }
       for (int d = 0; d < dim; ++d) {
-        lo[d] = std::min(lo[d], ccm[cm * dim + d]);
-        hi[d] = std::max(hi[d], ccm[cm * dim + d]);
+        box[b * 2 * dim + d] = lo[d];
+        box[b * 2 * dim + dim + d] = hi[d];
       }
     }
-    for (int d = 0; d < dim; ++d) {
-      box[b * 2 * dim + d] = lo[d];
-      box[b * 2 * dim + dim + d] = hi[d];
-    }
-  }
+    return box;
+  };
+  const std::vector<double> box = boxes(p.grp_big), box_s = boxes(p.grp_small);
   // row (bin) order of the cells (target-major pass)
   p.cg = make_grid(clo, chi, dim, n_cells, 8.0);
   const int64_t ncb = (int64_t)p.cg.n[0] * p.cg.n[1] * p.cg.n[2];
@@ -697,6 +722,7 @@ extern "C" int sdot_set_source(sdot_ctx *c, int dim, int64_t n_cells, int nq, co
   HIP_TRY(upload(dv.qx2_m, qx2));
   HIP_TRY(upload(dv.cc_m, ccm));
   HIP_TRY(upload(dv.blkbox, box));
+  HIP_TRY(upload(dv.blkbox_s, box_s));
   HIP_TRY(upload(dv.cc_r, ccr));
   HIP_TRY(upload(dv.cstart, cstart));
   HIP_TRY(upload(dv.cell_r2m, r2m));
@@ -1236,12 +1262,12 @@ int sdot::ctx_source_view(sdot_ctx *c, SourceView &v) {
   v.device = c->device;
   v.dim = p.dim;
   v.nq = p.nq;
-  v.G = p.G;
-  v.gpc = p.gpc;
+  v.G = p.grp_big.G;
+  v.gpc = p.grp_big.gpc;
   v.n_cells = p.n_cells;
   v.nQ = p.nQ;
-  v.n_groups = p.n_groups;
-  v.n_blocks = p.n_blocks;
+  v.n_groups = p.grp_big.n_groups;
+  v.n_blocks = p.grp_big.n_blocks;
   v.cg = p.cg;
   for (int k = 0; k < 3; ++k) {
     v.origin[k] = p.origin[k];

@@ -48,6 +48,23 @@ __device__ __forceinline__ double expo(double d) {
   }
 }
 
+// v_max_f64 / v_min_f64 as one instruction: fmax() on a loop-carried value
+// also canonicalizes that operand (a second v_max_f64) to quiet signaling
+// NaNs, which the exponents and maxima here never are.
+__device__ __forceinline__ double vmax64(double a, double b) {
+  double r;
+  asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ double vmin64(double a, double b) {
+  double r;
+  asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
+// Records per chunk of the exponent-change extrema (seed bound of the FP32 sweep).
+constexpr int kChunk = 64;
+
 using Tested = std::true_type;    // run whose members need the candidate test
 using Inside = std::false_type;   // run of candidates of every query point
 
@@ -226,6 +243,24 @@ __device__ __forceinline__ void prep_body(const PrepArgs &a, int64_t blk, double
   if (threadIdx.x == 0) a.PNpart[blk] = s;
 }
 
+// Extrema of the exponent change dpsi_r over each chunk of kChunk row-order
+// records (one wave per chunk).
+__device__ __forceinline__ void chunkext_body(const double *dpsi_r, int64_t N, double *dcx,
+                                              double *dcn, int64_t chunk) {
+  const int64_t j = chunk * kChunk + (threadIdx.x & (kWave - 1));
+  double mx = -INFINITY, mn = INFINITY;
+  if (j < N) mx = mn = dpsi_r[j];
+#pragma unroll
+  for (int m = kWave / 2; m >= 1; m >>= 1) {
+    mx = fmax(mx, __shfl_xor(mx, m));
+    mn = fmin(mn, __shfl_xor(mn, m));
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0 && chunk * kChunk < N) {
+    dcx[chunk] = mx;
+    dcn[chunk] = mn;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // k_pass1: q-major sweeps
 // ---------------------------------------------------------------------------
@@ -274,7 +309,7 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
         double t = r.a;
 #pragma unroll
         for (int d = 0; d < DIM; ++d) t = fma(x[k][d], r.yl[d], t);
-        M[k] = fmax(M[k], t);
+        M[k] = vmax64(M[k], t);
       }
     } else if (PASS == 1) {
 #pragma unroll
@@ -285,13 +320,14 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
         S[k] = fma(r.nu, expo<MODE>(t - M[k]), S[k]);
       }
     } else {
-      // exponent relative to the bound M (<= 0 up to rounding), running maximum
+      // exponent relative to the bound M (<= 0 up to rounding), running
+      // maximum; the terms are accumulated in FP64
 #pragma unroll
       for (int k = 0; k < G; ++k) {
         double t = r.a;
 #pragma unroll
         for (int d = 0; d < DIM; ++d) t = fma(x[k][d], r.yl[d], t);
-        T[k] = fmax(T[k], t);
+        T[k] = vmax64(T[k], t);
         S[k] = fma(r.nu, (double)__builtin_amdgcn_exp2f((float)(t - M[k])), S[k]);
       }
     }
@@ -322,14 +358,28 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
     bool seed = false;
     double dmx = -INFINITY;
     if (a.hist) {
-      cdouble *dp = cptr(a.dpsi);
       double dmn = INFINITY;
-      sweep([&](int j0, int j1, auto) {
-        for (int j = j0; j < j1; ++j) {
-          dmx = fmax(dmx, dp[j]);
-          dmn = fmin(dmn, dp[j]);
-        }
-      });
+      if (a.dcx) {
+        // extrema per chunk of kChunk records: a bound over a superset of
+        // the item's candidates
+        cdouble *cx = cptr(a.dcx);
+        cdouble *cn = cptr(a.dcn);
+        sweep([&](int j0, int j1, auto) {
+          if (j1 <= j0) return;
+          for (int ch = j0 / kChunk; ch <= (j1 - 1) / kChunk; ++ch) {
+            dmx = vmax64(dmx, cx[ch]);
+            dmn = vmin64(dmn, cn[ch]);
+          }
+        });
+      } else {
+        cdouble *dp = cptr(a.dpsi);
+        sweep([&](int j0, int j1, auto) {
+          for (int j = j0; j < j1; ++j) {
+            dmx = vmax64(dmx, dp[j]);
+            dmn = vmin64(dmn, dp[j]);
+          }
+        });
+      }
       seed = dmx - dmn < kSeedSpread;
     }
     if (seed) {
@@ -349,7 +399,7 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
             double t = r.a;
 #pragma unroll
             for (int d = 0; d < DIM; ++d) t = fma(x[k][d], r.yl[d], t);
-            T[k] = fmax(T[k], t);
+            T[k] = vmax64(T[k], t);
           }
         }
       });

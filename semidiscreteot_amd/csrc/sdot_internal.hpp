@@ -53,6 +53,7 @@ struct Dev {
   double *qx2_m = nullptr;     // [nQ] |x'|^2 (plain-mode shift)
   double *cc_m = nullptr;      // [n_cells*dim] raw centers, Morton order
   double *blkbox = nullptr;    // [n_blocks*2*dim] AABB of the block's cell centers
+  double *blkbox_s = nullptr;  // the same for the small grouping
   double *cc_r = nullptr;      // [n_cells*4] raw centers, row (bin) order
   int32_t *cstart = nullptr;   // [n_cbins+1] row-order cell offsets per bin
   int64_t *cell_r2m = nullptr; // [n_cells] row-order cell -> Morton cell
@@ -84,6 +85,8 @@ struct Dev {
   double *Upart = nullptr;     // [nFpart]
   double *psi_prev = nullptr;  // [N] psi of the previous FP32 evaluation
   double *dpsi_r = nullptr;    // [N] exponent change since then, row order
+  double *dpsi_cx = nullptr;   // [ceil(N/64)] its maximum per chunk of 64 records
+  double *dpsi_cn = nullptr;   // [ceil(N/64)] its minimum
   double *ucount = nullptr;    // [1] q-points whose FP32 sum lost its range
   double *scratch = nullptr;   // misc
   size_t scratch_bytes = 0;
@@ -91,10 +94,22 @@ struct Dev {
   size_t i64scratch_n = 0;
 };
 
+// Quadrature points per lane of the q-major passes (G), lanes per cell (gpc)
+// and the resulting 64-lane blocks.
+struct Grouping {
+  int G = 1, gpc = 1;
+  int64_t n_groups = 0, n_blocks = 0;
+};
+
 // Everything a kernel launch needs, by value.
 struct Plan {
   int dim = 0, nq = 0, G = 1, gpc = 1, mode = kF64Lse;
   int64_t n_cells = 0, nQ = 0, n_groups = 0, n_blocks = 0;
+  // big: most points per lane (dense sweeps: fewest record loads per pair);
+  // small: at most 4 per lane (truncated sweeps: half-size blocks of cells,
+  // fewer lanes idle at the edge of the candidate balls); `small` selects it
+  Grouping grp_big, grp_small;
+  int small = 0;
   int64_t N = 0, n_twaves = 0;
   int K1 = 1, K2 = 1;
   int64_t nFpart = 0, nPpart = 0;
@@ -135,6 +150,7 @@ struct P1Args {
   const int32_t *tstart;
   const double *Mpart;   // [K1*nQ]  maxima of PASS 0, read by PASS 1
   const double *dpsi;    // [N]      FP32 history: exponent change per target, row order
+  const double *dcx, *dcn;  // its maximum / minimum per chunk of 64 records (null: scan dpsi)
   double *Mout;          // [K1*nQ]  PASS 0: segment maxima; PASS 2: reference value used
   double *Sout;          // [K1*nQ]  PASS 1 / 2: segment sums
   double *Tout;          // [K1*nQ]  PASS 2: segment maxima of the exponent (read first:

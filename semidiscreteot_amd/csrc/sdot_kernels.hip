@@ -47,6 +47,11 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
   prep_body<DIM>(a, blockIdx.x, sh);
 }
 
+__global__ __launch_bounds__(256) void k_chunkext(const double *dpsi_r, int64_t N, double *dcx,
+                                                  double *dcn) {
+  chunkext_body(dpsi_r, N, dcx, dcn, (int64_t)blockIdx.x * (256 / kWave) + threadIdx.x / kWave);
+}
+
 template <int DIM, int G, int MODE, int PASS>
 __global__ __launch_bounds__(64) void k_pass1(P1Args a) {
   pass1_body<DIM, G, MODE, PASS>(a, blockIdx.x, blockIdx.y);
@@ -353,16 +358,24 @@ static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_g
   for (int k = 0; k < 3; ++k) pa.o[k] = p.origin[k];
   pa.coef = coef;
   hipLaunchKernelGGL((k_prep<DIM>), dim3((unsigned)p.nPpart), dim3(256), 0, s, pa);
+  const bool chunks = MODE == kF32Lse && p.hist;
+  if (chunks) {
+    const int64_t nch = (p.N + kChunk - 1) / kChunk;
+    hipLaunchKernelGGL(k_chunkext, dim3((unsigned)((nch + 3) / 4)), dim3(256), 0, s, d.dpsi_r, p.N,
+                       d.dpsi_cx, d.dpsi_cn);
+  }
 
   P1Args a{};
   a.qx = d.qx_m;
   a.qx2 = d.qx2_m;
   a.cc = d.cc_m;
-  a.blkbox = d.blkbox;
+  a.blkbox = p.small ? d.blkbox_s : d.blkbox;
   a.trec = d.trec_r;
   a.tstart = d.tstart;
   a.Mpart = d.Mpart;
   a.dpsi = d.dpsi_r;
+  a.dcx = chunks ? d.dpsi_cx : nullptr;
+  a.dcn = chunks ? d.dpsi_cn : nullptr;
   a.tg = p.tg;
   a.n_groups = p.n_groups;
   a.nQ = p.nQ;
@@ -463,7 +476,7 @@ hipError_t launch_candidate_stats(const Plan &p, Dev &d, int64_t *d_count, uint6
                                   hipStream_t s) {
   CSArgs a{};
   a.cc = d.cc_m;
-  a.blkbox = d.blkbox;
+  a.blkbox = p.small ? d.blkbox_s : d.blkbox;
   a.trec = d.trec_r;
   a.tstart = d.tstart;
   a.tr2o = d.tr2o;
