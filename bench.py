@@ -75,8 +75,9 @@ def cpu_baseline(wl, psi, R, target_seconds=15.0):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_eval_gmm3d_32cube_N50000.md")
-KERNEL_KEYS = {"lse_max": "k_pass1<3, 8, 2, 0>", "lse_sum": "k_pass1<3, 8, 2, 2>",
-               "grad": "k_grad<3, 2, 8>"}
+# kernel names of the FP32 evaluation's sweeps (any number of points per lane)
+KERNEL_KEYS = {"lse_max": r"k_pass1<3, \d, 2, 0>", "lse_sum": r"k_pass1<3, \d, 2, 2>",
+               "grad": r"k_grad<3, 2, \d>"}
 
 
 def pmc_summary(fetch_csv, write_csv, copy_bytes):
@@ -127,9 +128,11 @@ def measured_traffic(kernel_key):
         text = open(TRAFFIC_FILE).read()
     except OSError:
         return None
+    import re
     want = KERNEL_KEYS.get(kernel_key)
     for line in text.splitlines():
-        if line.startswith("traffic-bytes ") and want and line.split("=")[0].strip().endswith(want):
+        if line.startswith("traffic-bytes ") and want and \
+                re.search(want + "$", line.split("=")[0].strip()):
             return float(line.split("=")[1])
     return None
 

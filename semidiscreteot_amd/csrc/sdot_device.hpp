@@ -69,8 +69,12 @@ using Tested = std::true_type;    // run whose members need the candidate test
 using Inside = std::false_type;   // run of candidates of every query point
 
 // Visit the bin rows (x-runs of bins, contiguous in row-major bin order) whose
-// bins can hold a point within Rq of the box [lo,hi]; every nseg-th visited row
-// (phase seg) is handed to body(first_item, end_item, tag).  With s_in > 0 a row
+// bins can hold a point within Rq of the box [lo,hi]; the rows whose global
+// index (bz * n_y + by) is seg modulo nseg are handed to body(first_item,
+// end_item, tag).  The split depends on the row alone, not on the box, so the
+// segments a workgroup of one XCD runs (seg = xcd modulo 8) touch a fixed
+// eighth of the rows whatever wave they serve, and that eighth of the records
+// stays in the XCD's L2.  With s_in > 0 a row
 // is split into runs: bins every point of which lies within sqrt(s_in) of every
 // point of the box (tag Inside: candidates of every query point in the box,
 // s_in being the candidate bound with a relative margin far above the rounding
@@ -87,7 +91,6 @@ __device__ __forceinline__ void for_rows(const Grid &g, const int32_t *__restric
   }
   const double R2 = Rq * Rq;
   const double pad = 1e-9 * g.h;
-  int phase = 0;
   for (int bz = b0[2]; bz <= b1[2]; ++bz) {
     double dz = 0.0, dzf = 0.0;
     if (DIM == 3) {
@@ -95,14 +98,14 @@ __device__ __forceinline__ void for_rows(const Grid &g, const int32_t *__restric
       dz = fmax(0.0, fmax(zl - hi[2], lo[2] - zh));
       dzf = fmax(zh - lo[2], hi[2] - zl);
     }
-    for (int by = b0[1]; by <= b1[1]; ++by) {
+    // this segment's rows of the slab: by = seg - bz * n_y (mod nseg)
+    const int want = ((seg - (bz * g.n[1]) % nseg) % nseg + nseg) % nseg;
+    const int by0 = b0[1] + ((want - b0[1] % nseg) % nseg + nseg) % nseg;
+    for (int by = by0; by <= b1[1]; by += nseg) {
       const double yl = g.o[1] + by * g.h, yh = yl + g.h;
       const double dy = fmax(0.0, fmax(yl - hi[1], lo[1] - yh));
       const double r2 = R2 - dy * dy - dz * dz;
       if (r2 < 0.0) continue;
-      const int mine = (phase == seg);
-      phase = (phase + 1 == nseg) ? 0 : phase + 1;
-      if (!mine) continue;
       const double wx = sqrt(r2);
       const int x0 = bin_of(lo[0] - wx, g.o[0], g.inv_h, g.n[0]);
       const int x1 = bin_of(hi[0] + wx, g.o[0], g.inv_h, g.n[0]);
