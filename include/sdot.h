@@ -149,6 +149,14 @@ int sdot_eval_R(sdot_ctx *ctx, double R, const double *psi, double *grad, double
 int sdot_eval_device(sdot_ctx *ctx, double R, const double *d_psi, double *d_grad, double *d_FC);
 int sdot_synchronize(sdot_ctx *ctx);
 
+/* SotSolver::evaluate_weighted_barycenters / compute_weighted_barycenters_euclidean
+ * (SotSolver.cc:585-610, 871-971): per target j, sum over q of (w_q/S_q) nu_j e_qj x_q
+ * divided by nu_j (the first moment of the transport plan's column j over nu_j),
+ * with the threshold from the stored state (the functional state is not changed).
+ * bary [N*dim] out, caller order.  The _R form takes the threshold explicitly. */
+int sdot_barycenters(sdot_ctx *ctx, const double *psi, double *bary);
+int sdot_barycenters_R(sdot_ctx *ctx, double R, const double *psi, double *bary);
+
 /* SotSolver::compute_hessian (declared at SotSolver.h:338-344, no reference
  * definition): dense H (row-major, leading dimension ld >= N) of the
  * functional at psi with the current threshold rule. */

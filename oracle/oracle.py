@@ -93,6 +93,13 @@ def lib():
                                 C.POINTER(_State), _dp, C.POINTER(_Info)]
         L.orc_bfgs_generic.argtypes = [_FUN, C.c_void_p, C.c_int64, _dp, C.c_int, C.c_double,
                                        C.POINTER(_Info)]
+        L.orc_barycenters_R.argtypes = [C.POINTER(_Source), C.POINTER(_Target), C.POINTER(_Params),
+                                        C.c_double, _dp, _dp]
+        L.orc_conditional_density.argtypes = [C.POINTER(_Target), C.c_int, _dp, C.c_double,
+                                              C.c_double, C.c_int64, _dp, _dp, C.c_int64, _i64p,
+                                              _dp]
+        L.orc_transport_map.argtypes = [C.c_int, C.POINTER(_Target), C.c_int, _dp, C.c_double,
+                                        C.c_double, C.c_int64, _dp, _dp]
         L.orc_softmax_refinement.argtypes = [C.POINTER(_Source), C.c_int64, _dp,
                                              C.POINTER(_Target), _dp, C.c_int64, _i64p, _i64p,
                                              C.c_double, C.c_double, C.c_int, C.c_int, _dp]
@@ -315,3 +322,34 @@ def softmax_refinement(src, y_fine, coarse, psi_coarse, children, lam, threshold
                                       int(n_threads), _p(out))
     assert rc == 0
     return out
+
+
+def barycenters_R(src, tgt, prm, R, psi):
+    """compute_weighted_barycenters_euclidean (SotSolver.cc:871-971) at threshold R."""
+    out = np.zeros(tgt.n * src.dim)
+    rc = lib().orc_barycenters_R(C.byref(src._s), C.byref(tgt._t), C.byref(prm), float(R),
+                                 _p(_f64(psi)), _p(out))
+    assert rc == 0
+    return out.reshape(tgt.n, src.dim)
+
+
+def conditional_density(tgt, dim, psi, eps, R, points, density, indices):
+    """get_potential_conditioned_density (SotSolver.cc:973-1046); [n_idx, n_points]."""
+    pts = _f64(points).ravel()
+    n = pts.size // dim
+    idx = np.ascontiguousarray(indices, dtype=np.int64)
+    out = np.zeros(idx.size * n)
+    lib().orc_conditional_density(C.byref(tgt._t), dim, _p(_f64(psi)), float(eps), float(R), n,
+                                  _p(pts), _p(_f64(density)), idx.size, idx.ctypes.data_as(_i64p),
+                                  _p(out))
+    return out.reshape(idx.size, n)
+
+
+def transport_map(strategy, tgt, dim, psi, eps, radius, points):
+    """Modal (0) / barycentric (1) compute_map (OptimalTransportPlan.cc:99-310)."""
+    pts = _f64(points).ravel()
+    n = pts.size // dim
+    out = np.zeros(n * dim)
+    lib().orc_transport_map(int(strategy), C.byref(tgt._t), dim, _p(_f64(psi)), float(eps),
+                            float(radius), n, _p(pts), _p(out))
+    return out.reshape(n, dim)

@@ -22,6 +22,17 @@
  *   orc_hessian               FD of the gradient, H*1 = 0, symmetry, PSD.
  *   orc_bfgs_generic/solve    quadratic (-> A^-1 b) and Rosenbrock minima,
  *                             converged SOT gradients below tolerance.
+ *   orc_barycenters_R         one target: the mean of the source (sum w x);
+ *                             a shift of every point shifts the barycenters;
+ *                             sum_j nu_j b_j = sum_q w_q x_q when the plan's
+ *                             marginal is exact; first moments vs gradient.
+ *   orc_conditional_density   partition of unity: sum_i nu_i c_i(x) = rho(x)
+ *                             over all targets; one target: rho; 50-digit
+ *                             values (tests/golden).
+ *   orc_transport_map         modal = nearest target when psi = eps log nu
+ *                             terms cancel (Voronoi); barycentric of one
+ *                             target = that target; eps -> 0 barycentric =
+ *                             modal; 50-digit values (tests/golden).
  *   orc_softmax_refinement    identity hierarchy (fine = coarse, one child
  *                             each): psi_f = psi_c - lambda log(1 + g_c/nu_c)
  *                             with g from orc_evaluate_R at the same psi;
@@ -615,6 +626,220 @@ int orc_hessian(const orc_source_t *s, const orc_target_t *t,
     H[k] = v;
   }
   free(tH);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* compute_weighted_barycenters_euclidean (SotSolver.cc:871-971): the          */
+/* local_assemble loop with the barycenter function_call (:912-934), copier    */
+/* sum, then division by nu_j (:957-963).                                     */
+/* ------------------------------------------------------------------------ */
+
+int orc_barycenters_R(const orc_source_t *s, const orc_target_t *t,
+                      const orc_params_t *p, double R, const double *psi,
+                      double *bary) {
+  const int dim = s->dim;
+  const int64_t N = t->n;
+  const int nq = s->nq;
+  const double epsilon_inv = 1.0 / p->epsilon;
+  const int use_lse = p->use_log_sum_exp;
+  const int nthr = orc_threads(p);
+  double *tb = (double *)calloc((size_t)nthr * (size_t)N * dim, sizeof(double));
+  if (!tb) return -1;
+#pragma omp parallel num_threads(nthr)
+  {
+#ifdef _OPENMP
+    const int tid = omp_get_thread_num();
+#else
+    const int tid = 0;
+#endif
+    double *acc = tb + (size_t)tid * N * dim;
+    int64_t *idx = (int64_t *)malloc(sizeof(int64_t) * (N > 0 ? N : 1));
+    double *exp_terms = (double *)malloc(sizeof(double) * (N > 0 ? N : 1));
+#pragma omp for schedule(static)
+    for (int64_t c = 0; c < s->n_cells; ++c) {
+      const int64_t nt = orc_candidates(t, s->center + c * dim, dim, R, idx);
+      if (nt == 0) continue;
+      for (int q = 0; q < nq; ++q) {
+        const int64_t g = c * nq + q;
+        const double *xq = s->x + g * dim;
+        double total_sum_exp = 0.0, max_exponent = -DBL_MAX;
+        if (use_lse)
+          for (int64_t i = 0; i < nt; ++i) {
+            const int64_t j = idx[i];
+            const double d2 = pow(orc_dist(xq, t->y + j * dim, dim), 2);
+            const double e = (psi[j] - 0.5 * d2) * epsilon_inv;
+            max_exponent = e > max_exponent ? e : max_exponent;
+          }
+        for (int64_t i = 0; i < nt; ++i) {
+          const int64_t j = idx[i];
+          const double d2 = pow(orc_dist(xq, t->y + j * dim, dim), 2);
+          const double e = (psi[j] - 0.5 * d2) * epsilon_inv;
+          exp_terms[i] = t->nu[j] * (use_lse ? exp(e - max_exponent) : exp(e));
+          total_sum_exp += exp_terms[i];
+        }
+        if (total_sum_exp <= 0.0) continue;
+        const double scale = s->rho[g] * s->jxw[g] / total_sum_exp;
+        for (int64_t i = 0; i < nt; ++i)
+          if (exp_terms[i] > 0.0)
+            for (int d = 0; d < dim; ++d) acc[dim * idx[i] + d] += scale * exp_terms[i] * xq[d];
+      }
+    }
+    free(idx);
+    free(exp_terms);
+  }
+  for (int64_t j = 0; j < N; ++j)
+    for (int d = 0; d < dim; ++d) {
+      double v = 0.0;
+      for (int k = 0; k < nthr; ++k) v += tb[(size_t)k * N * dim + dim * j + d];
+      bary[j * dim + d] = v / t->nu[j];
+    }
+  free(tb);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* get_potential_conditioned_density (SotSolver.cc:973-1046): at every point  */
+/* (a DoF support point), the LSE over the targets within R of it; for each   */
+/* requested target i among them, rho_p exp(e_pi - M_p) / sum_j nu_j           */
+/* exp(e_pj - M_p).  out [n_idx * n_points], 0 where i is not a candidate.     */
+/* ------------------------------------------------------------------------ */
+
+int orc_conditional_density(const orc_target_t *t, int dim, const double *psi,
+                            double epsilon, double R, int64_t n_points,
+                            const double *points, const double *density,
+                            int64_t n_idx, const int64_t *indices, double *out) {
+  const int64_t N = t->n;
+  const double epsilon_inv = 1.0 / epsilon;
+  memset(out, 0, sizeof(double) * (size_t)(n_idx * n_points));
+#pragma omp parallel
+  {
+    int64_t *cand = (int64_t *)malloc(sizeof(int64_t) * (N > 0 ? N : 1));
+    char *is_cand = (char *)calloc((size_t)(N > 0 ? N : 1), 1);
+    double *ex = (double *)calloc((size_t)(N > 0 ? N : 1), sizeof(double));
+#pragma omp for schedule(static)
+    for (int64_t p = 0; p < n_points; ++p) {
+      const double *sp = points + p * dim;
+      const int64_t nt = orc_candidates(t, sp, dim, R, cand);
+      double total_sum_exp = 0.0, max_exponent = -DBL_MAX;
+      for (int64_t i = 0; i < nt; ++i) {
+        const int64_t j = cand[i];
+        const double e = (psi[j] - 0.5 * pow(orc_dist(sp, t->y + j * dim, dim), 2)) * epsilon_inv;
+        max_exponent = e > max_exponent ? e : max_exponent;
+      }
+      for (int64_t i = 0; i < nt; ++i) {
+        const int64_t j = cand[i];
+        ex[j] = exp((psi[j] - 0.5 * pow(orc_dist(sp, t->y + j * dim, dim), 2)) * epsilon_inv -
+                    max_exponent);
+        total_sum_exp += t->nu[j] * ex[j];
+        is_cand[j] = 1;
+      }
+      if (total_sum_exp > 0.0)
+        for (int64_t k = 0; k < n_idx; ++k) {
+          const int64_t i = indices[k];
+          if (i >= 0 && i < N && is_cand[i]) out[k * n_points + p] = density[p] * (ex[i] / total_sum_exp);
+        }
+      for (int64_t i = 0; i < nt; ++i) {
+        is_cand[cand[i]] = 0;
+        ex[cand[i]] = 0.0;
+      }
+    }
+    free(cand);
+    free(is_cand);
+    free(ex);
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* ModalStrategy / BarycentricStrategy::compute_map (OptimalTransportPlan.cc: */
+/* 99-189, 218-310).  Candidates of x: the targets with dist(x, y) < radius   */
+/* when radius > 0 (all otherwise), in index order; none: the nearest target  */
+/* (lowest index among equals).                                               */
+/*   modal:       the target maximising psi_j - |x-y_j|^2/2 + eps log nu_j    */
+/*                (first maximum in candidate order);                         */
+/*   barycentric: sum_j nu_j e^(e_j - M) y_j / sum_j nu_j e^(e_j - M),         */
+/*                e_j = (psi_j - |x-y_j|^2/2) / eps; the nearest target when  */
+/*                the sum is not positive.                                    */
+/* ------------------------------------------------------------------------ */
+
+static int64_t orc_nearest(const orc_target_t *t, const double *x, int dim) {
+  int64_t best = 0;
+  double bd = INFINITY;
+  for (int64_t j = 0; j < t->n; ++j) {
+    const double d = orc_dist(x, t->y + j * dim, dim);
+    if (d < bd) {
+      bd = d;
+      best = j;
+    }
+  }
+  return best;
+}
+
+int orc_transport_map(int strategy, const orc_target_t *t, int dim,
+                      const double *psi, double epsilon, double radius,
+                      int64_t n_points, const double *points, double *mapped) {
+  const int64_t N = t->n;
+  const int use_truncation = radius > 0.0;
+#pragma omp parallel
+  {
+    int64_t *cand = (int64_t *)malloc(sizeof(int64_t) * (N > 0 ? N : 1));
+    double *et = (double *)malloc(sizeof(double) * (N > 0 ? N : 1));
+#pragma omp for schedule(static)
+    for (int64_t i = 0; i < n_points; ++i) {
+      const double *x = points + i * dim;
+      int64_t nc = 0;
+      if (use_truncation) {
+        for (int64_t j = 0; j < N; ++j)
+          if (orc_dist(x, t->y + j * dim, dim) < radius) cand[nc++] = j;
+        if (nc == 0) {
+          const int64_t j = orc_nearest(t, x, dim);
+          for (int d = 0; d < dim; ++d) mapped[i * dim + d] = t->y[j * dim + d];
+          continue;
+        }
+      } else {
+        for (int64_t j = 0; j < N; ++j) cand[nc++] = j;
+      }
+      if (strategy == 0) {
+        double max_score = -INFINITY;
+        int64_t best = 0;
+        for (int64_t k = 0; k < nc; ++k) {
+          const int64_t j = cand[k];
+          const double sq = pow(orc_dist(x, t->y + j * dim, dim), 2);
+          const double log_term = t->nu[j] > 0 ? epsilon * log(t->nu[j]) : -INFINITY;
+          const double score = psi[j] - 0.5 * sq + log_term;
+          if (score > max_score) {
+            max_score = score;
+            best = j;
+          }
+        }
+        for (int d = 0; d < dim; ++d) mapped[i * dim + d] = t->y[best * dim + d];
+      } else {
+        double max_exponent = -INFINITY;
+        for (int64_t k = 0; k < nc; ++k) {
+          const int64_t j = cand[k];
+          const double sq = pow(orc_dist(x, t->y + j * dim, dim), 2);
+          et[k] = (psi[j] - 0.5 * sq) / epsilon;
+          max_exponent = et[k] > max_exponent ? et[k] : max_exponent;
+        }
+        double ws[3] = {0.0, 0.0, 0.0}, sum_exp = 0.0;
+        for (int64_t k = 0; k < nc; ++k) {
+          const int64_t j = cand[k];
+          const double weight = t->nu[j] * exp(et[k] - max_exponent);
+          for (int d = 0; d < dim; ++d) ws[d] += weight * t->y[j * dim + d];
+          sum_exp += weight;
+        }
+        if (sum_exp > 0) {
+          for (int d = 0; d < dim; ++d) mapped[i * dim + d] = ws[d] / sum_exp;
+        } else {
+          const int64_t j = orc_nearest(t, x, dim);
+          for (int d = 0; d < dim; ++d) mapped[i * dim + d] = t->y[j * dim + d];
+        }
+      }
+    }
+    free(cand);
+    free(et);
+  }
   return 0;
 }
 

@@ -109,6 +109,23 @@ int orc_discarded_fraction(const orc_source_t *s, const orc_target_t *t,
 int orc_hessian(const orc_source_t *s, const orc_target_t *t,
                 const orc_params_t *p, double R, const double *psi, double *H);
 
+/* ---- barycenters (Lloyd step) ---- */
+/* compute_weighted_barycenters_euclidean (SotSolver.cc:871-971) at threshold R:
+ * bary [N*dim]. */
+int orc_barycenters_R(const orc_source_t *s, const orc_target_t *t,
+                      const orc_params_t *p, double R, const double *psi,
+                      double *bary);
+
+/* ---- conditional densities and transport maps ---- */
+int orc_conditional_density(const orc_target_t *t, int dim, const double *psi,
+                            double epsilon, double R, int64_t n_points,
+                            const double *points, const double *density,
+                            int64_t n_idx, const int64_t *indices, double *out);
+/* strategy 0 = modal, 1 = barycentric */
+int orc_transport_map(int strategy, const orc_target_t *t, int dim,
+                      const double *psi, double epsilon, double radius,
+                      int64_t n_points, const double *points, double *mapped);
+
 /* ---- multilevel potential transfer ---- */
 /* SoftmaxRefinement::compute_refinement (SoftmaxRefinement.cc:200-280) with
  * its local_assemble (:61-198).  The coarse-to-fine child lists of the level

@@ -10,6 +10,7 @@ from .mesh import SourceArrays, simplex_source, structured_source  # noqa: F401
 from .solver import SolverParameters, SotSolver  # noqa: F401
 from .server import ServeResult, SotServer  # noqa: F401
 from .epsilon_scaling import EpsilonScalingHandler  # noqa: F401
+from .lloyd import run_lloyd  # noqa: F401
 from .multilevel import (MultilevelParameters, SoftmaxRefinement,  # noqa: F401
                          generate_target_hierarchy, run_combined_multilevel,
                          run_target_multilevel)
@@ -17,4 +18,4 @@ from .multilevel import (MultilevelParameters, SoftmaxRefinement,  # noqa: F401
 __all__ = ["SotSolver", "SolverParameters", "SotServer", "ServeResult", "SourceArrays", "structured_source",
            "simplex_source", "SdotError", "load", "LIB_PATH", "EpsilonScalingHandler",
            "SoftmaxRefinement", "MultilevelParameters", "generate_target_hierarchy",
-           "run_target_multilevel", "run_combined_multilevel"]
+           "run_target_multilevel", "run_combined_multilevel", "run_lloyd"]

@@ -96,6 +96,8 @@ SIGNATURES = {
     "sdot_eval_device": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]),
     "sdot_synchronize": (C.c_int, [C.c_void_p]),
     "sdot_hessian": (C.c_int, [C.c_void_p, _dp, _dp, C.c_int64]),
+    "sdot_barycenters": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "sdot_barycenters_R": (C.c_int, [C.c_void_p, C.c_double, _dp, _dp]),
     "sdot_softmax_refinement": (C.c_int, [C.c_void_p, C.c_int64, _dp, C.c_int64, _dp, _dp, _dp,
                                           C.c_int64, _i64p, _i64p, C.c_double, C.c_double,
                                           C.c_int, _dp]),

@@ -445,6 +445,55 @@ int host_eval(sdot_ctx *c, double R, const double *psi, double *grad, double *F,
   return SDOT_OK;
 }
 
+// The evaluation at R with the plan's first moments per target (barycenters,
+// caller order, host memory).
+int host_barycenters(sdot_ctx *c, double R, const double *psi, double *bary) {
+  if (!psi || !bary) return fail(SDOT_ERR_INVALID, "null argument");
+  if (!c->have_source || !c->have_target)
+    return fail(SDOT_ERR_STATE, "Invalid measures configuration: source and target must be set");
+  const int64_t N = c->plan.N;
+  const int dim = c->plan.dim;
+  int rc = prepare(c, R);
+  if (rc) return rc;
+  double *d_bary = nullptr, *d_B = nullptr;
+  auto cleanup = [&]() {
+    if (d_bary) (void)hipFree(d_bary);
+    if (d_B) (void)hipFree(d_B);
+  };
+  hipError_t e = hipMalloc((void **)&d_bary, sizeof(double) * N * dim);
+  if (e == hipSuccess) e = hipMalloc((void **)&d_B, sizeof(double) * (size_t)c->plan.K2 * dim * N);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(c->dev.psi, psi, sizeof(double) * N, hipMemcpyHostToDevice, c->stream);
+  for (int attempt = 0; e == hipSuccess && attempt < 2; ++attempt) {
+    const bool seeded = c->plan.hist != 0;
+    e = launch_eval_bary(c->plan, c->dev, c->dev.psi, c->dev.grad, c->dev.out, d_bary, d_B,
+                         c->stream);
+    if (e != hipSuccess) break;
+    if (c->plan.mode == kF32Lse) {
+      c->hist_valid = true;
+      c->hist_R = R;
+    }
+    if (!seeded) break;
+    // a seed far above the new maxima would have flushed the FP32 terms: redo unseeded
+    double lost = 0.0;
+    e = hipMemcpyAsync(&lost, c->dev.ucount, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess || !(lost > 0.0)) break;
+    c->hist_valid = false;
+    rc = prepare(c, R);
+    if (rc) {
+      cleanup();
+      return rc;
+    }
+  }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(bary, d_bary, sizeof(double) * N * dim, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  cleanup();
+  if (e != hipSuccess) return fail(SDOT_ERR_HIP, std::string("barycenters: ") + hipGetErrorString(e));
+  return SDOT_OK;
+}
+
 int raw_covering_radius(sdot_ctx *c, double *r0) {
   const Plan &p = c->plan;
   double *d_min = nullptr;
@@ -789,6 +838,27 @@ extern "C" int sdot_eval(sdot_ctx *c, const double *psi, double *grad, double *F
   if (!psi) return fail(SDOT_ERR_INVALID, "psi is null");
   HIP_TRY(hipSetDevice(c->device));
   return stateful_eval(c, psi, grad, F, C);
+}
+
+extern "C" int sdot_barycenters_R(sdot_ctx *c, double R, const double *psi, double *bary) {
+  if (ctx_check(c)) return SDOT_ERR_INVALID;
+  HIP_TRY(hipSetDevice(c->device));
+  return host_barycenters(c, R, psi, bary);
+}
+
+extern "C" int sdot_barycenters(sdot_ctx *c, const double *psi, double *bary) {
+  if (ctx_check(c)) return SDOT_ERR_INVALID;
+  if (!c->have_source || !c->have_target)
+    return fail(SDOT_ERR_STATE, "Invalid measures configuration");
+  if (!psi) return fail(SDOT_ERR_INVALID, "psi is null");
+  HIP_TRY(hipSetDevice(c->device));
+  // compute_weighted_barycenters_euclidean (SotSolver.cc:871-890): the threshold
+  // from the stored state; the functional state is left as it is
+  double R;
+  int rc = threshold(c, psi, &R);
+  if (rc) return rc;
+  c->current_threshold = R;
+  return host_barycenters(c, R, psi, bary);
 }
 
 extern "C" int sdot_eval_device(sdot_ctx *c, double R, const double *d_psi, double *d_grad,

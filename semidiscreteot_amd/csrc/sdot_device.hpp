@@ -508,7 +508,7 @@ __device__ __forceinline__ void qfin_body(const QFArgs &a, int64_t blk, double *
 // k_grad: target-major gradient partials
 // ---------------------------------------------------------------------------
 
-template <int DIM, int MODE, int G>
+template <int DIM, int MODE, int G, bool BARY = false>
 __device__ __forceinline__ void grad_body(const GArgs &a, int64_t blk, int seg) {
   const int64_t j = blk * kWave + threadIdx.x;
   const bool active = j < a.N;
@@ -521,6 +521,9 @@ __device__ __forceinline__ void grad_body(const GArgs &a, int64_t blk, int seg) 
     hi[d] = a.twbox[blk * 2 * DIM + DIM + d];
   }
   double acc = 0.0;
+  double accx[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) accx[d] = 0.0;
   const int nq = a.nq;
   cdouble *qrec = cptr(a.qrec_r);
   cdouble *ccr = cptr(a.cc_r);
@@ -539,7 +542,15 @@ __device__ __forceinline__ void grad_body(const GArgs &a, int64_t blk, int seg) 
 #pragma unroll
       for (int d = 0; d < DIM; ++d) t = fma(qx[k][d], me.yl[d], t);
       const double e = expo<MODE>(t - qm[k]);
-      if (MODE == kF64Plain) {
+      if constexpr (BARY) {
+        // SotSolver.cc:927-933: (w/S) exp_term x, exp_terms > 0 only
+        if (MODE != kF64Plain || e > 0.0) {
+          const double wt = qs[k] * e;
+          acc += wt;
+#pragma unroll
+          for (int d = 0; d < DIM; ++d) accx[d] = fma(wt, qx[k][d], accx[d]);
+        }
+      } else if (MODE == kF64Plain) {
         // the reference adds only exp_terms > 0 (SotSolver.cc:292); in plain
         // mode w/S_q can overflow to inf when S_q is subnormal
         if (e > 0.0) acc = fma(qs[k], e, acc);
@@ -567,7 +578,13 @@ __device__ __forceinline__ void grad_body(const GArgs &a, int64_t blk, int seg) 
   } else {
     for_rows<DIM>(a.cg, a.cstart, lo, hi, a.Rq, a.s_in, seg, a.K2, body);
   }
-  if (active) a.Gpart[(int64_t)seg * a.N + j] = acc;
+  if (active) {
+    a.Gpart[(int64_t)seg * a.N + j] = acc;
+    if constexpr (BARY) {
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) a.Bpart[((int64_t)seg * DIM + d) * a.N + j] = accx[d];
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -582,6 +599,27 @@ __device__ __forceinline__ void gfin_body(const GFArgs &a, int64_t blk) {
   for (int k = 0; k < a.K2; ++k) s += a.Gpart[(int64_t)k * a.N + j];
   const double nu = a.trec_m[j * kRec + 1 + 2 * DIM];
   a.grad[a.tm2o[j]] = nu * s - nu;
+}
+
+// Barycenters (SotSolver.cc:957-963): sum_seg of the first moments plus the
+// shift origin times the plan's mass, scattered to caller order.
+template <int DIM>
+__device__ __forceinline__ void bfin_body(const double *Gpart, const double *Bpart,
+                                          const int32_t *tm2o, const double *o, int64_t N,
+                                          int K2, double *bary, int64_t blk) {
+  const int64_t j = blk * 256 + threadIdx.x;
+  if (j >= N) return;
+  double g = 0.0, b[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) b[d] = 0.0;
+  for (int k = 0; k < K2; ++k) {
+    g += Gpart[(int64_t)k * N + j];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) b[d] += Bpart[((int64_t)k * DIM + d) * N + j];
+  }
+  const int64_t jo = tm2o[j];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) bary[jo * DIM + d] = fma(o[d], g, b[d]);
 }
 
 // Fixed-order totals: out[0] = sum F_q - sum psi.nu, out[1] = sum C_q,

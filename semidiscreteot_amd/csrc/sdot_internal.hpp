@@ -178,6 +178,7 @@ struct GArgs {
   const int32_t *cstart;
   const double *qrec_r;  // [nQ*kRec]
   double *Gpart;         // [K2*N]
+  double *Bpart;         // [K2*DIM*N] first moments sum_q (w/S) e x' (barycenters), or null
   Grid cg;
   int64_t N, n_cells;
   int nq, K2, dense;
@@ -282,6 +283,10 @@ hipError_t launch_refinement(int dim, const RefArgs &a, hipStream_t s);
 // Kernel launchers (sdot_kernels.hip).
 hipError_t launch_eval(const Plan &p, Dev &d, const double *d_psi, double *d_grad,
                        double *d_FC, hipStream_t s, Profile *prof);
+// The same evaluation with the first moments of the transport plan per target:
+// d_bary [N*dim] (caller order) = sum_q (w_q / S_q) e_qj x_q, d_Bpart scratch [K2*dim*N].
+hipError_t launch_eval_bary(const Plan &p, Dev &d, const double *d_psi, double *d_grad,
+                            double *d_FC, double *d_bary, double *d_Bpart, hipStream_t s);
 hipError_t launch_candidate_stats(const Plan &p, Dev &d, int64_t *d_count,
                                   uint64_t *d_hash, hipStream_t s);
 hipError_t launch_covering(const Plan &p, Dev &d, double *d_cellmin, hipStream_t s);

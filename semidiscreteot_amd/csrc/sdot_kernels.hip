@@ -73,6 +73,24 @@ __global__ __launch_bounds__(64) void k_grad(GArgs a) {
   grad_body<DIM, MODE, G>(a, blk, (int)(blockIdx.x - blk * a.K2));
 }
 
+// The gradient sweep with the first moments of the plan (barycenters).
+template <int DIM, int MODE, int G>
+__global__ __launch_bounds__(64) void k_grad_bary(GArgs a) {
+  const int64_t blk = blockIdx.x / a.K2;
+  grad_body<DIM, MODE, G, true>(a, blk, (int)(blockIdx.x - blk * a.K2));
+}
+
+struct Origin {
+  double v[3];
+};
+
+template <int DIM>
+__global__ __launch_bounds__(256) void k_bfin(const double *Gpart, const double *Bpart,
+                                              const int32_t *tm2o, Origin o, int64_t N, int K2,
+                                              double *bary) {
+  bfin_body<DIM>(Gpart, Bpart, tm2o, o.v, N, K2, bary, blockIdx.x);
+}
+
 template <int DIM>
 __global__ __launch_bounds__(256) void k_gfin(GFArgs a) {
   gfin_body<DIM>(a, blockIdx.x);
@@ -335,8 +353,25 @@ static void launch_grad_g(int G, const GArgs &g, int64_t ntw, int K2, hipStream_
 }
 
 template <int DIM, int MODE>
+static void launch_grad_bary_g(int G, const GArgs &g, int64_t ntw, int K2, hipStream_t s) {
+  const dim3 gg((unsigned)(ntw * K2));
+  switch (G) {
+    case 1: hipLaunchKernelGGL((k_grad_bary<DIM, MODE, 1>), gg, dim3(64), 0, s, g); break;
+    case 2: hipLaunchKernelGGL((k_grad_bary<DIM, MODE, 2>), gg, dim3(64), 0, s, g); break;
+    case 3: hipLaunchKernelGGL((k_grad_bary<DIM, MODE, 3>), gg, dim3(64), 0, s, g); break;
+    case 4: hipLaunchKernelGGL((k_grad_bary<DIM, MODE, 4>), gg, dim3(64), 0, s, g); break;
+    case 5: hipLaunchKernelGGL((k_grad_bary<DIM, MODE, 5>), gg, dim3(64), 0, s, g); break;
+    case 6: hipLaunchKernelGGL((k_grad_bary<DIM, MODE, 6>), gg, dim3(64), 0, s, g); break;
+    case 7: hipLaunchKernelGGL((k_grad_bary<DIM, MODE, 7>), gg, dim3(64), 0, s, g); break;
+    case 8: hipLaunchKernelGGL((k_grad_bary<DIM, MODE, 8>), gg, dim3(64), 0, s, g); break;
+    default: hipLaunchKernelGGL((k_grad_bary<DIM, MODE, 9>), gg, dim3(64), 0, s, g); break;
+  }
+}
+
+template <int DIM, int MODE>
 static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_grad,
-                         double *d_FC, hipStream_t s, Profile *prof) {
+                         double *d_FC, hipStream_t s, Profile *prof,
+                         double *d_bary = nullptr, double *d_Bpart = nullptr) {
   const double ln_scale = (MODE == kF32Lse) ? 0.69314718055994530942 : 1.0;
   const double coef = (1.0 / p.eps) * ((MODE == kF32Lse) ? 1.44269504088896340736 : 1.0);
   const bool timing = prof && prof->on;
@@ -444,10 +479,19 @@ static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_g
   g.smax = p.smax;
   g.Rq = p.Rq;
   g.s_in = p.s_in;
-  launch_grad_g<DIM, MODE>(p.G, g, p.n_twaves, p.K2, s);
+  g.Bpart = d_Bpart;
+  if (d_bary)
+    launch_grad_bary_g<DIM, MODE>(p.G, g, p.n_twaves, p.K2, s);
+  else
+    launch_grad_g<DIM, MODE>(p.G, g, p.n_twaves, p.K2, s);
   if (timing) (void)hipEventRecord(prof->ev[4], s);
   GFArgs gf{d.Gpart, d.trec_m, d.tm2o, d_grad, p.N, p.K2};
   hipLaunchKernelGGL((k_gfin<DIM>), dim3((unsigned)((p.N + 255) / 256)), dim3(256), 0, s, gf);
+  if (d_bary) {
+    Origin o{{p.origin[0], p.origin[1], p.origin[2]}};
+    hipLaunchKernelGGL((k_bfin<DIM>), dim3((unsigned)((p.N + 255) / 256)), dim3(256), 0, s,
+                       d.Gpart, d_Bpart, d.tm2o, o, p.N, p.K2, d_bary);
+  }
   FinArgs fa{d.Fpart, d.Cpart, d.Upart, d.PNpart, p.nFpart, p.nPpart, d_FC, d.ucount};
   hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, s, fa);
   if (timing) (void)hipEventRecord(prof->ev[5], s);
@@ -467,6 +511,24 @@ hipError_t launch_eval(const Plan &p, Dev &d, const double *d_psi, double *d_gra
       case kF64Lse: return eval_t<3, kF64Lse>(p, d, d_psi, d_grad, d_FC, s, prof);
       case kF64Plain: return eval_t<3, kF64Plain>(p, d, d_psi, d_grad, d_FC, s, prof);
       case kF32Lse: return eval_t<3, kF32Lse>(p, d, d_psi, d_grad, d_FC, s, prof);
+    }
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_eval_bary(const Plan &p, Dev &d, const double *d_psi, double *d_grad,
+                            double *d_FC, double *d_bary, double *d_Bpart, hipStream_t s) {
+  if (p.dim == 2) {
+    switch (p.mode) {
+      case kF64Lse: return eval_t<2, kF64Lse>(p, d, d_psi, d_grad, d_FC, s, nullptr, d_bary, d_Bpart);
+      case kF64Plain: return eval_t<2, kF64Plain>(p, d, d_psi, d_grad, d_FC, s, nullptr, d_bary, d_Bpart);
+      case kF32Lse: return eval_t<2, kF32Lse>(p, d, d_psi, d_grad, d_FC, s, nullptr, d_bary, d_Bpart);
+    }
+  } else if (p.dim == 3) {
+    switch (p.mode) {
+      case kF64Lse: return eval_t<3, kF64Lse>(p, d, d_psi, d_grad, d_FC, s, nullptr, d_bary, d_Bpart);
+      case kF64Plain: return eval_t<3, kF64Plain>(p, d, d_psi, d_grad, d_FC, s, nullptr, d_bary, d_Bpart);
+      case kF32Lse: return eval_t<3, kF32Lse>(p, d, d_psi, d_grad, d_FC, s, nullptr, d_bary, d_Bpart);
     }
   }
   return hipErrorInvalidValue;

@@ -174,6 +174,25 @@ class SotSolver:
         L.check(self.lib.sdot_eval_R(self._h, float(R), _p(psi), _p(g), C.byref(F), C.byref(Cg)))
         return F.value, g, Cg.value
 
+    def evaluate_weighted_barycenters(self, potential, params: SolverParameters | None = None):
+        """SotSolver::evaluate_weighted_barycenters with the Euclidean distance
+        (SotSolver.cc:585-610, compute_weighted_barycenters_euclidean :871-971):
+        per target the first moment of its column of the transport plan over
+        nu_j, with the threshold from the stored state.  `params` is accepted
+        for the reference's signature; the evaluation uses the configured
+        epsilon, as the reference uses the last solve's.  Returns [N, dim]."""
+        if self._n == 0:
+            raise RuntimeError("Invalid measures configuration")
+        out = np.zeros(self._n * self.dim)
+        L.check(self.lib.sdot_barycenters(self._h, _p(_f64(potential)), _p(out)))
+        return out.reshape(self._n, self.dim)
+
+    def compute_weighted_barycenters_R(self, R, potential):
+        """The barycenters at an explicit threshold R (no state involved)."""
+        out = np.zeros(self._n * self.dim)
+        L.check(self.lib.sdot_barycenters_R(self._h, float(R), _p(_f64(potential)), _p(out)))
+        return out.reshape(self._n, self.dim)
+
     def compute_hessian(self, potential):
         psi = _f64(potential)
         H = np.zeros((self._n, self._n))

@@ -16,7 +16,10 @@ values are the mathematical ones for those inputs; quadrature points come
 from textbook Gauss-Legendre nodes, not from the oracle.
 
 refine_cases.json holds the same kind of values for the softmax potential
-transfer between target levels (refine() below).
+transfer between target levels (refine() below), bary_cases.json for the
+weighted barycenters of the Lloyd step (barycenters() below), points_cases.json
+for the conditional densities and the barycentric transport map
+(point_queries() below).
 
 Run: python tests/golden/make_golden.py   (needs mpmath and numpy only)
 """
@@ -197,6 +200,111 @@ def refine_case(name, dim, n, Nc, Nf, seed, lam, R, lse):
     }
 
 
+def barycenters(dim, nq, x, rho, jxw, cen, y, nu, psi, eps, R):
+    """compute_weighted_barycenters_euclidean (SotSolver.cc:871-971):
+        b_j = sum_q (rho_q JxW_q / S_q) nu_j exp(e_qj) x_q / nu_j,
+    j over the candidates of the cell of q (cell rule by the center)."""
+    eps = mp.mpf(eps)
+    b = [[mp.mpf(0)] * dim for _ in range(len(y))]
+    for c in range(len(cen)):
+        cand = [j for j in range(len(y))
+                if mp.sqrt(sum((mp.mpf(y[j][k]) - mp.mpf(cen[c][k])) ** 2 for k in range(dim))) <= mp.mpf(R)]
+        for q in range(nq):
+            i = c * nq + q
+            es = {j: (mp.mpf(psi[j]) - sum((mp.mpf(x[i][k]) - mp.mpf(y[j][k])) ** 2
+                                            for k in range(dim)) / 2) / eps for j in cand}
+            S = sum(mp.mpf(nu[j]) * mp.exp(e) for j, e in es.items())
+            if S == 0:
+                continue
+            w = mp.mpf(rho[i]) * mp.mpf(jxw[i])
+            for j, e in es.items():
+                for k in range(dim):
+                    b[j][k] += w * mp.exp(e) / S * mp.mpf(x[i][k])
+    return b
+
+
+def bary_case(name, dim, n, N, seed, eps, R):
+    x, rho, jxw, cen, nq = box_source(dim, n, seed)
+    rng = np.random.default_rng(seed + 300)
+    y = rng.random((N, dim))
+    nu = rng.random(N) + 0.2
+    nu = nu / nu.sum()
+    psi = 0.05 * rng.standard_normal(N)
+    b = barycenters(dim, nq, x, rho, jxw, cen, y, nu, psi, eps, R)
+    return {"name": name, "dim": dim, "nq": nq, "eps": eps, "R": R,
+            "x": x.ravel().tolist(), "rho": rho.tolist(), "jxw": jxw.tolist(),
+            "center": cen.ravel().tolist(), "y": y.ravel().tolist(), "nu": nu.tolist(),
+            "psi": psi.tolist(), "bary": [[mp.nstr(v, 40) for v in r] for r in b]}
+
+
+def main_bary():
+    cases = [
+        bary_case("2d_untruncated", 2, 3, 6, 21, 0.05, 1.7976931348623157e308),
+        bary_case("3d_truncated", 3, 2, 7, 22, 0.1, 0.6),
+        bary_case("2d_truncated_small_eps", 2, 4, 9, 23, 0.01, 0.4),
+    ]
+    out = os.path.join(HERE, "bary_cases.json")
+    with open(out, "w") as f:
+        json.dump(cases, f)
+    print("wrote", out)
+
+
+def point_queries(dim, y, nu, psi, eps, R, pts, rho, idx, radius):
+    """get_potential_conditioned_density (SotSolver.cc:973-1046) at the points
+    with threshold R, and the barycentric map (OptimalTransportPlan.cc:218-310)
+    with truncation radius `radius` (strict <, nearest target when no target
+    is inside)."""
+    eps = mp.mpf(eps)
+    cond = [[mp.mpf(0)] * len(pts) for _ in idx]
+    bary = []
+    for p, x in enumerate(pts):
+        d2 = [sum((mp.mpf(x[k]) - mp.mpf(y[j][k])) ** 2 for k in range(dim)) for j in range(len(y))]
+        cand = [j for j in range(len(y)) if mp.sqrt(d2[j]) <= mp.mpf(R)]
+        Z = sum(mp.mpf(nu[j]) * mp.exp((mp.mpf(psi[j]) - d2[j] / 2) / eps) for j in cand)
+        if Z > 0:
+            for a, i in enumerate(idx):
+                if i in cand:
+                    cond[a][p] = mp.mpf(rho[p]) * mp.exp((mp.mpf(psi[i]) - d2[i] / 2) / eps) / Z
+        near = [j for j in range(len(y)) if mp.sqrt(d2[j]) < mp.mpf(radius)]
+        if not near:
+            jn = min(range(len(y)), key=lambda j: d2[j])
+            bary.append([mp.mpf(y[jn][k]) for k in range(dim)])
+            continue
+        wts = {j: mp.mpf(nu[j]) * mp.exp((mp.mpf(psi[j]) - d2[j] / 2) / eps) for j in near}
+        W = sum(wts.values())
+        bary.append([sum(w * mp.mpf(y[j][k]) for j, w in wts.items()) / W for k in range(dim)])
+    return cond, bary
+
+
+def points_case(name, dim, N, P, seed, eps, R, radius):
+    rng = np.random.default_rng(seed + 400)
+    y = rng.random((N, dim))
+    nu = rng.random(N) + 0.2
+    nu = nu / nu.sum()
+    psi = 0.05 * rng.standard_normal(N)
+    pts = rng.random((P, dim)) * 1.2 - 0.1
+    rho = 0.5 + rng.random(P)
+    idx = [0, N // 2, N - 1]
+    cond, bary = point_queries(dim, y, nu, psi, eps, R, pts, rho, idx, radius)
+    return {"name": name, "dim": dim, "eps": eps, "R": R, "radius": radius,
+            "y": y.ravel().tolist(), "nu": nu.tolist(), "psi": psi.tolist(),
+            "points": pts.ravel().tolist(), "rho": rho.tolist(), "indices": idx,
+            "cond": [[mp.nstr(v, 40) for v in r] for r in cond],
+            "bary_map": [[mp.nstr(v, 40) for v in r] for r in bary]}
+
+
+def main_points():
+    cases = [
+        points_case("2d", 2, 12, 40, 31, 0.02, 0.45, 0.3),
+        points_case("3d", 3, 15, 30, 32, 0.05, 0.6, 0.35),
+        points_case("2d_small_radius", 2, 10, 30, 33, 0.01, 1.7976931348623157e308, 0.12),
+    ]
+    out = os.path.join(HERE, "points_cases.json")
+    with open(out, "w") as f:
+        json.dump(cases, f)
+    print("wrote", out)
+
+
 def main_refine():
     cases = [
         refine_case("2d_truncated_lse", 2, 3, 6, 20, 11, 0.02, 0.35, True),
@@ -228,3 +336,5 @@ def main():
 if __name__ == "__main__":
     main()
     main_refine()
+    main_bary()
+    main_points()

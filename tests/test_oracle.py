@@ -242,3 +242,95 @@ def test_refinement_threshold_below_every_distance():
     pf = O.softmax_refinement(src, tgt.y, far, np.zeros(tgt.n), [[i] for i in range(tgt.n)],
                               0.01, 1.0, True)
     assert np.all(pf == 0.0)
+
+
+# ---- weighted barycenters of the Lloyd step (SotSolver.cc:871-971) ----
+
+BARY = json.load(open(os.path.join(HERE, "golden", "bary_cases.json")))
+
+
+@pytest.mark.parametrize("case", BARY, ids=[c["name"] for c in BARY])
+def test_barycenters_match_50_digit_golden(case):
+    src = O.Source(case["dim"], case["nq"], case["x"], case["rho"], case["jxw"], case["center"])
+    tgt = O.Target(case["y"], case["nu"])
+    b = O.barycenters_R(src, tgt, O.params(epsilon=case["eps"], use_lse=True, n_threads=2),
+                        case["R"], case["psi"])
+    ref = np.array([[float(v) for v in r] for r in case["bary"]])
+    assert np.max(np.abs(b - ref)) <= 1e-14 * max(1.0, np.max(np.abs(ref)))
+
+
+def test_barycenters_closed_forms():
+    """One target of weight 1 takes the whole source: its barycenter is
+    sum_q w_q x_q.  Moving source and targets by v moves b_j by
+    v (1 + g_j / nu_j), the plan's column mass over nu_j, with g the gradient
+    the evaluation assembles (no truncation)."""
+    src, _ = structured_case(2, 8, 2, 1, 3, gmm_source=True)
+    one = O.Target(np.array([0.3, 0.6]), np.array([1.0]))
+    b = O.barycenters_R(src, one, O.params(epsilon=0.01, use_lse=True), np.finfo(float).max,
+                        np.zeros(1))
+    xs = src.x.reshape(-1, 2)
+    assert np.max(np.abs(b[0] - (src.w[:, None] * xs).sum(0))) <= 1e-14
+    src3, tgt3 = structured_case(3, 6, 2, 40, 9, gmm_source=True, dirichlet=4.0)
+    psi = 0.01 * np.random.default_rng(9).standard_normal(tgt3.n)
+    prm = O.params(epsilon=0.02, use_lse=True, n_threads=2)
+    dmax = np.finfo(float).max
+    b0 = O.barycenters_R(src3, tgt3, prm, dmax, psi)
+    _, g, _ = O.evaluate_R(src3, tgt3, prm, dmax, psi)
+    v = np.array([0.25, -0.5, 0.125])
+    moved = O.Source(3, src3.nq, (src3.x.reshape(-1, 3) + v).ravel(), src3.rho, src3.jxw,
+                     (src3.center.reshape(-1, 3) + v).ravel())
+    b1 = O.barycenters_R(moved, O.Target((tgt3.y.reshape(-1, 3) + v).ravel(), tgt3.nu), prm,
+                         dmax, psi)
+    pred = b0 + v[None, :] * (1.0 + g / tgt3.nu)[:, None]
+    assert np.max(np.abs(b1 - pred)) <= 1e-12
+
+
+# ---- conditional densities and transport maps ----
+
+POINTS = json.load(open(os.path.join(HERE, "golden", "points_cases.json")))
+
+
+@pytest.mark.parametrize("case", POINTS, ids=[c["name"] for c in POINTS])
+def test_point_queries_match_50_digit_golden(case):
+    """Conditional densities at points with threshold R (zero where the
+    requested target is outside), barycentric map with a strict truncation
+    radius and the nearest-target fallback."""
+    tgt = O.Target(case["y"], case["nu"])
+    cond = O.conditional_density(tgt, case["dim"], case["psi"], case["eps"], case["R"],
+                                 case["points"], case["rho"], case["indices"])
+    ref = np.array([[float(v) for v in r] for r in case["cond"]])
+    assert np.array_equal(cond == 0.0, ref == 0.0)
+    assert np.max(np.abs(cond - ref)) <= 1e-13 * max(1.0, np.max(np.abs(ref)))
+    b = O.transport_map(1, tgt, case["dim"], case["psi"], case["eps"], case["radius"],
+                        case["points"])
+    rb = np.array([[float(v) for v in r] for r in case["bary_map"]])
+    assert np.max(np.abs(b - rb)) <= 1e-14
+
+
+def test_point_queries_closed_forms():
+    """sum_i nu_i c_i(x) = rho(x) over all targets (no truncation); the modal
+    map with psi = -eps log nu is the nearest-target (Voronoi) map; the
+    barycentric map of one target is that target, and tends to the modal map
+    as eps -> 0 away from ties."""
+    rng = np.random.default_rng(0)
+    y = rng.random((30, 2))
+    nu = rng.dirichlet(np.ones(30))
+    psi = 0.01 * rng.standard_normal(30)
+    tgt = O.Target(y.ravel(), nu)
+    pts = rng.random((200, 2))
+    rho = 1.0 + rng.random(200)
+    c = O.conditional_density(tgt, 2, psi, 0.02, np.finfo(float).max, pts, rho, np.arange(30))
+    assert np.max(np.abs((nu[:, None] * c).sum(0) - rho)) <= 1e-14
+    eps = 0.02
+    m = O.transport_map(0, tgt, 2, -eps * np.log(nu), eps, 0.0, pts)
+    vor = y[np.argmin(((pts[:, None, :] - y[None]) ** 2).sum(2), 1)]
+    assert np.array_equal(m, vor)
+    one = O.transport_map(1, O.Target(y[:1].ravel(), np.ones(1)), 2, psi[:1], eps, 0.0, pts)
+    assert np.array_equal(one, np.repeat(y[:1], len(pts), 0))
+    e = 1e-4
+    b = O.transport_map(1, tgt, 2, psi, e, 0.0, pts)
+    mo = O.transport_map(0, tgt, 2, psi, e, 0.0, pts)
+    sc = psi[None, :] - 0.5 * ((pts[:, None, :] - y[None]) ** 2).sum(2) + e * np.log(nu)[None, :]
+    srt = np.sort(sc, 1)
+    ok = srt[:, -1] - srt[:, -2] > 50 * e
+    assert ok.sum() > 50 and np.max(np.abs(b[ok] - mo[ok])) <= 1e-12
