@@ -9,6 +9,7 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -17,7 +18,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libsdot.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
-HIP_SOURCES = ["sdot_kernels.hip", "sdot_serve_kernels.hip", "sdot_refine.hip"]
+HIP_SOURCES = ["sdot_kernels.hip", "sdot_serve_kernels.hip", "sdot_refine.hip", "sdot_points.hip"]
 CXX_SOURCES = ["sdot_api.cpp", "sdot_mesh.cpp", "sdot_prm.cpp", "sdot_serve.cpp"]
 
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
@@ -36,17 +37,21 @@ def _run(cmd, verbose):
 def build(verbose: bool = False, extra_hip_flags=None) -> str:
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
-    objs = []
     hipcc = os.path.join(ROCM, "bin", "hipcc")
+    jobs = []
     for src in HIP_SOURCES:
         obj = os.path.join(BUILD, src + ".o")
-        _run([hipcc, *HIPCC_FLAGS, *(extra_hip_flags or []), "-c",
-              os.path.join(CSRC, src), "-o", obj], verbose)
-        objs.append(obj)
+        jobs.append(([hipcc, *HIPCC_FLAGS, *(extra_hip_flags or []), "-c",
+                      os.path.join(CSRC, src), "-o", obj], obj))
     for src in CXX_SOURCES:
         obj = os.path.join(BUILD, src + ".o")
-        _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj], verbose)
-        objs.append(obj)
+        jobs.append((["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj], obj))
+    # the translation units compile independently: one compiler process each
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        for f in [pool.submit(_run, cmd, verbose) for cmd, _ in jobs]:
+            f.result()
+    objs = [obj for _, obj in jobs]
     tmp = LIB + ".tmp"
     _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs,
           f"-L{ROCM}/lib", "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib"], verbose)
