@@ -197,6 +197,35 @@ int sdot_softmax_refinement(sdot_ctx *ctx, int64_t n_fine, const double *y_fine,
                             double lambda, double distance_threshold,
                             int use_log_sum_exp_trick, double *psi_fine);
 
+/* ---- point queries after a solve ------------------------------------------ */
+/* SotSolver::get_potential_conditioned_density (SotSolver.cc:973-1046) over the
+ * context's target measure at epsilon = the configured epsilon (the reference's
+ * current_epsilon) and threshold R (its current_distance_threshold; R >= DBL_MAX:
+ * every target): at each point p (the reference's DoF support points) the
+ * targets j with |p - y_j| <= R,
+ *   c_i(p) = rho_p exp(e_pi - M_p) / sum_j nu_j exp(e_pj - M_p),
+ *   e_pj = (psi_j - |p - y_j|^2 / 2) / eps,  M_p = max_j e_pj,
+ * 0 where i is not among p's candidates or the sum is not positive.
+ *   points [n_points*dim], density [n_points] (rho at the points),
+ *   indices [n_idx] (potential_indices), out [n_idx*n_points] row k = index k. */
+int sdot_conditional_density(sdot_ctx *ctx, const double *psi, double R, int64_t n_points,
+                             const double *points, const double *density, int64_t n_idx,
+                             const int64_t *indices, double *out);
+
+/* ModalStrategy / BarycentricStrategy::compute_map (OptimalTransportPlan.cc:
+ * 99-189, 218-310) over the context's target measure: at each point x the
+ * targets with |x - y_j| < radius (every target when radius <= 0; the nearest
+ * target when none is that close),
+ *   MODAL:       y_j of the maximum of psi_j - |x - y_j|^2 / 2 + epsilon log nu_j
+ *                (first maximum in index order),
+ *   BARYCENTRIC: sum_j nu_j e^(e_j - M) y_j / sum_j nu_j e^(e_j - M),
+ *                e_j = (psi_j - |x - y_j|^2 / 2) / epsilon (the nearest target
+ *                where the sum is not positive).
+ * Nearest: lowest index among equal distances.  mapped [n_points*dim] out. */
+typedef enum { SDOT_MAP_MODAL = 0, SDOT_MAP_BARYCENTRIC = 1 } sdot_map_strategy;
+int sdot_transport_map(sdot_ctx *ctx, int strategy, const double *psi, double epsilon,
+                       double radius, int64_t n_points, const double *points, double *mapped);
+
 /* ---- thresholds and audit ------------------------------------------------ */
 /* compute_covering_radius (SotSolver.cc:431-473), raw (without the 1.1). */
 int sdot_covering_radius(sdot_ctx *ctx, double *r0);

@@ -99,6 +99,32 @@ class SotSolverGPU {
     check(sdot_hessian(ctx_, potential.data(), hessian_out.data(), n_));
   }
 
+  // SotSolver::evaluate_weighted_barycenters (SotSolver.cc:585-610): [N*spacedim].
+  void evaluate_weighted_barycenters(const std::vector<double> &potential,
+                                     std::vector<double> &barycenters_out) {
+    barycenters_out.resize((size_t)n_ * spacedim);
+    check(sdot_barycenters(ctx_, potential.data(), barycenters_out.data()));
+  }
+
+  // SotSolver::get_potential_conditioned_density (SotSolver.cc:973-1046) at the
+  // caller's support points [n*spacedim] with the density there [n]; row k of
+  // the result [indices.size()*n] belongs to potential_indices[k].  Threshold:
+  // the last distance threshold, epsilon: the configured one.
+  void get_potential_conditioned_density(const std::vector<double> &points,
+                                         const std::vector<double> &density,
+                                         const std::vector<double> &potential,
+                                         const std::vector<int64_t> &potential_indices,
+                                         std::vector<double> &conditioned_densities) {
+    const int64_t n = (int64_t)density.size();
+    if ((int64_t)points.size() != n * spacedim)
+      throw Error(SDOT_ERR_INVALID, "conditioned density: points and density differ in size");
+    conditioned_densities.assign(potential_indices.size() * (size_t)n, 0.0);
+    check(sdot_conditional_density(ctx_, potential.data(), get_last_distance_threshold(), n,
+                                   points.data(), density.data(),
+                                   (int64_t)potential_indices.size(), potential_indices.data(),
+                                   conditioned_densities.data()));
+  }
+
   double get_last_functional_value() const { return state().last_functional; }
   unsigned int get_last_iteration_count() const { return (unsigned)state().last_iteration_count; }
   bool get_convergence_status() const { return state().converged != 0; }
@@ -167,6 +193,69 @@ class SoftmaxRefinementGPU {
   sdot_ctx *ctx_;
   double threshold_;
   bool lse_;
+};
+
+// OptimalTransportPlan<spacedim> (OptimalTransportPlan.h, OptimalTransportPlan.cc:
+// 8-96) with its "modal" and "barycentric" strategies (:99-189, :218-310).
+// Points flat (n*spacedim).  The source points are held by a context as
+// one-point cells; the map reads only the targets.
+template <int spacedim>
+class OptimalTransportPlanGPU {
+  static_assert(spacedim == 2 || spacedim == 3, "spacedim must be 2 or 3");
+
+ public:
+  explicit OptimalTransportPlanGPU(const std::string &strategy_name = "modal", int device = 0)
+      : device_(device) {
+    set_strategy(strategy_name);
+  }
+  void set_strategy(const std::string &name) {
+    if (name == "modal")
+      strategy_ = SDOT_MAP_MODAL;
+    else if (name == "barycentric")
+      strategy_ = SDOT_MAP_BARYCENTRIC;
+    else
+      throw std::runtime_error("Unknown strategy: " + name);
+  }
+  static std::vector<std::string> get_available_strategies() { return {"modal", "barycentric"}; }
+  void set_source_measure(const std::vector<double> &points, const std::vector<double> &density) {
+    if (points.size() != density.size() * spacedim)
+      throw Error(SDOT_ERR_INVALID, "source points and density differ in size");
+    src_ = points;
+    rho_ = density;
+  }
+  void set_target_measure(const std::vector<double> &points, const std::vector<double> &density) {
+    if (points.size() != density.size() * spacedim)
+      throw Error(SDOT_ERR_INVALID, "target points and density differ in size");
+    tgt_ = points;
+    nu_ = density;
+  }
+  void set_potential(const std::vector<double> &potential, double regularization_param) {
+    if (potential.size() != nu_.size())
+      throw Error(SDOT_ERR_INVALID, "potential size differs from the target count");
+    psi_ = potential;
+    eps_ = regularization_param;
+  }
+  void set_truncation_radius(double radius) { radius_ = radius; }
+  void compute_map() {
+    if (rho_.empty()) throw std::runtime_error("Source measure not set");
+    if (nu_.empty()) throw std::runtime_error("Target measure not set");
+    if (psi_.empty()) throw std::runtime_error("Transport potential not set");
+    SotSolverGPU<spacedim, spacedim> s(device_);
+    std::vector<double> w(rho_.size());
+    for (size_t i = 0; i < w.size(); ++i) w[i] = rho_[i] < 0 ? -rho_[i] : rho_[i];
+    s.setup_source(src_, w, src_, 1);
+    s.setup_target(tgt_, nu_);
+    mapped_points.assign(src_.size(), 0.0);
+    check(sdot_transport_map(s.handle(), strategy_, psi_.data(), eps_, radius_,
+                             (int64_t)rho_.size(), src_.data(), mapped_points.data()));
+    transport_density = rho_;
+  }
+  std::vector<double> mapped_points, transport_density;
+
+ private:
+  int device_, strategy_ = SDOT_MAP_MODAL;
+  std::vector<double> src_, rho_, tgt_, nu_, psi_;
+  double eps_ = 1e-2, radius_ = -1.0;
 };
 
 }  // namespace sdot

@@ -11,6 +11,7 @@ from .solver import SolverParameters, SotSolver  # noqa: F401
 from .server import ServeResult, SotServer  # noqa: F401
 from .epsilon_scaling import EpsilonScalingHandler  # noqa: F401
 from .lloyd import run_lloyd  # noqa: F401
+from .transport_plan import OptimalTransportPlan  # noqa: F401
 from .multilevel import (MultilevelParameters, SoftmaxRefinement,  # noqa: F401
                          generate_target_hierarchy, run_combined_multilevel,
                          run_target_multilevel)
@@ -18,4 +19,5 @@ from .multilevel import (MultilevelParameters, SoftmaxRefinement,  # noqa: F401
 __all__ = ["SotSolver", "SolverParameters", "SotServer", "ServeResult", "SourceArrays", "structured_source",
            "simplex_source", "SdotError", "load", "LIB_PATH", "EpsilonScalingHandler",
            "SoftmaxRefinement", "MultilevelParameters", "generate_target_hierarchy",
-           "run_target_multilevel", "run_combined_multilevel", "run_lloyd"]
+           "run_target_multilevel", "run_combined_multilevel", "run_lloyd",
+           "OptimalTransportPlan"]

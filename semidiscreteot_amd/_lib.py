@@ -101,6 +101,10 @@ SIGNATURES = {
     "sdot_softmax_refinement": (C.c_int, [C.c_void_p, C.c_int64, _dp, C.c_int64, _dp, _dp, _dp,
                                           C.c_int64, _i64p, _i64p, C.c_double, C.c_double,
                                           C.c_int, _dp]),
+    "sdot_conditional_density": (C.c_int, [C.c_void_p, _dp, C.c_double, C.c_int64, _dp, _dp,
+                                           C.c_int64, _i64p, _dp]),
+    "sdot_transport_map": (C.c_int, [C.c_void_p, C.c_int, _dp, C.c_double, C.c_double,
+                                     C.c_int64, _dp, _dp]),
     "sdot_solve": (C.c_int, [C.c_void_p, _dp, C.POINTER(SolveInfo)]),
     "sdot_covering_radius": (C.c_int, [C.c_void_p, _dp]),
     "sdot_distance_threshold": (C.c_int, [C.c_void_p, _dp, _dp]),

@@ -1074,6 +1074,191 @@ extern "C" int sdot_softmax_refinement(sdot_ctx *c, int64_t n_fine, const double
   return SDOT_OK;
 }
 
+// Largest double s with sqrt(s) < R: the strict test of the transport maps'
+// truncation (OptimalTransportPlan.cc:136, :262) as s <= smax.
+static double smax_strict(double R) {
+  if (std::isnan(R) || !(R > 0)) return -1.0;
+  if (std::isinf(R)) return INFINITY;
+  double s = R * R;
+  if (std::isinf(s)) s = DBL_MAX;
+  while (s > 0 && !(std::sqrt(s) < R)) s = std::nextafter(s, 0.0);
+  for (;;) {
+    const double n = std::nextafter(s, INFINITY);
+    if (std::isinf(n) || !(std::sqrt(n) < R)) break;
+    s = n;
+  }
+  return s;
+}
+
+// Point queries over the context's target measure (op 0: conditional
+// densities at R; 1: barycentric map; 2: modal map, truncated at `radius`).
+static int host_points(sdot_ctx *c, int op, const double *psi, double eps, double R,
+                       int64_t n, const double *points, const double *density, int64_t n_idx,
+                       const int64_t *indices, double *out) {
+  if (!c->have_target) return fail(SDOT_ERR_STATE, "Invalid measures configuration: no target");
+  if (n < 0 || n >= (1ll << 40)) return fail(SDOT_ERR_INVALID, "point count out of range");
+  if (n_idx < 0) return fail(SDOT_ERR_INVALID, "index count < 0");
+  if (!psi || (n > 0 && (!points || !out)) || (op == 0 && n > 0 && (!density || (n_idx > 0 && !indices))))
+    return fail(SDOT_ERR_INVALID, "null argument");
+  if (!(eps > 0) || !std::isfinite(eps))
+    return fail(SDOT_ERR_INVALID, "regularization parameter must be finite and > 0");
+  if (std::isnan(R)) return fail(SDOT_ERR_INVALID, "radius is NaN");
+  const Plan &p = c->plan;
+  const int dim = p.dim;
+  const int64_t N = p.N;
+  if (!all_finite(psi, N) || !all_finite(points, n * dim))
+    return fail(SDOT_ERR_INVALID, "non-finite potential or points");
+  if (n == 0) return SDOT_OK;
+  HIP_TRY(hipSetDevice(c->device));
+
+  // query points in Morton order (waves of nearby points share their bin rows)
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t i = 0; i < n; ++i)
+    for (int d = 0; d < dim; ++d) {
+      lo[d] = std::min(lo[d], points[i * dim + d]);
+      hi[d] = std::max(hi[d], points[i * dim + d]);
+    }
+  std::vector<uint64_t> key(n);
+  for (int64_t i = 0; i < n; ++i) key[i] = morton_key(points + i * dim, lo, hi, dim);
+  std::vector<int64_t> m2o(n);
+  std::iota(m2o.begin(), m2o.end(), 0);
+  std::stable_sort(m2o.begin(), m2o.end(), [&](int64_t a, int64_t b) { return key[a] < key[b]; });
+  std::vector<double> px((size_t)n * dim);
+  for (int64_t m = 0; m < n; ++m)
+    for (int d = 0; d < dim; ++d) px[m * dim + d] = points[m2o[m] * dim + d];
+  std::vector<double> logterm;
+  if (op == 2) {
+    // the reference's log term, with the same libm log (OptimalTransportPlan.cc:170-175)
+    const std::vector<double> &nu = c->h_nu;
+    logterm.resize(N);
+    for (int64_t j = 0; j < N; ++j) logterm[j] = nu[j] > 0 ? eps * std::log(nu[j]) : -INFINITY;
+  }
+
+  PtArgs a{};
+  a.tg = p.tg;
+  a.n = n;
+  a.N = N;
+  a.tstart = c->dev.tstart;
+  a.tpos_r = c->dev.tpos_r;
+  a.ty = c->dev.ty_o;
+  a.nu = c->dev.tnu_o;
+  a.eps = eps;
+  a.eps_inv = 1.0 / eps;
+  a.divide = op == 1 ? 1 : 0;
+  a.moments = op == 1 ? 1 : 0;
+  const bool trunc = op == 0 ? !(R >= DBL_MAX) : R > 0.0;
+  a.truncated = op != 0 && trunc ? 1 : 0;
+  a.smax = op == 0 ? smax_for(R) : (trunc ? smax_strict(R) : INFINITY);
+  a.Rq = trunc ? R * (1.0 + 1e-9) + 1e-9 * (p.tg.h + c->coord_scale) : INFINITY;
+  if (op == 0 && R < 0) a.Rq = 0.0;
+  double span2 = 0.0;
+  for (int d = 0; d < dim; ++d) {
+    const double a0 = std::min(lo[d], p.tg.o[d]), a1 = std::max(hi[d], p.tg.o[d] + p.tg.n[d] * p.tg.h);
+    span2 += (a1 - a0) * (a1 - a0);
+  }
+  a.rho_all = std::sqrt(span2) * (1.0 + 1e-9) + 1e-9;
+  a.rho0 = std::min(a.rho_all, std::max(trunc ? R : 0.0, p.tg.h));
+  const int64_t nw = (n + kWave - 1) / kWave;
+  a.K = (int)std::min<int64_t>(64, std::max<int64_t>(1, 8192 / nw));
+  if (a.K >= 8) a.K -= a.K % 8;
+  a.n_idx = op == 0 ? n_idx : 0;
+
+  std::vector<void *> tmp;
+  auto alloc = [&](auto *&ptr, size_t cnt) {
+    void *b = nullptr;
+    hipError_t e = hipMalloc(&b, std::max<size_t>(cnt, 1) * sizeof(*ptr));
+    if (e == hipSuccess) tmp.push_back(b);
+    ptr = (std::remove_reference_t<decltype(ptr)>)b;
+    return e;
+  };
+  double *d_psi = nullptr, *d_px = nullptr, *d_praw = nullptr, *d_rec = nullptr, *d_log = nullptr,
+         *d_M = nullptr,
+         *d_S = nullptr, *d_B = nullptr, *d_Mo = nullptr, *d_So = nullptr, *d_map = nullptr,
+         *d_rho = nullptr, *d_cond = nullptr;
+  int64_t *d_m2o = nullptr, *d_C = nullptr, *d_I = nullptr, *d_idx = nullptr;
+  int32_t *d_need = nullptr;
+  const size_t Kn = (size_t)a.K * n;
+  hipError_t e = alloc(d_px, (size_t)n * dim);
+  if (e == hipSuccess) e = alloc(d_psi, (size_t)N);
+  if (e == hipSuccess) e = alloc(d_m2o, (size_t)n);
+  if (e == hipSuccess) e = alloc(d_rec, (size_t)N * kRec);
+  if (e == hipSuccess) e = alloc(d_M, Kn);
+  if (e == hipSuccess) e = alloc(d_C, Kn);
+  if (e == hipSuccess && op != 2) e = alloc(d_S, Kn);
+  if (e == hipSuccess && op == 1) e = alloc(d_B, Kn * dim);
+  if (e == hipSuccess && op == 2) e = alloc(d_I, Kn);
+  if (e == hipSuccess && op == 2) e = alloc(d_log, (size_t)N);
+  if (e == hipSuccess && op != 0) e = alloc(d_need, (size_t)n);
+  if (e == hipSuccess && op != 0) e = alloc(d_map, (size_t)n * dim);
+  if (e == hipSuccess && op == 0) e = alloc(d_Mo, (size_t)n);
+  if (e == hipSuccess && op == 0) e = alloc(d_So, (size_t)n);
+  if (e == hipSuccess && op == 0) e = alloc(d_praw, (size_t)n * dim);
+  if (e == hipSuccess && op == 0) e = alloc(d_rho, (size_t)n);
+  if (e == hipSuccess && op == 0) e = alloc(d_idx, (size_t)n_idx);
+  if (e == hipSuccess && op == 0) e = alloc(d_cond, (size_t)n_idx * n);
+  auto h2d = [&](void *dst, const void *src, size_t bytes) {
+    if (e == hipSuccess && bytes > 0)
+      e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream);
+  };
+  h2d(d_psi, psi, sizeof(double) * N);
+  h2d(d_px, px.data(), px.size() * sizeof(double));
+  h2d(d_m2o, m2o.data(), m2o.size() * sizeof(int64_t));
+  if (op == 2) h2d(d_log, logterm.data(), logterm.size() * sizeof(double));
+  if (op == 0) {
+    h2d(d_praw, points, sizeof(double) * n * dim);
+    h2d(d_rho, density, sizeof(double) * n);
+    h2d(d_idx, indices, sizeof(int64_t) * n_idx);
+  }
+  a.px = d_px;
+  a.psi = d_psi;
+  a.praw = d_praw;
+  a.pm2o = d_m2o;
+  a.rec_w = d_rec;
+  a.logterm = d_log;
+  a.Mpart = d_M;
+  a.Spart = d_S;
+  a.Bpart = d_B;
+  a.Cpart = d_C;
+  a.Ipart = d_I;
+  a.Mo = d_Mo;
+  a.So = d_So;
+  a.need = d_need;
+  a.mapped = d_map;
+  a.idx = d_idx;
+  a.rho = d_rho;
+  a.cond = d_cond;
+  if (e == hipSuccess) e = launch_points(dim, a, op, c->stream);
+  if (e == hipSuccess) {
+    if (op == 0 && n_idx > 0)
+      e = hipMemcpyAsync(out, d_cond, sizeof(double) * n_idx * n, hipMemcpyDeviceToHost, c->stream);
+    else if (op != 0)
+      e = hipMemcpyAsync(out, d_map, sizeof(double) * n * dim, hipMemcpyDeviceToHost, c->stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  for (void *b : tmp) (void)hipFree(b);
+  if (e != hipSuccess) return fail(SDOT_ERR_HIP, std::string("point query: ") + hipGetErrorString(e));
+  return SDOT_OK;
+}
+
+extern "C" int sdot_conditional_density(sdot_ctx *c, const double *psi, double R,
+                                        int64_t n_points, const double *points,
+                                        const double *density, int64_t n_idx,
+                                        const int64_t *indices, double *out) {
+  if (ctx_check(c)) return SDOT_ERR_INVALID;
+  return host_points(c, 0, psi, c->params.epsilon, R, n_points, points, density, n_idx, indices,
+                     out);
+}
+
+extern "C" int sdot_transport_map(sdot_ctx *c, int strategy, const double *psi, double epsilon,
+                                  double radius, int64_t n_points, const double *points,
+                                  double *mapped) {
+  if (ctx_check(c)) return SDOT_ERR_INVALID;
+  if (strategy != SDOT_MAP_MODAL && strategy != SDOT_MAP_BARYCENTRIC)
+    return fail(SDOT_ERR_INVALID, "Unknown strategy");
+  return host_points(c, strategy == SDOT_MAP_MODAL ? 2 : 1, psi, epsilon, radius, n_points,
+                     points, nullptr, 0, nullptr, mapped);
+}
+
 extern "C" int sdot_solve(sdot_ctx *c, double *psi, sdot_solve_info *info) {
   if (ctx_check(c)) return SDOT_ERR_INVALID;
   if (!c->have_source || !c->have_target)

@@ -30,6 +30,40 @@ __device__ __forceinline__ bool in_set(const double *y, const double *c, double 
   return s <= smax;
 }
 
+// Squared distance ((0 + t0^2) + t1^2) + t2^2 without FMA contraction: the
+// reference's FP64 sum of squares under its square root.
+template <int DIM>
+__device__ __forceinline__ double sq_dist(const double *a, const double *b) {
+#pragma clang fp contract(off)
+  double s = 0.0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const double t = a[d] - b[d];
+    s = s + t * t;
+  }
+  return s;
+}
+
+// The value of lane 0 in every lane, held in scalar registers.
+__device__ __forceinline__ double wave_uniform(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(unsigned)(b & 0xffffffffll));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = fmin(v, __shfl_xor(v, m));
+  return wave_uniform(v);
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m));
+  return wave_uniform(v);
+}
+
 // Records a kernel only reads are addressed through the constant address
 // space: the compiler may then keep wave-uniform record loads in scalar
 // registers (s_load) even where a store of the same kernel could alias them

@@ -280,6 +280,42 @@ struct RefArgs {
 double s2max_for(double R);
 hipError_t launch_refinement(int dim, const RefArgs &a, hipStream_t s);
 
+// Point queries over a context's target measure (sdot_points.hip): conditional
+// densities (SotSolver::get_potential_conditioned_density, SotSolver.cc:973-1046)
+// and transport maps (OptimalTransportPlan.cc:99-189, 218-310), FP64.
+struct PtArgs {
+  const double *px;       // [n*DIM] query points, Morton order
+  const double *praw;     // [n*DIM] query points, caller order
+  const int64_t *pm2o;    // [n] Morton position -> caller index
+  const double *rec;      // [N*kRec] targets, row (bin) order (written by k_pt_rec)
+  double *rec_w;
+  const int32_t *tstart;  // target bin offsets
+  const int32_t *tpos_r;  // caller index -> row position
+  const double *ty, *nu, *psi;  // caller order
+  const double *logterm;  // [N] eps log nu_j (-inf where nu_j <= 0), modal map only
+  Grid tg;
+  int64_t n, N;
+  int K;
+  int divide;     // exponent (psi - sq/2) / eps instead of * (1/eps)
+  int moments;    // sum the first moments (barycentric map)
+  int truncated;  // points without a candidate take the nearest target
+  double smax;    // candidate test s <= smax
+  double Rq;      // enumeration radius
+  double eps, eps_inv;
+  double rho0, rho_all;  // nearest-target scans
+  double *Mpart, *Spart, *Bpart;  // [K*n], [K*n], [K*DIM*n], Morton order
+  int64_t *Cpart, *Ipart;         // [K*n] candidate counts, modal indices
+  double *Mo, *So;                // [n] caller order (conditional densities)
+  int32_t *need;                  // [n] Morton order: nearest-target fallback
+  double *mapped;                 // [n*DIM] caller order
+  int64_t n_idx;
+  const int64_t *idx;             // [n_idx] requested target indices
+  const double *rho;              // [n] density at the points, caller order
+  double *cond;                   // [n_idx*n]
+};
+// op 0: conditional densities, 1: barycentric map, 2: modal map.
+hipError_t launch_points(int dim, const PtArgs &a, int op, hipStream_t s);
+
 // Kernel launchers (sdot_kernels.hip).
 hipError_t launch_eval(const Plan &p, Dev &d, const double *d_psi, double *d_grad,
                        double *d_FC, hipStream_t s, Profile *prof);

@@ -35,42 +35,6 @@
 
 namespace sdot {
 
-namespace {
-
-template <int DIM>
-__device__ __forceinline__ double sq_dist(const double *a, const double *b) {
-#pragma clang fp contract(off)
-  double s = 0.0;
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) {
-    const double t = a[d] - b[d];
-    s = s + t * t;
-  }
-  return s;
-}
-
-// The value of lane 0 in every lane, held in scalar registers.
-__device__ __forceinline__ double wave_uniform(double v) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readfirstlane((int)(unsigned)(b & 0xffffffffll));
-  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
-  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-}
-
-__device__ __forceinline__ double wave_min(double v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v = fmin(v, __shfl_xor(v, m));
-  return wave_uniform(v);
-}
-
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m));
-  return wave_uniform(v);
-}
-
-}  // namespace
-
 template <int DIM, int PASS>
 __global__ __launch_bounds__(64) void k_ref_coarse(RefArgs a) {
   const int64_t q = (int64_t)blockIdx.x * kWave + threadIdx.x;

@@ -193,6 +193,30 @@ class SotSolver:
         L.check(self.lib.sdot_barycenters_R(self._h, float(R), _p(_f64(potential)), _p(out)))
         return out.reshape(self._n, self.dim)
 
+    def get_potential_conditioned_density(self, points, density, potential, potential_indices,
+                                          R=None):
+        """SotSolver::get_potential_conditioned_density (SotSolver.cc:973-1046)
+        at `points` (the reference's DoF support points, [n, dim]) with the
+        source density there: row k is rho(p) exp(e_pi - M_p) / S_p for
+        i = potential_indices[k], 0 where i is not within the threshold of p.
+        The threshold is the last distance threshold (the reference's
+        current_distance_threshold) unless R is given; epsilon is the configured
+        one.  Returns [len(potential_indices), n]."""
+        if self._n == 0:
+            raise RuntimeError("Invalid measures configuration")
+        pts = _f64(points).reshape(-1, self.dim)
+        n = len(pts)
+        rho = _f64(density).ravel()
+        if rho.size != n:
+            raise ValueError("density must have one value per point")
+        idx = np.ascontiguousarray(np.asarray(potential_indices, dtype=np.int64).ravel())
+        out = np.zeros((idx.size, n))
+        thr = self.get_last_distance_threshold() if R is None else float(R)
+        L.check(self.lib.sdot_conditional_density(
+            self._h, _p(_f64(potential)), thr, n, _p(pts), _p(rho), idx.size,
+            idx.ctypes.data_as(L._i64p), _p(out)))
+        return out
+
     def compute_hessian(self, potential):
         psi = _f64(potential)
         H = np.zeros((self._n, self._n))

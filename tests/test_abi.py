@@ -144,6 +144,13 @@ def test_cpp_facade_compiles_and_links(tmp_path):
                    "    const std::vector<std::vector<std::vector<size_t>>> &ch) {\n"
                    "  sdot::SoftmaxRefinementGPU<3> r(s, 0.1, true);\n"
                    "  return r.compute_refinement({}, {}, {}, {}, {}, 0.1, 0, ch); }\n"
+                   "void post(sdot::SotSolverGPU<2> &s, std::vector<double> &out) {\n"
+                   "  s.evaluate_weighted_barycenters({}, out);\n"
+                   "  s.get_potential_conditioned_density({}, {}, {}, {}, out);\n"
+                   "  sdot::OptimalTransportPlanGPU<2> plan(\"barycentric\");\n"
+                   "  plan.set_source_measure({}, {}); plan.set_target_measure({}, {});\n"
+                   "  plan.set_potential({}, 0.1); plan.set_truncation_radius(0.2);\n"
+                   "  plan.compute_map(); out = plan.mapped_points; }\n"
                    "int main() { auto p = sdot::SotSolverGPU<3>::default_parameters();\n"
                    "  return (p.max_iterations == 1000 && p.tau == 1e-8) ? 0 : 1; }\n")
     exe = tmp_path / "facade"
