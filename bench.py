@@ -146,15 +146,19 @@ def serving_leg(s, args):
       one-at-a-time: the FP64 single-solve path (sdot_solve), requests in
                 arrival order, each started when it has arrived and the
                 previous one has finished.
-    Both use the noise-aware line search; every batched result is re-checked
-    with an FP64 evaluation at the returned potential."""
+    Both use the noise-aware line search and the same initial inverse Hessian
+    of the L-BFGS recursion (`--serve-precond`: diag(eps/nu_j) by default);
+    every batched result is re-checked with an FP64 evaluation at the returned
+    potential.  With `--serve-identity-leg` the one-at-a-time FP64 path also
+    runs with the identity (the reference's SolverBFGS) for comparison."""
     import itertools
     import semidiscreteot_amd as sd
     from semidiscreteot_amd import workloads
     dmax = np.finfo(float).max
     reqs = list(itertools.islice(workloads.serving_trace(), args.serve_requests))
     kw = dict(control="componentwise", max_iterations=args.serve_max_it,
-              epsilon_scaling_steps=args.serve_eps_steps, epsilon_scaling_factor=2.0)
+              epsilon_scaling_steps=args.serve_eps_steps, epsilon_scaling_factor=2.0,
+              preconditioner=args.serve_precond)
 
     srv = sd.SotServer(s, precision="fp32_lse", max_slots=256, hbm_budget_bytes=96 << 30)
     t0 = srv.clock()
@@ -192,29 +196,42 @@ def serving_leg(s, args):
         f"{[done[r.index][0].iterations for r in reqs]}; fp64 relerr max {max(errs):.3e}")
 
     # one-at-a-time FP64 single-solve path
-    end, lat_s, ok_s, its = 0.0, [], 0, []
-    for r in sorted(reqs, key=lambda r: r.arrival):
-        ta = time.perf_counter()
-        s.setup_target(r.y, r.nu)
-        prm = sd.SolverParameters(epsilon=r.epsilon, tau=0.0, tolerance=r.tolerance,
-                                  verbose_output=False,
-                                  solver_control_type="componentwise",
-                                  max_iterations=args.serve_max_it, use_log_sum_exp_trick=True,
-                                  precision="fp64", line_search="noise_aware",
-                                  use_epsilon_scaling=args.serve_eps_steps > 1,
-                                  epsilon_scaling_steps=max(1, args.serve_eps_steps),
-                                  epsilon_scaling_factor=2.0)
-        psi, infos = s.solve_epsilon_scaling(None, prm)
-        dt = time.perf_counter() - ta
-        end = max(end, r.arrival) + dt
-        lat_s.append(end - r.arrival)
-        ok_s += infos[-1]["status"] == "converged"
-        its.append(sum(i["iterations"] for i in infos))
-        log(f"  one-at-a-time: request {r.index} (N={r.nu.size}) {infos[-1]['status']} after "
-            f"{its[-1]} iterations, {dt:.2f}s")
-    span_s = end - min(r.arrival for r in reqs)
-    lat_s = np.array(lat_s)
-    log(f"serving one-at-a-time fp64: {len(reqs)} solves in {span_s:.2f}s; iterations {its}")
+    def one_at_a_time(precond):
+        end, lat, ok, its = 0.0, [], 0, []
+        for r in sorted(reqs, key=lambda r: r.arrival):
+            ta = time.perf_counter()
+            s.setup_target(r.y, r.nu)
+            prm = sd.SolverParameters(epsilon=r.epsilon, tau=0.0, tolerance=r.tolerance,
+                                      verbose_output=False,
+                                      solver_control_type="componentwise",
+                                      max_iterations=args.serve_max_it,
+                                      use_log_sum_exp_trick=True,
+                                      precision="fp64", line_search="noise_aware",
+                                      preconditioner=precond,
+                                      use_epsilon_scaling=args.serve_eps_steps > 1,
+                                      epsilon_scaling_steps=max(1, args.serve_eps_steps),
+                                      epsilon_scaling_factor=2.0)
+            psi, infos = s.solve_epsilon_scaling(None, prm)
+            dt = time.perf_counter() - ta
+            end = max(end, r.arrival) + dt
+            lat.append(end - r.arrival)
+            ok += infos[-1]["status"] == "converged"
+            its.append(sum(i["iterations"] for i in infos))
+            log(f"  one-at-a-time ({precond}): request {r.index} (N={r.nu.size}) "
+                f"{infos[-1]['status']} after {its[-1]} iterations, {dt:.2f}s")
+        span = end - min(r.arrival for r in reqs)
+        lat = np.array(lat)
+        log(f"serving one-at-a-time fp64 ({precond}): {len(reqs)} solves in {span:.2f}s; "
+            f"iterations {its}")
+        return span, {"solves_per_s": len(reqs) / span, "p50_s": float(np.percentile(lat, 50)),
+                      "p99_s": float(np.percentile(lat, 99)), "within_tol": int(ok),
+                      "preconditioner": precond, "iterations": its}
+
+    span_s, single = one_at_a_time(args.serve_precond)
+    extra = {}
+    if args.serve_identity_leg and args.serve_precond != "none":
+        span_i, ident = one_at_a_time("none")
+        extra = {"one_at_a_time_fp64_identity": ident, "speedup_vs_identity": span_i / span_b}
     return {
         "workload": f"first {len(reqs)} requests of the seeded serving trace (seed 4, Poisson 50/s, "
                     "N logU[256,65536], eps {1e-2,3e-3,1e-3}, tol 1e-6 componentwise) on the "
@@ -222,11 +239,12 @@ def serving_leg(s, args):
         "max_iterations": args.serve_max_it, "epsilon_scaling_steps": args.serve_eps_steps,
         "batched_fp32": {"solves_per_s": len(reqs) / span_b, "p50_s": float(np.percentile(lat_b, 50)),
                          "p99_s": float(np.percentile(lat_b, 99)), "within_tol": int(ok_b),
-                         "max_fp64_rel_marginal_err": max(errs)},
-        "one_at_a_time_fp64": {"solves_per_s": len(reqs) / span_s,
-                               "p50_s": float(np.percentile(lat_s, 50)),
-                               "p99_s": float(np.percentile(lat_s, 99)), "within_tol": int(ok_s)},
+                         "max_fp64_rel_marginal_err": max(errs),
+                         "preconditioner": args.serve_precond,
+                         "iterations": [done[r.index][0].iterations for r in reqs]},
+        "one_at_a_time_fp64": single,
         "speedup": span_s / span_b,
+        **extra,
     }
 
 
@@ -303,6 +321,10 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=8,
                     help="requests of the serving trace to replay (0: skip the serving leg)")
     ap.add_argument("--serve-max-it", type=int, default=1000)
+    ap.add_argument("--serve-precond", default="diagonal", choices=["none", "diagonal"],
+                    help="initial inverse Hessian of both serving legs")
+    ap.add_argument("--serve-identity-leg", type=int, default=1,
+                    help="also time the one-at-a-time FP64 path with the identity")
     ap.add_argument("--serve-eps-steps", type=int, default=1,
                     help="epsilon-scaling stages per request (1: none)")
     ap.add_argument("--multilevel", action="store_true",

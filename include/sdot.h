@@ -59,6 +59,15 @@ typedef enum { SDOT_PREC_FP64 = 0, SDOT_PREC_FP32_LSE = 1 } sdot_precision;
  *               at which F differences are below their rounding noise. */
 typedef enum { SDOT_LS_REFERENCE = 0, SDOT_LS_NOISE_AWARE = 1 } sdot_line_search;
 
+/* Initial inverse Hessian of the L-BFGS two-loop recursion.
+ *  NONE:     the identity, as deal.II's SolverBFGS (SotSolver.cc:190-198).
+ *  DIAGONAL: diag(epsilon / nu_j), the inverse of the Hessian's diagonal
+ *            m_j / epsilon at the solution (m = nu there).  Its first step is
+ *            the target-side Sinkhorn update psi_j -= epsilon (m_j / nu_j - 1)
+ *            to first order; it makes the iteration independent of the scale
+ *            of the target weights. */
+typedef enum { SDOT_PRECOND_NONE = 0, SDOT_PRECOND_DIAGONAL = 1 } sdot_preconditioner;
+
 /* SotParameterManager::SolverParameters (ParameterManager.h:56-72), the keys
  * of subsection rsot_solver (ParameterManager.cc:64-95), plus `precision`. */
 typedef struct {
@@ -79,6 +88,7 @@ typedef struct {
   int32_t use_log_sum_exp_trick;  /* 0 */
   int32_t precision;              /* sdot_precision, FP64 */
   int32_t line_search;            /* sdot_line_search, REFERENCE */
+  int32_t preconditioner;         /* sdot_preconditioner, NONE */
 } sdot_params;
 
 typedef struct {
@@ -298,6 +308,7 @@ typedef struct {
    * factor <= 1 is replaced by 2 (EpsilonScalingHandler.cc:18-27) */
   double epsilon_scaling_factor;
   uint32_t epsilon_scaling_steps;
+  int32_t preconditioner;      /* sdot_preconditioner */
 } sdot_request;
 
 typedef struct {

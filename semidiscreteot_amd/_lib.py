@@ -36,7 +36,7 @@ class Params(C.Structure):
                 ("number_of_threads", C.c_uint32), ("use_epsilon_scaling", C.c_int32),
                 ("epsilon_scaling_factor", C.c_double), ("epsilon_scaling_steps", C.c_uint32),
                 ("use_log_sum_exp_trick", C.c_int32), ("precision", C.c_int32),
-                ("line_search", C.c_int32)]
+                ("line_search", C.c_int32), ("preconditioner", C.c_int32)]
 
 
 class SolveInfo(C.Structure):
@@ -69,7 +69,8 @@ class Request(C.Structure):
                 ("epsilon", C.c_double), ("tolerance", C.c_double),
                 ("solver_control_type", C.c_int32), ("max_iterations", C.c_uint32),
                 ("radius", C.c_double), ("line_search", C.c_int32),
-                ("epsilon_scaling_factor", C.c_double), ("epsilon_scaling_steps", C.c_uint32)]
+                ("epsilon_scaling_factor", C.c_double), ("epsilon_scaling_steps", C.c_uint32),
+                ("preconditioner", C.c_int32)]
 
 
 class Result(C.Structure):

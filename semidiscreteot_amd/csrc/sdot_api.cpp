@@ -635,6 +635,8 @@ extern "C" int sdot_set_params(sdot_ctx *c, const sdot_params *p) {
     return fail(SDOT_ERR_INVALID, "unknown precision");
   if (p->line_search != SDOT_LS_REFERENCE && p->line_search != SDOT_LS_NOISE_AWARE)
     return fail(SDOT_ERR_INVALID, "unknown line_search");
+  if (p->preconditioner != SDOT_PRECOND_NONE && p->preconditioner != SDOT_PRECOND_DIAGONAL)
+    return fail(SDOT_ERR_INVALID, "unknown preconditioner");
   c->params = *p;
   c->hist_valid = false;
   return SDOT_OK;
@@ -1327,6 +1329,18 @@ extern "C" int sdot_solve(sdot_ctx *c, double *psi, sdot_solve_info *info) {
   };
   const bool noise_aware = c->params.line_search == SDOT_LS_NOISE_AWARE;
   const bool low = c->params.use_log_sum_exp_trick && c->params.precision == SDOT_PREC_FP32_LSE;
+  // initial inverse Hessian diag(eps / nu_j) (identity: the reference's)
+  std::vector<double> h0v;
+  if (c->params.preconditioner == SDOT_PRECOND_DIAGONAL) {
+    // weights <= 0 (their targets receive no mass) take the mean weight's scale
+    double mean = 0.0;
+    for (int64_t j = 0; j < N; ++j) mean += std::fabs(c->h_nu[j]);
+    mean = mean > 0.0 ? mean / (double)N : 1.0;
+    h0v.resize(N);
+    for (int64_t j = 0; j < N; ++j)
+      h0v[j] = c->params.epsilon / (c->h_nu[j] > 0.0 ? c->h_nu[j] : mean);
+  }
+  const double *h0 = h0v.empty() ? nullptr : h0v.data();
   BfgsResult r;
   int its = 0, evals = 0;
   if (low) {
@@ -1341,7 +1355,7 @@ extern "C" int sdot_solve(sdot_ctx *c, double *psi, sdot_solve_info *info) {
       if ((unsigned)step >= max_it || std::isnan(cv)) return kFailure;
       return kIterate;
     };
-    r = lbfgs_solve(N, psi, eval, check_low, 5, noise_aware ? kNoiseRelF32 : 0.0);
+    r = lbfgs_solve(N, psi, eval, check_low, 5, noise_aware ? kNoiseRelF32 : 0.0, h0);
     if (eval_rc) return eval_rc;
     its = r.iterations;
     evals = r.evaluations;
@@ -1357,14 +1371,14 @@ extern "C" int sdot_solve(sdot_ctx *c, double *psi, sdot_solve_info *info) {
       };
       c->force_mode = kF64Lse;
       c->hist_valid = false;
-      r = lbfgs_solve(N, psi, eval, check_hi, 5, noise_aware ? kNoiseRelF64 : 0.0);
+      r = lbfgs_solve(N, psi, eval, check_hi, 5, noise_aware ? kNoiseRelF64 : 0.0, h0);
       c->force_mode = -1;
       if (eval_rc) return eval_rc;
       its += r.iterations;
       evals += r.evaluations;
     }
   } else {
-    r = lbfgs_solve(N, psi, eval, check, 5, noise_aware ? kNoiseRelF64 : 0.0);
+    r = lbfgs_solve(N, psi, eval, check, 5, noise_aware ? kNoiseRelF64 : 0.0, h0);
     if (eval_rc) return eval_rc;
     its = r.iterations;
     evals = r.evaluations;

@@ -150,9 +150,11 @@ inline double first_step(bool first, bool surrogate, double f_prev, double f, do
 
 // noise_rel > 0 selects the noise-aware line search with noise floor
 // noise_rel |f|; 0 is the reference algorithm.
+// h0: diagonal of the initial inverse Hessian of the two-loop recursion
+// (nullptr: the identity, as deal.II's SolverBFGS).
 template <class Eval, class Check>
 BfgsResult lbfgs_solve(int64_t n, double *x, Eval &&eval, Check &&check, int history = 5,
-                       double noise_rel = 0.0) {
+                       double noise_rel = 0.0, const double *h0 = nullptr) {
   BfgsResult res;
   std::vector<double> g(n), p(n), sk(n), yk(n), x0(n);
   std::vector<std::vector<double>> S(history, std::vector<double>(n)),
@@ -169,7 +171,7 @@ BfgsResult lbfgs_solve(int64_t n, double *x, Eval &&eval, Check &&check, int his
   bool first = true, sur = false;
   double f_prev = 0.0;
   while (st == kIterate) {
-    // two-loop recursion, H0 = identity
+    // two-loop recursion, H0 = identity or diag(h0)
     std::memcpy(p.data(), g.data(), sizeof(double) * n);
     for (int i = 0; i < count; ++i) {
       const int s = slot(i);
@@ -178,6 +180,8 @@ BfgsResult lbfgs_solve(int64_t n, double *x, Eval &&eval, Check &&check, int his
       const double *ys = Y[s].data();
       for (int64_t t = 0; t < n; ++t) p[t] += c * ys[t];
     }
+    if (h0)
+      for (int64_t t = 0; t < n; ++t) p[t] *= h0[t];
     for (int i = count - 1; i >= 0; --i) {
       const int s = slot(i);
       const double c2 = rho[s] * bfgs_dot(Y[s].data(), p.data(), n);

@@ -7,8 +7,8 @@
 // reference's reading: solver_control_type is componentwise only for
 // "componentwise" (SotSolver.cc:156), distance_threshold_type falls back to
 // pointwise for anything but "integral"/"geometric" (SotSolver.cc:392-406).
-// `precision` (fp64 | fp32_lse) and `line_search` (reference | noise_aware)
-// are this library's own keys.
+// `precision` (fp64 | fp32_lse), `line_search` (reference | noise_aware) and
+// `preconditioner` (none | diagonal) are this library's own keys.
 #include <cctype>
 #include <cstdio>
 #include <cstdlib>
@@ -75,6 +75,7 @@ extern "C" int sdot_params_default(sdot_params *p) {
   p->use_log_sum_exp_trick = 0;
   p->precision = SDOT_PREC_FP64;
   p->line_search = SDOT_LS_REFERENCE;
+  p->preconditioner = SDOT_PRECOND_NONE;
   return SDOT_OK;
 }
 
@@ -140,6 +141,11 @@ extern "C" int sdot_params_parse_prm(const char *text, sdot_params *p) {
       const std::string l = lower(val);
       if (l == "reference") p->line_search = SDOT_LS_REFERENCE;
       else if (l == "noise_aware") p->line_search = SDOT_LS_NOISE_AWARE;
+      else ok = false;
+    } else if (key == "preconditioner") {
+      const std::string l = lower(val);
+      if (l == "none") p->preconditioner = SDOT_PRECOND_NONE;
+      else if (l == "diagonal") p->preconditioner = SDOT_PRECOND_DIAGONAL;
       else ok = false;
     } else
       ok = false;  // deal.II rejects undeclared entries

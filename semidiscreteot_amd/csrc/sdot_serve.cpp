@@ -170,6 +170,7 @@ struct Request {
   std::vector<double> eps_seq;  // epsilon schedule, largest first (last = eps)
   int control = SDOT_CTRL_L1NORM;
   int line_search = SDOT_LS_REFERENCE;
+  int precond = SDOT_PRECOND_NONE;
   uint32_t max_it = 0;
   std::vector<double> y, nu, psi0;
   TargetLayout L;
@@ -483,7 +484,12 @@ int upload_batch(sdot_server *s) {
     s->dirty = false;
   }
   auto *in = reinterpret_cast<SlotIn *>(h + s->off_in);
-  for (int i = 0; i < n; ++i) in[i] = s->active[i]->drv.next_in();
+  for (int i = 0; i < n; ++i) {
+    const Solve &v = *s->active[i];
+    in[i] = v.drv.next_in();
+    // diag(eps / nu_j) at the epsilon of the solve's current stage
+    in[i].h0 = v.req->precond == SDOT_PRECOND_DIAGONAL ? v.req->eps_seq[v.stage] : 0.0;
+  }
   *reinterpret_cast<int *>(h + s->off_nact) = n;
   SV_TRY(hipMemcpyAsync(s->d_block + s->off_in, h + s->off_in, s->block_bytes - s->off_in,
                         hipMemcpyHostToDevice, s->stream));
@@ -704,6 +710,9 @@ extern "C" int sdot_server_submit(sdot_server *s, const sdot_request *q, int64_t
   if (q->line_search != SDOT_LS_REFERENCE && q->line_search != SDOT_LS_NOISE_AWARE)
     return set_error(SDOT_ERR_INVALID, "unknown line_search");
   r->line_search = q->line_search;
+  if (q->preconditioner != SDOT_PRECOND_NONE && q->preconditioner != SDOT_PRECOND_DIAGONAL)
+    return set_error(SDOT_ERR_INVALID, "unknown preconditioner");
+  r->precond = q->preconditioner;
   r->R = (q->radius <= 0.0 || q->radius >= DBL_MAX || std::isnan(q->radius)) ? INFINITY : q->radius;
   // epsilon schedule of EpsilonScalingHandler::generate_epsilon_sequence
   // (EpsilonScalingHandler.cc:30-41; factor <= 1 -> 2, steps 0 -> 1 as its

@@ -31,10 +31,11 @@ enum : int { kActNone = 0, kActIter = 1, kActTrial = 2 };
 
 // Written by the host before a step.  The noise-aware line search is on for
 // the new direction when surrogate != 0 or |g0| < noise (first trial step 1);
-// noise = 0 and surrogate = 0 select the reference search.
+// noise = 0 and surrogate = 0 select the reference search.  h0 > 0: initial
+// inverse Hessian diag(h0 / nu_j) of the two-loop recursion (0: identity).
 struct SlotIn {
   int32_t action, first, has_prev, surrogate;
-  double alpha, f, f_prev, tol, noise;
+  double alpha, f, f_prev, tol, noise, h0;
 };
 
 // Written by the device during a step (F and C first: the final reduction

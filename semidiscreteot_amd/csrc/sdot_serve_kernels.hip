@@ -224,6 +224,10 @@ __global__ __launch_bounds__(kBT) void kl_action(const SlotVecs *__restrict__ ve
       for (int64_t t = threadIdx.x; t < N; t += kBT) v.p[t] += -c1[i] * v.Y[sl][t];
       __syncthreads();
     }
+    if (c.h0 > 0.0) {
+      for (int64_t t = threadIdx.x; t < N; t += kBT) v.p[t] *= c.h0 / v.nu[t];
+      __syncthreads();
+    }
     for (int i = count - 1; i >= 0; --i) {
       const int sl = ((newest - i) % kHist + kHist) % kHist;
       const double c2 = h.rho[sl] * bdot(v.Y[sl], v.p, N, sh);

@@ -14,7 +14,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib as L
-from .solver import SotSolver, _f64, _p
+from .solver import _PC, SotSolver, _f64, _p
 
 _PREC = {"fp64": 0, "fp32_lse": 1}
 _CTRL = {"l1norm": 0, "componentwise": 1}
@@ -85,7 +85,7 @@ class SotServer:
                control: str = "componentwise", max_iterations: int = 1000,
                radius: float | None = None, psi0=None,
                line_search: str = "noise_aware", epsilon_scaling_steps: int = 0,
-               epsilon_scaling_factor: float = 2.0) -> int:
+               epsilon_scaling_factor: float = 2.0, preconditioner: str = "none") -> int:
         y, nu = _f64(y), _f64(nu)
         if y.size != nu.size * self.dim:
             raise ValueError(f"Dimension mismatch: {y.size // self.dim} points, {nu.size} weights")
@@ -106,6 +106,7 @@ class SotServer:
         q.line_search = _LS[line_search]
         q.epsilon_scaling_steps = int(epsilon_scaling_steps)
         q.epsilon_scaling_factor = float(epsilon_scaling_factor)
+        q.preconditioner = _PC[preconditioner]
         rid = C.c_int64(-1)
         L.check(self.lib.sdot_server_submit(self._h, C.byref(q), C.byref(rid)))
         self._n[rid.value] = nu.size

@@ -19,6 +19,7 @@ _THRESH = {"pointwise": 0, "integral": 1, "geometric": 2}
 _CTRL = {"l1norm": 0, "componentwise": 1}
 _PREC = {"fp64": 0, "fp32_lse": 1}
 _LS = {"reference": 0, "noise_aware": 1}
+_PC = {"none": 0, "diagonal": 1}
 
 
 def _f64(a):
@@ -32,8 +33,9 @@ def _p(a):
 @dataclass
 class SolverParameters:
     """SotParameterManager::SolverParameters (ParameterManager.h:56-72) plus
-    `precision` ("fp64" | "fp32_lse") and `line_search` ("reference" |
-    "noise_aware", include/sdot.h)."""
+    `precision` ("fp64" | "fp32_lse"), `line_search` ("reference" |
+    "noise_aware") and `preconditioner` ("none" | "diagonal": initial inverse
+    Hessian diag(epsilon / nu_j) of the L-BFGS recursion), include/sdot.h."""
     max_iterations: int = 1000
     tolerance: float = 1.0
     solver_control_type: str = "l1norm"
@@ -51,6 +53,7 @@ class SolverParameters:
     use_log_sum_exp_trick: bool = False
     precision: str = "fp64"
     line_search: str = "reference"
+    preconditioner: str = "none"
 
     def to_c(self) -> L.Params:
         p = L.Params()
@@ -71,6 +74,7 @@ class SolverParameters:
         p.use_log_sum_exp_trick = int(bool(self.use_log_sum_exp_trick))
         p.precision = _PREC[self.precision]
         p.line_search = _LS[self.line_search]
+        p.preconditioner = _PC[self.preconditioner]
         return p
 
     @classmethod
@@ -78,6 +82,7 @@ class SolverParameters:
         inv_t = {v: k for k, v in _THRESH.items()}
         inv_p = {v: k for k, v in _PREC.items()}
         inv_l = {v: k for k, v in _LS.items()}
+        inv_c = {v: k for k, v in _PC.items()}
         return cls(max_iterations=p.max_iterations, tolerance=p.tolerance,
                    solver_control_type="componentwise" if p.solver_control_type == 1 else "l1norm",
                    epsilon=p.epsilon, tau=p.tau,
@@ -88,7 +93,8 @@ class SolverParameters:
                    epsilon_scaling_factor=p.epsilon_scaling_factor,
                    epsilon_scaling_steps=p.epsilon_scaling_steps,
                    use_log_sum_exp_trick=bool(p.use_log_sum_exp_trick),
-                   precision=inv_p[p.precision], line_search=inv_l[p.line_search])
+                   precision=inv_p[p.precision], line_search=inv_l[p.line_search],
+                   preconditioner=inv_c[p.preconditioner])
 
     @classmethod
     def from_prm(cls, text: str | None = None, path: str | None = None) -> "SolverParameters":

@@ -4,7 +4,9 @@
 // loop form computes them) and compares with lbfgs_solve on Rosenbrock:
 // iterates, iteration and evaluation counts must be identical.  A second run
 // flags every 5th evaluation as "lost" (repeated unseeded) and must give the
-// same result.
+// same result.  With a diagonal initial inverse Hessian the two forms agree
+// the same way, and on a separable quadratic whose Hessian it inverts the
+// first step reaches the minimiser.
 #include <cmath>
 #include <cstdio>
 #include <vector>
@@ -64,6 +66,8 @@ struct Emu {
       c1[i] = rho[sl] * dot(S[sl], p);
       for (int64_t t = 0; t < n; ++t) p[t] += -c1[i] * Y[sl][t];
     }
+    if (c.h0 > 0.0)
+      for (int64_t t = 0; t < n; ++t) p[t] *= c.h0 / nu[t];
     for (int i = count - 1; i >= 0; --i) {
       const int sl = ((newest - i) % kHist + kHist) % kHist;
       const double cc = c1[i] - rho[sl] * dot(Y[sl], p);
@@ -95,15 +99,18 @@ struct Emu {
 };
 
 static int served(int64_t n, double tol, uint32_t max_it, int lose_every, std::vector<double> &xout,
-                  int &it, int &ev, double noise_rel = 0.0) {
+                  int &it, int &ev, double noise_rel = 0.0, double h0 = 0.0,
+                  const std::vector<double> *nu = nullptr) {
   Emu e(n);
   for (auto &v : e.x) v = -1.0;
   e.x[1] = 1.2;
+  if (nu) e.nu = *nu;
   SolveDriver d;
   d.init(0, tol, max_it, noise_rel);
   int status = -1, raw = 0;
   while (status < 0) {
-    const SlotIn c = d.next_in();
+    SlotIn c = d.next_in();
+    c.h0 = h0;
     SlotOut o{};
     e.action(c, o);
     e.evaluate(o, tol);
@@ -141,6 +148,41 @@ int main() {
     std::printf("loop: it=%d ev=%d st=%d  served(lose every %d): it=%d ev=%d st=%d  same=%d\n",
                 ra.iterations, ra.evaluations, ra.status, lose, it, ev, st, (int)same);
     ok = ok && same;
+  }
+  // diagonal initial inverse Hessian: the driver with h0 / nu_j reproduces
+  // lbfgs_solve with h0[j] = h0 / nu_j; on a separable quadratic whose Hessian
+  // is diag(nu_j / h0) the first step (trial step 1) lands on the minimiser
+  {
+    std::vector<double> nu(n), h(n);
+    for (int64_t i = 0; i < n; ++i) {
+      nu[i] = 0.5 + 0.37 * i;
+      h[i] = 0.25 / nu[i];
+    }
+    std::vector<double> xp(n, -1.0);
+    xp[1] = 1.2;
+    BfgsResult rp = lbfgs_solve(n, xp.data(), eval, check, 5, 0.0, h.data());
+    std::vector<double> xs;
+    int it = 0, ev = 0;
+    const int st = served(n, tol, max_it, 0, xs, it, ev, 0.0, 0.25, &nu);
+    const bool same = xs == xp && it == rp.iterations && ev == rp.evaluations && st == rp.status;
+    auto quad = [&](const double *x, double *g) {
+      double f = 0;
+      for (int64_t i = 0; i < n; ++i) {
+        const double d = x[i] - (0.1 * i - 0.3);
+        g[i] = nu[i] / 0.25 * d;
+        f += 0.5 * nu[i] / 0.25 * d * d;
+      }
+      return f;
+    };
+    std::vector<double> xq(n, 2.0);
+    BfgsResult rq = lbfgs_solve(n, xq.data(), quad, check, 5, 0.0, h.data());
+    double err = 0;
+    for (int64_t i = 0; i < n; ++i) err = std::fmax(err, std::fabs(xq[i] - (0.1 * i - 0.3)));
+    std::printf("diagonal H0: loop it=%d ev=%d st=%d  served it=%d ev=%d st=%d  same=%d; "
+                "quadratic it=%d ev=%d err=%.1e\n", rp.iterations, rp.evaluations, rp.status,
+                it, ev, st, (int)same, rq.iterations, rq.evaluations, err);
+    ok = ok && same && rp.status == kConverged && rq.status == kConverged &&
+         rq.iterations == 1 && rq.evaluations == 2 && err <= 1e-15;
   }
   // noise-aware line search on Rosenbrock + 1000 with a deterministic
   // rounding-like perturbation of 1e-7 on F (gradient exact): the reference

@@ -67,6 +67,7 @@ subsection SotParameterManager
     set epsilon_scaling_steps = 3
     set precision = fp32_lse
     set line_search = noise_aware
+    set preconditioner = diagonal
   end
   subsection power_diagram_parameters
     set implementation = geogram
@@ -82,7 +83,8 @@ def test_prm_reader():
     assert p.verbose_output is False and p.quadrature_order == 2
     assert p.use_log_sum_exp_trick is True and p.epsilon_scaling_factor == 10.0
     assert p.epsilon_scaling_steps == 3 and p.precision == "fp32_lse"
-    assert p.line_search == "noise_aware"
+    assert p.line_search == "noise_aware" and p.preconditioner == "diagonal"
+    assert sd.SolverParameters.from_prm("").preconditioner == "none"
 
 
 def test_prm_reader_errors():
@@ -92,6 +94,8 @@ def test_prm_reader_errors():
         sd.SolverParameters.from_prm("subsection rsot_solver\n set epsilon = abc\nend\n")
     with pytest.raises(sd.SdotError):
         sd.SolverParameters.from_prm("subsection rsot_solver\n set epsilon = 1\n")
+    with pytest.raises(sd.SdotError):
+        sd.SolverParameters.from_prm("subsection rsot_solver\n set preconditioner = jacobi\nend\n")
     # unknown threshold names fall back to pointwise as in SotSolver.cc:406
     p = sd.SolverParameters.from_prm("subsection rsot_solver\n set distance_threshold_type = foo\nend\n")
     assert p.distance_threshold_type == "pointwise"

@@ -262,3 +262,50 @@ def test_server_epsilon_scaling_matches_run_sot_loop():
     assert abs(it1 - res.iterations) <= 3 * steps
     assert np.max(np.abs((psi1 - psi1.mean()) - (psi2 - psi2.mean()))) < 1e-6
     assert true_relerr(src, y, nu, eps, psi2) <= tol * 1.001
+
+
+def test_diagonal_preconditioner_single_and_served():
+    """Initial inverse Hessian diag(eps / nu_j): the served FP64 solve takes the
+    decisions of sdot_solve step for step (20 iterations, potentials to
+    rounding); solved to 1e-6 (noise-aware search) it converges in fewer
+    iterations than with the identity, both to the tolerance on the FP64
+    oracle's marginals and to the same potential; an FP32 served solve reaches
+    the tolerance on the oracle's marginals too."""
+    say("diagonal preconditioner")
+    src, s = source()
+    y, nu = request(12, 400, dirichlet=2.0)
+    eps, tol = 3e-3, 1e-6
+    s.setup_target(y, nu)
+    base = dict(epsilon=eps, tau=0.0, tolerance=tol, solver_control_type="componentwise",
+                use_log_sum_exp_trick=True, preconditioner="diagonal")
+    s.configure(sd.SolverParameters(max_iterations=20, **base))
+    psi1 = s.solve()
+    info = s.get_last_solve_info()
+    with sd.SotServer(s, precision="fp64", max_slots=8, hbm_budget_bytes=1 << 30) as srv:
+        rid = srv.submit(y, nu, eps, tol, "componentwise", 20, line_search="reference",
+                         preconditioner="diagonal")
+        res, psi2 = srv.run([rid])[rid]
+    assert (info["iterations"], info["evaluations"]) == (res.iterations, res.evaluations)
+    assert np.max(np.abs(psi1 - psi2)) < 1e-12
+
+    its, sol = {}, {}
+    for pc in ("none", "diagonal"):
+        s.configure(sd.SolverParameters(max_iterations=2000, line_search="noise_aware",
+                                        **{**base, "preconditioner": pc}))
+        sol[pc] = s.solve()
+        inf = s.get_last_solve_info()
+        its[pc] = inf["iterations"]
+        assert inf["status"] == "converged"
+        assert true_relerr(src, y, nu, eps, sol[pc]) <= tol * 1.001
+    a, b = sol["none"], sol["diagonal"]
+    dpsi = np.max(np.abs((a - a.mean()) - (b - b.mean())))
+    say(f"iterations: identity {its['none']}, diagonal {its['diagonal']}; |dpsi| {dpsi:.2e}")
+    assert its["diagonal"] < its["none"]
+    assert dpsi < 1e-5
+
+    with sd.SotServer(s, precision="fp32_lse", max_slots=8, hbm_budget_bytes=1 << 30) as srv:
+        rid = srv.submit(y, nu, eps, tol, "componentwise", 2000, preconditioner="diagonal")
+        res, psi3 = srv.run([rid])[rid]
+    err = true_relerr(src, y, nu, eps, psi3)
+    say(f"fp32 served diagonal: {res.status} it={res.iterations} fp64 relerr {err:.3e}")
+    assert res.status == "converged" and err <= tol * 1.001
