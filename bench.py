@@ -308,6 +308,106 @@ def multilevel_leg(args):
                         "threshold": r.threshold} for r in recs]}
 
 
+def tets_leg(args):
+    """Large unstructured workload (Delaunay tetrahedra of 200,000 points in
+    the unit ball, P1 density 1 + 0.5 sin(4 pi x0), 4-point rule, 500,000
+    uniform targets, eps = 5e-4, pointwise truncation with tau = 1e-12):
+    an FP64 and an FP32_LSE solve from 0 (componentwise 1e-6, noise-aware
+    search, diagonal initial inverse Hessian; the FP32 solve finishes on FP64
+    evaluations), timed evaluations of both precisions at the FP64 solution,
+    and the FP32-vs-FP64 differences of the functional, the gradient (relative
+    to nu), C_global, the transport cost and the solved potential."""
+    import semidiscreteot_amd as sd
+    from semidiscreteot_amd import devmem, workloads
+    t0 = time.perf_counter()
+    wl = workloads.ball_tets()
+    n_q = wl.source.x.size // 3
+    log(f"tets workload {wl.name}: {wl.source.n_cells} cells, {n_q} q, {wl.nu.size} targets "
+        f"({time.perf_counter() - t0:.1f}s)")
+    s = sd.SotSolver(3)
+    s.setup_source(wl.source.x, wl.source.w, wl.source.centers, wl.source.nq)
+    s.setup_target(wl.y, wl.nu)
+    base = dict(epsilon=wl.epsilon, tau=wl.tau, tolerance=1e-6,
+                solver_control_type="componentwise", max_iterations=args.tets_max_it,
+                use_log_sum_exp_trick=True, line_search="noise_aware",
+                preconditioner="diagonal", verbose_output=False)
+    sol = {}
+    for prec in ("fp64", "fp32_lse"):
+        s.configure(sd.SolverParameters(precision=prec, **{**base, "verbose_output": True}))
+        ta = time.perf_counter()
+        # the solve's iteration lines go to stderr (stdout carries the JSON line)
+        saved = os.dup(1)
+        sys.stdout.flush()
+        os.dup2(2, 1)
+        try:
+            psi = s.solve()
+        finally:
+            sys.stdout.flush()
+            os.dup2(saved, 1)
+            os.close(saved)
+        info = s.get_last_solve_info()
+        sol[prec] = (psi, info, time.perf_counter() - ta)
+        log(f"tets solve {prec}: {info['status']} after {info['iterations']} iterations "
+            f"({info['evaluations']} evaluations) in {sol[prec][2]:.1f}s")
+    psi64 = sol["fp64"][0]
+    s.configure(sd.SolverParameters(precision="fp64", **base))
+    R = s.distance_threshold(psi64)
+    _, _, pairs = s.candidate_stats(R)
+    x = wl.source.x.reshape(-1, 3)
+    y = wl.y.reshape(-1, 3)
+    sx2 = float(np.sum(wl.source.w * 0.5 * np.sum(x * x, axis=1)))
+    vals = {}
+    d_psi = devmem.DeviceArray.from_host(psi64)
+    d_grad = devmem.DeviceArray(wl.nu.size)
+    d_fc = devmem.DeviceArray(4)
+    for prec in ("fp64", "fp32_lse"):
+        s.configure(sd.SolverParameters(precision=prec, **base))
+        F, g, C = s.evaluate_functional_R(R, psi64)
+        b = s.compute_weighted_barycenters_R(R, psi64)
+        m = g + wl.nu
+        # <c, pi> = sum_q w_q |x_q|^2/2 - sum_j nu_j y_j . b_j + sum_j m_j |y_j|^2/2
+        cost = sx2 - float(np.sum(wl.nu[:, None] * y * b)) + float(np.sum(m * 0.5 * np.sum(y * y, 1)))
+        s.eval_device(R, d_psi.address, d_grad.address, d_fc.address)
+        s.synchronize()
+        ta = time.perf_counter()
+        for _ in range(args.tets_steps):
+            s.eval_device(R, d_psi.address, d_grad.address, d_fc.address)
+        s.synchronize()
+        ms = (time.perf_counter() - ta) * 1e3 / args.tets_steps
+        vals[prec] = (F, g, C, cost, ms)
+        log(f"tets eval {prec}: {ms:.2f} ms, {pairs / (ms * 1e-3):.3e} pair-evals/s, F={F:.15e} "
+            f"C={C:.10e} cost={cost:.10e}")
+    F64, g64, C64, c64, ms64 = vals["fp64"]
+    F32, g32, C32, c32, ms32 = vals["fp32_lse"]
+    psi32 = sol["fp32_lse"][0]
+    a, b = psi64 - psi64.mean(), psi32 - psi32.mean()
+    s.configure(sd.SolverParameters(precision="fp64", **base))
+    _, g_at32, _ = s.evaluate_functional_R(R, psi32)
+    dm = s.discarded_mass(R, psi64, wl.source.w)
+    out = {
+        "workload": wl.name, "n_q": n_q, "n_targets": int(wl.nu.size), "epsilon": wl.epsilon,
+        "R": R, "discarded_mass": dm, "pairs_per_eval": int(pairs),
+        "fp64_eval_ms": ms64, "fp32_eval_ms": ms32,
+        "fp64_pair_evals_per_s": pairs / (ms64 * 1e-3), "fp32_pair_evals_per_s": pairs / (ms32 * 1e-3),
+        "solve": {p: {"status": sol[p][1]["status"], "iterations": sol[p][1]["iterations"],
+                      "evaluations": sol[p][1]["evaluations"], "seconds": sol[p][2]}
+                  for p in sol},
+        "fp32_vs_fp64": {
+            "functional_rel": abs(F32 - F64) / abs(F64),
+            "gradient_max_rel_to_nu": float(np.max(np.abs(g32 - g64) / wl.nu)),
+            "C_global_rel": abs(C32 - C64) / abs(C64),
+            "transport_cost_rel": abs(c32 - c64) / abs(c64),
+            "psi_max_abs": float(np.max(np.abs(a - b))),
+            "psi_max_abs_rel_to_range": float(np.max(np.abs(a - b)) / (a.max() - a.min())),
+            "fp64_rel_marginal_err_of_fp32_solution": float(np.max(np.abs(g_at32) / wl.nu)),
+            "fp64_rel_marginal_err_of_fp64_solution": float(np.max(np.abs(g64) / wl.nu))},
+        "transport_cost_fp64": c64,
+    }
+    log(f"tets: {json.dumps(out['fp32_vs_fp64'])}")
+    s.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -331,6 +431,10 @@ def main():
                     help="also run the combined multilevel workload end to end")
     ap.add_argument("--ml-max-it", type=int, default=3000,
                     help="iteration budget of each multilevel solve")
+    ap.add_argument("--tets", action="store_true",
+                    help="also run the large unstructured (Delaunay ball) workload")
+    ap.add_argument("--tets-max-it", type=int, default=2000)
+    ap.add_argument("--tets-steps", type=int, default=5)
     ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_BYTES"),
                     help="summarise two rocprofv3 --pmc runs into per-kernel HBM traffic and exit")
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -436,6 +540,10 @@ def main():
     if args.multilevel:
         log("multilevel leg ...")
         multilevel = multilevel_leg(args)
+    tets = None
+    if args.tets:
+        log("tets leg ...")
+        tets = tets_leg(args)
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -472,6 +580,7 @@ def main():
         "kernel_ms": kern,
         "serving": serving,
         "multilevel": multilevel,
+        "large_unstructured": tets,
         "cpu_baseline": cpu,
     }
     if args.copy_bytes > 0:

@@ -356,6 +356,22 @@ int sdot_mesh_simplex(int dim, int64_t n_vertices, const double *vertices, int64
                       const int64_t *cells, int qorder, const double *nodal_rho,
                       int normalize, double *x, double *w, double *centers);
 
+/* General unstructured cells, as GridIn::read_msh / read_vtk meshes reach
+ * setup_source (MeshManager.templates.h:80-140): kind SDOT_CELL_Q1 (FE_Q(1),
+ * MappingQ1, QGauss(qorder); 2^cell_dim vertices per cell in deal.II's
+ * lexicographic order) or SDOT_CELL_SIMPLEX (FE_SimplexP(1),
+ * QGaussSimplex(qorder), orders 1-2; cell_dim + 1 vertices), cells of
+ * dimension cell_dim in a space of dimension spacedim >= cell_dim (surfaces
+ * in 3D: SotSolver<2, 3>; JxW is then the Gram measure).  vertices
+ * [n_vertices*spacedim], cells [n_cells*vertices per cell], nodal_rho
+ * [n_vertices] or NULL; x [n_cells*nq*spacedim], w [n_cells*nq], centers
+ * [n_cells*spacedim] out (nq = qorder^cell_dim for Q1). */
+typedef enum { SDOT_CELL_SIMPLEX = 0, SDOT_CELL_Q1 = 1 } sdot_cell_kind;
+int sdot_mesh_cells(int kind, int cell_dim, int spacedim, int64_t n_vertices,
+                    const double *vertices, int64_t n_cells, const int64_t *cells, int qorder,
+                    const double *nodal_rho, int normalize, double *x, double *w,
+                    double *centers);
+
 #ifdef __cplusplus
 }
 #endif

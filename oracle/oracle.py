@@ -71,6 +71,8 @@ def lib():
                                             _dp, _dp, _dp, _dp, _dp]
         L.orc_simplex_source.argtypes = [C.c_int, C.c_int64, _dp, C.c_int64, _i64p, C.c_int,
                                          _dp, _dp, _dp, _dp, _dp]
+        L.orc_cells_source.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int64, _dp, C.c_int64,
+                                       _i64p, C.c_int, _dp, _dp, _dp, _dp, _dp]
         L.orc_normalize_density.argtypes = [C.c_int64, _dp, _dp]
         L.orc_normalize_density.restype = C.c_double
         L.orc_covering_radius.argtypes = [C.POINTER(_Source), C.POINTER(_Target)]
@@ -200,6 +202,35 @@ def simplex_source(dim, vertices, cells, qorder, nodal_rho=None, normalize=True)
         rho_n = rho_n / L
         x, rho, jxw, c = run(rho_n)
     return Source(dim, nq, x, rho, jxw, c)
+
+
+def cells_source(q1, cdim, sdim, vertices, cells, qorder, nodal_rho=None, normalize=True):
+    """General Q1 (q1=True) or P1 cells of dimension cdim in dimension sdim;
+    the Source has dim = sdim."""
+    vertices = _f64(vertices)
+    cells = np.ascontiguousarray(cells, dtype=np.int64)
+    nv = 2 ** cdim if q1 else cdim + 1
+    n_cells = cells.size // nv
+    nq = qorder ** cdim if q1 else {1: 1, 2: cdim + 1}[qorder]
+    rho_n = np.ones(vertices.size // sdim) if nodal_rho is None else _f64(nodal_rho).copy()
+
+    def run(rn):
+        x = np.zeros(n_cells * nq * sdim)
+        rho = np.zeros(n_cells * nq)
+        jxw = np.zeros(n_cells * nq)
+        c = np.zeros(n_cells * sdim)
+        rc = lib().orc_cells_source(int(bool(q1)), cdim, sdim, vertices.size // sdim,
+                                    _p(vertices), n_cells, cells.ctypes.data_as(_i64p), qorder,
+                                    _p(rn), _p(x), _p(rho), _p(jxw), _p(c))
+        assert rc == nq
+        return x, rho, jxw, c
+
+    x, rho, jxw, c = run(rho_n)
+    if normalize:
+        L = lib().orc_normalize_density(rho.size, _p(rho), _p(jxw))
+        rho_n = rho_n / L
+        x, rho, jxw, c = run(rho_n)
+    return Source(sdim, nq, x, rho, jxw, c)
 
 
 def params(epsilon=1e-3, tau=1e-8, tolerance=1.0, use_lse=False, threshold="pointwise",

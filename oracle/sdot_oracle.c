@@ -13,6 +13,12 @@
  *   orc_structured_source     closed-form integrals of polynomials over the
  *                             box; sum of JxW = volume.
  *   orc_simplex_source        Keast 4-point rule closed form; tet volumes.
+ *   orc_cells_source          affine Q1 boxes equal orc_structured_source;
+ *                             volumes and moments of trilinear hexes with
+ *                             a raised vertex (closed forms); areas of tilted
+ *                             and of bilinear (saddle) surface quads in 3D
+ *                             (closed form, 50-digit quadrature); tets equal
+ *                             orc_simplex_source; surface triangle areas.
  *   orc_evaluate(_R)          N=1 closed form, quadrant-symmetric g(0)=0,
  *                             sum(g)=mass-1, F(psi+c)=F(psi), g = FD of F,
  *                             50-digit brute force on tiny inputs
@@ -259,6 +265,140 @@ int orc_simplex_source(int dim, int64_t n_vertices, const double *vertices,
 
 /* L1 norm used by normalize_density (SemiDiscreteOT.cc:422-449): the caller
  * divides the nodal density by it. */
+/* General cells as GridIn::read_msh / read_vtk meshes reach setup_source
+ * (MeshManager.templates.h:80-140), with the FEValues values of
+ * SotSolver.cc:788-790: q1 != 0: FE_Q(1) / MappingQ1 / QGauss(qorder) on
+ * 2^cdim vertices in lexicographic order; q1 == 0: FE_SimplexP(1) /
+ * QGaussSimplex(qorder) on cdim + 1 vertices.  Cells of dimension cdim in a
+ * space of dimension sdim (surfaces in 3D: SotSolver<2,3>).
+ * The Jacobian of the multilinear map is taken column by column as the
+ * difference of the two opposite faces' interpolants (exact for a map linear
+ * in each reference coordinate); the measure is |det J| for cdim == sdim and
+ * the cross-product length |J_0 x J_1| for a surface in 3D.  Returns nq. */
+static void q1_point(int cdim, int sdim, double v[8][3], const double *xi, double *out) {
+  for (int d = 0; d < sdim; ++d) out[d] = 0.0;
+  for (int k = 0; k < (1 << cdim); ++k) {
+    double phi = 1.0;
+    for (int a = 0; a < cdim; ++a) phi *= ((k >> a) & 1) ? xi[a] : 1.0 - xi[a];
+    for (int d = 0; d < sdim; ++d) out[d] += phi * v[k][d];
+  }
+}
+
+int orc_cells_source(int q1, int cdim, int sdim, int64_t n_vertices,
+                     const double *vertices, int64_t n_cells, const int64_t *cells,
+                     int qorder, const double *nodal_rho, double *x, double *rho,
+                     double *jxw, double *center) {
+  (void)n_vertices;
+  if (cdim < 2 || cdim > 3 || sdim < cdim || sdim > 3) return -1;
+  double qp[1000][3], qw[1000];
+  int nq = 0;
+  if (q1) {
+    if (qorder < 1 || qorder > 10) return -1;
+    double gp[10], gw[10];
+    orc_gauss_legendre01(qorder, gp, gw);
+    const int nz = cdim == 3 ? qorder : 1;
+    for (int k = 0; k < nz; ++k)
+      for (int j = 0; j < qorder; ++j)
+        for (int i = 0; i < qorder; ++i) {
+          qp[nq][0] = gp[i];
+          qp[nq][1] = gp[j];
+          qp[nq][2] = cdim == 3 ? gp[k] : 0.0;
+          qw[nq] = gw[i] * gw[j] * (cdim == 3 ? gw[k] : 1.0);
+          ++nq;
+        }
+  } else {
+    /* the QGaussSimplex rules of orc_simplex_source */
+    if (qorder == 1) {
+      nq = 1;
+      for (int d = 0; d < 3; ++d) qp[0][d] = d < cdim ? 1.0 / (cdim + 1) : 0.0;
+      qw[0] = cdim == 2 ? 0.5 : 1.0 / 6.0;
+    } else if (qorder == 2 && cdim == 2) {
+      const double a = 1.0 / 6.0, b = 2.0 / 3.0;
+      const double p[3][2] = {{a, a}, {b, a}, {a, b}};
+      nq = 3;
+      for (int q = 0; q < 3; ++q) {
+        qp[q][0] = p[q][0];
+        qp[q][1] = p[q][1];
+        qp[q][2] = 0.0;
+        qw[q] = 1.0 / 6.0;
+      }
+    } else if (qorder == 2 && cdim == 3) {
+      const double a = (5.0 - sqrt(5.0)) / 20.0, b = (5.0 + 3.0 * sqrt(5.0)) / 20.0;
+      const double p[4][3] = {{a, a, a}, {b, a, a}, {a, b, a}, {a, a, b}};
+      nq = 4;
+      for (int q = 0; q < 4; ++q) {
+        for (int d = 0; d < 3; ++d) qp[q][d] = p[q][d];
+        qw[q] = 1.0 / 24.0;
+      }
+    } else {
+      return -1;
+    }
+  }
+  const int nv = q1 ? 1 << cdim : cdim + 1;
+#pragma omp parallel for schedule(static)
+  for (int64_t c = 0; c < n_cells; ++c) {
+    double v[8][3] = {{0}}, vr[8], cc[3] = {0, 0, 0};
+    for (int i = 0; i < nv; ++i) {
+      const int64_t vid = cells[c * nv + i];
+      for (int d = 0; d < sdim; ++d) v[i][d] = vertices[vid * sdim + d];
+      vr[i] = nodal_rho ? nodal_rho[vid] : 1.0;
+      for (int d = 0; d < sdim; ++d) cc[d] += v[i][d];
+    }
+    for (int d = 0; d < sdim; ++d) center[c * sdim + d] = cc[d] / nv;
+    for (int q = 0; q < nq; ++q) {
+      double xp[3] = {0, 0, 0}, J[3][3] = {{0}}, r = 0.0;
+      if (q1) {
+        q1_point(cdim, sdim, v, qp[q], xp);
+        for (int a = 0; a < cdim; ++a) {
+          double lo_xi[3], hi_xi[3], plo[3], phi_[3];
+          for (int b = 0; b < 3; ++b) lo_xi[b] = hi_xi[b] = qp[q][b];
+          lo_xi[a] = 0.0;
+          hi_xi[a] = 1.0;
+          q1_point(cdim, sdim, v, lo_xi, plo);
+          q1_point(cdim, sdim, v, hi_xi, phi_);
+          for (int d = 0; d < sdim; ++d) J[d][a] = phi_[d] - plo[d];
+        }
+        for (int k = 0; k < nv; ++k) {
+          double phi = 1.0;
+          for (int a = 0; a < cdim; ++a) phi *= ((k >> a) & 1) ? qp[q][a] : 1.0 - qp[q][a];
+          r += phi * vr[k];
+        }
+      } else {
+        double lam[4];
+        lam[0] = 1.0;
+        for (int a = 0; a < cdim; ++a) {
+          lam[a + 1] = qp[q][a];
+          lam[0] -= qp[q][a];
+        }
+        for (int k = 0; k < nv; ++k) {
+          for (int d = 0; d < sdim; ++d) xp[d] += lam[k] * v[k][d];
+          r += lam[k] * vr[k];
+        }
+        for (int a = 0; a < cdim; ++a)
+          for (int d = 0; d < sdim; ++d) J[d][a] = v[a + 1][d] - v[0][d];
+      }
+      double meas;
+      if (cdim == sdim) {
+        meas = cdim == 2 ? J[0][0] * J[1][1] - J[0][1] * J[1][0]
+                         : J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                               J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                               J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+        meas = fabs(meas);
+      } else {
+        const double cx = J[1][0] * J[2][1] - J[2][0] * J[1][1];
+        const double cy = J[2][0] * J[0][1] - J[0][0] * J[2][1];
+        const double cz = J[0][0] * J[1][1] - J[1][0] * J[0][1];
+        meas = sqrt(cx * cx + cy * cy + cz * cz);
+      }
+      const int64_t g = c * nq + q;
+      for (int d = 0; d < sdim; ++d) x[g * sdim + d] = xp[d];
+      rho[g] = r;
+      jxw[g] = meas * qw[q];
+    }
+  }
+  return nq;
+}
+
 double orc_normalize_density(int64_t n_q, const double *rho, const double *jxw) {
   double s = 0.0;
   for (int64_t i = 0; i < n_q; ++i) s += fabs(rho[i]) * jxw[i];

@@ -86,6 +86,10 @@ int orc_simplex_source(int dim, int64_t n_vertices, const double *vertices,
                        int64_t n_cells, const int64_t *cells, int qorder,
                        const double *nodal_rho, double *x, double *rho,
                        double *jxw, double *center);
+int orc_cells_source(int q1, int cdim, int sdim, int64_t n_vertices,
+                     const double *vertices, int64_t n_cells, const int64_t *cells,
+                     int qorder, const double *nodal_rho, double *x, double *rho,
+                     double *jxw, double *center);
 double orc_normalize_density(int64_t n_q, const double *rho, const double *jxw);
 
 /* ---- thresholds ---- */

@@ -4,8 +4,13 @@
 //   * structured Q1 boxes (GridGenerator::subdivided_hyper_rectangle, FE_Q(1),
 //     MappingQ1, QGauss(order)), and
 //   * P1 simplices (FE_SimplexP(1), MappingFE, QGaussSimplex(order)),
+//   * general Q1 quadrilaterals and hexahedra (FE_Q(1), MappingQ1,
+//     QGauss(order)) with vertices in deal.II's lexicographic order, also as
+//     surfaces in 3D (SotSolver<2, 3>), and P1 triangles in 3D,
 // with the element/quadrature choice of utils.h:776-840 and the density
-// normalisation of SemiDiscreteOT.cc:422-449.
+// normalisation of SemiDiscreteOT.cc:422-449.  For a cell of lower dimension
+// than the space, JxW is the Gram measure sqrt(det(J^T J)) times the weight,
+// as deal.II's FEValues computes it for codimension one.
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -186,6 +191,83 @@ void simplex_fill(int dim, const double *vert, int64_t n_cells, const int64_t *c
   }
 }
 
+// |det J| for a square Jacobian, sqrt(det(J^T J)) otherwise (J is
+// spacedim x cdim, columns = derivatives along the reference axes).
+double measure(int cdim, int sdim, const double J[3][3]) {
+  if (cdim == sdim) {
+    if (cdim == 2) return std::fabs(J[0][0] * J[1][1] - J[0][1] * J[1][0]);
+    return std::fabs(J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                     J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                     J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]));
+  }
+  double G[2][2] = {};
+  for (int a = 0; a < cdim; ++a)
+    for (int b = 0; b < cdim; ++b)
+      for (int d = 0; d < sdim; ++d) G[a][b] += J[d][a] * J[d][b];
+  if (cdim == 1) return std::sqrt(G[0][0]);
+  return std::sqrt(G[0][0] * G[1][1] - G[0][1] * G[1][0]);
+}
+
+// General cells: Q1 (2^cdim vertices, lexicographic) or P1 (cdim + 1 vertices)
+// of dimension cdim in a space of dimension sdim.
+void cells_fill(bool q1, int cdim, int sdim, const double *vert, int64_t n_cells,
+                const int64_t *cells, const Rule &r, const double *nodal, double *x, double *rho,
+                double *jxw, double *cen) {
+  const int nv = q1 ? 1 << cdim : cdim + 1;
+  for (int64_t c = 0; c < n_cells; ++c) {
+    double V[8][3] = {};
+    double Vr[8];
+    for (int i = 0; i < nv; ++i) {
+      const int64_t vid = cells[c * nv + i];
+      for (int d = 0; d < sdim; ++d) V[i][d] = vert[vid * sdim + d];
+      Vr[i] = nodal ? nodal[vid] : 1.0;
+    }
+    double cc[3] = {0, 0, 0};
+    for (int i = 0; i < nv; ++i)
+      for (int d = 0; d < sdim; ++d) cc[d] += V[i][d];
+    for (int d = 0; d < sdim; ++d) cen[c * sdim + d] = cc[d] / nv;
+    for (int q = 0; q < r.nq; ++q) {
+      const double *xi = &r.xi[(size_t)q * cdim];
+      double phi[8], dphi[8][3] = {};
+      if (q1) {
+        for (int v = 0; v < nv; ++v) {
+          phi[v] = 1.0;
+          for (int k = 0; k < cdim; ++k) {
+            const double f = ((v >> k) & 1) ? xi[k] : 1.0 - xi[k];
+            phi[v] *= f;
+          }
+          for (int k = 0; k < cdim; ++k) {
+            double g = ((v >> k) & 1) ? 1.0 : -1.0;
+            for (int m = 0; m < cdim; ++m)
+              if (m != k) g *= ((v >> m) & 1) ? xi[m] : 1.0 - xi[m];
+            dphi[v][k] = g;
+          }
+        }
+      } else {
+        phi[0] = 1.0;
+        for (int k = 0; k < cdim; ++k) {
+          phi[k + 1] = xi[k];
+          phi[0] -= xi[k];
+          dphi[0][k] = -1.0;
+          dphi[k + 1][k] = 1.0;
+        }
+      }
+      double xp[3] = {0, 0, 0}, rv = 0.0, J[3][3] = {};
+      for (int v = 0; v < nv; ++v) {
+        for (int d = 0; d < sdim; ++d) {
+          xp[d] += V[v][d] * phi[v];
+          for (int k = 0; k < cdim; ++k) J[d][k] += V[v][d] * dphi[v][k];
+        }
+        rv += Vr[v] * phi[v];
+      }
+      const int64_t g = c * r.nq + q;
+      for (int d = 0; d < sdim; ++d) x[g * sdim + d] = xp[d];
+      rho[g] = rv;
+      jxw[g] = measure(cdim, sdim, J) * r.w[q];
+    }
+  }
+}
+
 double l1_mass(int64_t n, const double *rho, const double *jxw) {
   double s = 0.0;
   for (int64_t i = 0; i < n; ++i) s += std::fabs(rho[i]) * jxw[i];
@@ -240,6 +322,42 @@ extern "C" int sdot_mesh_simplex(int dim, int64_t n_vertices, const double *vert
     for (auto &v : nodal) v /= L;
     simplex_fill(dim, vertices, n_cells, cells, r, nodal.data(), x, rho.data(), jxw.data(),
                  centers);
+  }
+  for (int64_t i = 0; i < nQ; ++i) w[i] = rho[i] * jxw[i];
+  return SDOT_OK;
+}
+
+extern "C" int sdot_mesh_cells(int kind, int cell_dim, int spacedim, int64_t n_vertices,
+                               const double *vertices, int64_t n_cells, const int64_t *cells,
+                               int qorder, const double *nodal_rho, int normalize, double *x,
+                               double *w, double *centers) {
+  if (kind != SDOT_CELL_SIMPLEX && kind != SDOT_CELL_Q1) return SDOT_ERR_INVALID;
+  if (cell_dim < 2 || cell_dim > 3 || spacedim < cell_dim || spacedim > 3) return SDOT_ERR_INVALID;
+  if (!vertices || !cells || !x || !w || !centers || n_cells < 1 || n_vertices < 1)
+    return SDOT_ERR_INVALID;
+  const bool q1 = kind == SDOT_CELL_Q1;
+  Rule r;
+  if (q1) {
+    if (qorder < 1 || qorder > 16) return SDOT_ERR_INVALID;
+    r = tensor_rule(cell_dim, qorder);
+  } else if (!simplex_rule(cell_dim, qorder, r)) {
+    return SDOT_ERR_UNSUPPORTED;
+  }
+  const int nv = q1 ? 1 << cell_dim : cell_dim + 1;
+  for (int64_t i = 0; i < n_cells * nv; ++i)
+    if (cells[i] < 0 || cells[i] >= n_vertices) return SDOT_ERR_INVALID;
+  const int64_t nQ = n_cells * r.nq;
+  std::vector<double> rho(nQ), jxw(nQ);
+  std::vector<double> nodal(n_vertices, 1.0);
+  if (nodal_rho) std::memcpy(nodal.data(), nodal_rho, sizeof(double) * n_vertices);
+  cells_fill(q1, cell_dim, spacedim, vertices, n_cells, cells, r, nodal.data(), x, rho.data(),
+             jxw.data(), centers);
+  if (normalize) {
+    const double L = l1_mass(nQ, rho.data(), jxw.data());
+    if (!(L > 0)) return SDOT_ERR_INVALID;
+    for (auto &v : nodal) v /= L;
+    cells_fill(q1, cell_dim, spacedim, vertices, n_cells, cells, r, nodal.data(), x, rho.data(),
+               jxw.data(), centers);
   }
   for (int64_t i = 0; i < nQ; ++i) w[i] = rho[i] * jxw[i];
   return SDOT_OK;

@@ -17,6 +17,13 @@ Hierarchy directory, level 0 the finest:
   level_<k>_density.txt   one weight per line
   level_<k>_children.txt  k > 0: per level-k point, "n c_1 ... c_n" (indices into level k-1)
   level_<k>_parents.txt   k < last: per level-k point, "n p_1 ... p_n" (indices into level k+1)
+
+Mesh files as MeshManager::load_source_mesh reads them with deal.II's GridIn
+(MeshManager.templates.h:80-140): gmsh 2.2 ASCII (.msh) and legacy VTK ASCII
+unstructured grids (.vtk).  The cells of the highest dimension present are
+kept (lines and points carry boundary data only); quadrilaterals and
+hexahedra are reordered from gmsh / VTK numbering (counter-clockwise faces)
+to deal.II's lexicographic vertex order.
 """
 from __future__ import annotations
 
@@ -170,3 +177,132 @@ def write_target_hierarchy(directory: str, points: list, densities: list,
                 for c in range(n_coarse):
                     ch = order[bounds[c]: bounds[c + 1]]
                     f.write(" ".join([str(len(ch))] + [str(int(i)) for i in ch]) + "\n")
+
+
+# gmsh element type -> (kind, cell dimension, vertices); VTK cell type likewise
+_GMSH = {2: ("simplex", 2, 3), 3: ("q1", 2, 4), 4: ("simplex", 3, 4), 5: ("q1", 3, 8)}
+_VTK = {5: ("simplex", 2, 3), 9: ("q1", 2, 4), 10: ("simplex", 3, 4), 12: ("q1", 3, 8)}
+# counter-clockwise quad / hex numbering (gmsh and VTK share it) -> lexicographic
+_TO_LEX = {4: [0, 1, 3, 2], 8: [0, 1, 3, 2, 4, 5, 7, 6]}
+
+
+class MeshData:
+    """Vertices [n, spacedim] and cells [n_cells, vertices per cell] (lexicographic
+    for Q1 cells) of the highest-dimensional cells of a mesh file."""
+
+    def __init__(self, vertices, cells, kind, cell_dim, spacedim):
+        self.vertices, self.cells = vertices, cells
+        self.kind, self.cell_dim, self.spacedim = kind, cell_dim, spacedim
+
+    @property
+    def n_cells(self):
+        return len(self.cells)
+
+
+def _finish(nodes, elems, table, spacedim):
+    if not elems:
+        raise ValueError("mesh file has no 2D or 3D cells")
+    top = max(table[t][1] for t, _ in elems)
+    kinds = {table[t][0] for t, _ in elems if table[t][1] == top}
+    if len(kinds) != 1:
+        raise ValueError("mixed quadrilateral/simplex meshes are not supported")
+    kind = kinds.pop()
+    cells = np.array([c for t, c in elems if table[t][1] == top], dtype=np.int64)
+    if kind == "q1":
+        cells = cells[:, _TO_LEX[cells.shape[1]]]
+    if spacedim is None:
+        spacedim = 3 if (top == 3 or np.any(nodes[:, 2] != 0.0)) else 2
+    return MeshData(np.ascontiguousarray(nodes[:, :spacedim]), np.ascontiguousarray(cells), kind,
+                    top, spacedim)
+
+
+def read_msh(path, spacedim=None) -> MeshData:
+    """GridIn::read_msh of a gmsh 2.2 ASCII file: $Nodes (tag x y z) and
+    $Elements (tag type n_tags tags... vertex tags); node tags may be sparse."""
+    with open(path) as f:
+        lines = f.read().split("\n")
+    it = iter(range(len(lines)))
+    nodes, tag_of, elems = None, {}, []
+    for i in it:
+        line = lines[i].strip()
+        if line == "$MeshFormat":
+            ver = lines[i + 1].split()
+            if not ver or not ver[0].startswith("2") or ver[1] != "0":
+                raise ValueError(f"{path}: only gmsh 2.x ASCII files are read (format {ver})")
+        elif line == "$Nodes":
+            n = int(lines[i + 1])
+            rows = np.array([lines[i + 2 + k].split() for k in range(n)], dtype=np.float64)
+            tags = rows[:, 0].astype(np.int64)
+            tag_of = {int(t): k for k, t in enumerate(tags)}
+            nodes = np.zeros((n, 3))
+            nodes[:, :rows.shape[1] - 1] = rows[:, 1:4]
+        elif line == "$Elements":
+            n = int(lines[i + 1])
+            for k in range(n):
+                v = lines[i + 2 + k].split()
+                t = int(v[1])
+                if t not in _GMSH:
+                    continue
+                ntags = int(v[2])
+                verts = [tag_of[int(a)] for a in v[3 + ntags:3 + ntags + _GMSH[t][2]]]
+                elems.append((t, verts))
+    if nodes is None:
+        raise ValueError(f"{path}: no $Nodes section")
+    return _finish(nodes, elems, _GMSH, spacedim)
+
+
+def read_vtk(path, spacedim=None) -> MeshData:
+    """GridIn::read_vtk of a legacy ASCII UNSTRUCTURED_GRID: POINTS, CELLS and
+    CELL_TYPES sections."""
+    with open(path) as f:
+        tok = f.read().split()
+    pos = {w: i for i, w in reversed(list(enumerate(tok))) if w in ("POINTS", "CELLS", "CELL_TYPES")}
+    if "POINTS" not in pos or "CELLS" not in pos or "CELL_TYPES" not in pos:
+        raise ValueError(f"{path}: not a legacy VTK unstructured grid")
+    i = pos["POINTS"]
+    n = int(tok[i + 1])
+    nodes = np.array(tok[i + 3:i + 3 + 3 * n], dtype=np.float64).reshape(n, 3)
+    i = pos["CELLS"]
+    nc = int(tok[i + 1])
+    k = i + 3
+    conn = []
+    for _ in range(nc):
+        m = int(tok[k])
+        conn.append([int(a) for a in tok[k + 1:k + 1 + m]])
+        k += 1 + m
+    i = pos["CELL_TYPES"]
+    types = [int(a) for a in tok[i + 2:i + 2 + nc]]
+    elems = [(t, c) for t, c in zip(types, conn) if t in _VTK]
+    return _finish(nodes, elems, _VTK, spacedim)
+
+
+def read_mesh(path, spacedim=None) -> MeshData:
+    """By extension, as load_source_mesh dispatches (.vtk, .msh); anything else
+    raises "Unsupported file format"."""
+    ext = path.rsplit(".", 1)[-1]
+    if ext == "msh":
+        return read_msh(path, spacedim)
+    if ext == "vtk":
+        return read_vtk(path, spacedim)
+    raise RuntimeError("Unsupported file format: " + ext)
+
+
+def write_msh(path, vertices, cells, kind, cell_dim):
+    """Gmsh 2.2 ASCII with the given cells (lexicographic Q1 order converted
+    back to gmsh numbering); the inverse of read_msh."""
+    v = np.asarray(vertices, dtype=np.float64)
+    if v.shape[1] == 2:
+        v = np.hstack([v, np.zeros((len(v), 1))])
+    cl = np.asarray(cells, dtype=np.int64)
+    gtype = {("simplex", 2): 2, ("q1", 2): 3, ("simplex", 3): 4, ("q1", 3): 5}[(kind, cell_dim)]
+    if kind == "q1":
+        inv = np.argsort(_TO_LEX[cl.shape[1]])
+        cl = cl[:, inv]
+    with open(path, "w") as f:
+        f.write("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n%d\n" % len(v))
+        for k, p in enumerate(v):
+            f.write("%d %s %s %s\n" % (k + 1, _fmt(p[0]), _fmt(p[1]), _fmt(p[2])))
+        f.write("$EndNodes\n$Elements\n%d\n" % len(cl))
+        for k, c in enumerate(cl):
+            f.write("%d %d 2 0 1 %s\n" % (k + 1, gtype, " ".join(str(int(a) + 1) for a in c)))
+        f.write("$EndElements\n")

@@ -10,6 +10,10 @@ mesh extraction so that the measured path never touches the oracle.
              U[0.2,0.8]^3, sigma U[0.05,0.15], seed 1); 50,000 targets from a
              6-component mixture (seed 101), nu ~ Dirichlet(5); eps = 1e-3 with
              pointwise truncation.
+  ball_tets  Delaunay tetrahedra of 200,000 seeded points in the unit ball
+             (seed 3), P1 density 1 + 0.5 sin(4 pi x0), 4-point rule (about
+             5.4M q); 500,000 targets uniform in the ball, uniform weights;
+             eps = 5e-4 with truncation.
   multilevel_3d  the gmm_3d density on 16^3 -> 32^3 -> 64^3 hexes, targets
              1,000 -> 10,000 -> 100,000 (k-means levels of the 100,000 set,
              seed 2), epsilon 1e-1 -> 1e-2 -> 1e-3.
@@ -81,6 +85,36 @@ def gmm_3d(n_cells=32, n_targets=50000, seed=1, eps=1e-3, tau=1e-16) -> Workload
     nu = trng.dirichlet(np.full(n_targets, 5.0))
     nu = nu / nu.sum()
     return Workload(f"gmm_3d_{n_cells}cube_q2_N{n_targets}", src, y.ravel(), nu, eps, tau)
+
+
+def ball_points(rng, n):
+    """n points uniform in the unit ball."""
+    p = rng.standard_normal((n, 3))
+    p *= (rng.random(n) ** (1.0 / 3.0) / np.linalg.norm(p, axis=1))[:, None]
+    return p
+
+
+def ball_tets(n_points=200000, n_targets=500000, seed=3, eps=5e-4, tau=1e-12) -> Workload:
+    """Large unstructured workload: Delaunay tetrahedra (scipy) of n_points
+    seeded points uniform in the unit ball (seed 3), P1 density
+    1 + 0.5 sin(4 pi x0) at the vertices, QGaussSimplex(2) (4 points per
+    tet); n_targets targets uniform in the ball, uniform weights; eps = 5e-4
+    with pointwise truncation.  Zero-volume slivers of the triangulation are
+    dropped (deal.II rejects them)."""
+    from scipy.spatial import Delaunay
+    from .mesh import simplex_source
+    rng = np.random.default_rng(seed)
+    p = ball_points(rng, n_points)
+    cells = Delaunay(p).simplices.astype(np.int64)
+    v = p[cells]
+    vol = np.abs(np.einsum("ij,ij->i", v[:, 1] - v[:, 0],
+                           np.cross(v[:, 2] - v[:, 0], v[:, 3] - v[:, 0]))) / 6.0
+    cells = np.ascontiguousarray(cells[vol > 1e-12])
+    rho = 1.0 + 0.5 * np.sin(4.0 * np.pi * p[:, 0])
+    src = simplex_source(3, p.ravel(), cells.ravel(), 2, rho)
+    y = ball_points(rng, n_targets)
+    nu = np.full(n_targets, 1.0 / n_targets)
+    return Workload(f"ball_tets_{n_points}pts_p1q4_N{n_targets}", src, y.ravel(), nu, eps, tau)
 
 
 @dataclass
