@@ -131,6 +131,8 @@ SIGNATURES = {
                                        C.c_int, _dp, _dp, _dp]),
     "sdot_mesh_simplex": (C.c_int, [C.c_int, C.c_int64, _dp, C.c_int64, _i64p, C.c_int, _dp,
                                     C.c_int, _dp, _dp, _dp]),
+    "sdot_mesh_cells": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int64, _dp, C.c_int64, _i64p,
+                                  C.c_int, _dp, C.c_int, _dp, _dp, _dp]),
 }
 
 _lib = None
