@@ -69,9 +69,57 @@ def cpu_baseline(wl, psi, R, target_seconds=15.0):
         k = min(n_cells, int(k * max(2.0, target_seconds / max(dt, 1e-3) * 0.5)))
     cnt, _ = O.candidate_stats(s, tgt, R)
     pairs = int(cnt.sum()) * nq
+    # the reference's CPU-runnable configuration solved end to end by the port
+    # (the same L-BFGS and line search as the GPU leg)
+    p2 = PLAIN_2D
+    src2 = O.structured_source(2, [64, 64], [0.0, 0.0], [1.0, 1.0], 3)
+    y2 = np.random.default_rng(0).random((1000, 2))
+    prm2 = O.params(epsilon=p2["epsilon"], tau=0.0, tolerance=p2["tolerance"], control="l1norm",
+                    max_iterations=p2["max_iterations"], use_lse=1, n_threads=cores)
+    t0 = time.perf_counter()
+    _, info2 = O.solve(src2, O.Target(y2.ravel(), np.full(1000, 1e-3)), prm2)
+    t_solve = time.perf_counter() - t0
     return {"value": pairs / dt, "unit": "pair-evals/s", "cores": cores, "kind": "port",
             "sample": f"{k} of {n_cells} cells (evenly spaced), one FP64 evaluation, "
-                      f"{pairs} pairs in {dt:.2f} s"}
+                      f"{pairs} pairs in {dt:.2f} s",
+            "plain_2d_solve": {"seconds": t_solve, "status": info2["status"],
+                               "iterations": info2["iterations"],
+                               "evaluations": info2["evaluations"],
+                               "check_value": info2["check_value"]}}
+
+
+# the reference's CPU-runnable configuration: 64x64 Q1 / QGauss(3), 1,000
+# uniform targets (seed 0), eps = 1e-2, no truncation, FP64, l1 tolerance 1e-8
+PLAIN_2D = {"epsilon": 1e-2, "tolerance": 1e-8, "max_iterations": 1000}
+
+
+def plain_2d_leg():
+    """The reference's CPU-runnable configuration solved on the GPU with the
+    reference's algorithm (FP64, identity initial Hessian, deal.II line search)
+    and with the noise-aware search."""
+    import semidiscreteot_amd as sd
+    from semidiscreteot_amd import workloads
+    wl = workloads.plain_2d()
+    s = sd.SotSolver(2)
+    s.setup_source(wl.source.x, wl.source.w, wl.source.centers, wl.source.nq)
+    s.setup_target(wl.y, wl.nu)
+    out = {"workload": wl.name}
+    for ls in ("reference", "noise_aware"):
+        s.configure(sd.SolverParameters(epsilon=PLAIN_2D["epsilon"], tau=0.0,
+                                        tolerance=PLAIN_2D["tolerance"], solver_control_type="l1norm",
+                                        max_iterations=PLAIN_2D["max_iterations"],
+                                        use_log_sum_exp_trick=True, verbose_output=False,
+                                        line_search=ls))
+        t0 = time.perf_counter()
+        s.solve()
+        dt = time.perf_counter() - t0
+        info = s.get_last_solve_info()
+        out[ls] = {"seconds": dt, "status": info["status"], "iterations": info["iterations"],
+                   "evaluations": info["evaluations"], "check_value": info["check_value"]}
+        log(f"plain 2D solve ({ls} line search): {info['status']} after {info['iterations']} "
+            f"iterations in {dt:.3f}s, ||g||_1 = {info['check_value']:.3e}")
+    s.close()
+    return out
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_eval_gmm3d_32cube_N50000.md")
@@ -540,6 +588,7 @@ def main():
     if args.multilevel:
         log("multilevel leg ...")
         multilevel = multilevel_leg(args)
+    plain2d = plain_2d_leg()
     tets = None
     if args.tets:
         log("tets leg ...")
@@ -580,6 +629,7 @@ def main():
         "kernel_ms": kern,
         "serving": serving,
         "multilevel": multilevel,
+        "plain_2d_solve": plain2d,
         "large_unstructured": tets,
         "cpu_baseline": cpu,
     }

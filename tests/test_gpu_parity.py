@@ -296,3 +296,38 @@ def test_discarded_mass_audit():
     fo = O.discarded_fraction(src, tgt, O.params(epsilon=5e-3, use_lse=1), R, psi, q)
     assert np.max(np.abs(frac[q] - fo)) < 1e-12
     assert abs(s.discarded_mass(R, psi, src.w) - float(np.sum(src.w * frac))) < 1e-15
+
+
+def test_plain_2d_config_solve():
+    """The reference's CPU-runnable configuration end to end: 64x64 Q1 / QGauss(3)
+    on the unit square (36,864 q), 1,000 uniform targets (seed 0), eps = 1e-2,
+    no truncation, FP64, l1-norm tolerance 1e-8 from psi = 0.  With the
+    reference's line search the GPU solve and the oracle's end the same way
+    (the functional's rounding noise stops both near ||g||_1 ~ 5e-8) after
+    nearly the same iterations at nearly the same potential; the noise-aware
+    search reaches the tolerance, confirmed on the oracle's gradient."""
+    say("plain 2D configuration")
+    src = O.structured_source(2, [64, 64], [0.0, 0.0], [1.0, 1.0], 3)
+    y = np.random.default_rng(0).random((1000, 2))
+    tgt = O.Target(y.ravel(), np.full(1000, 1e-3))
+    kw = dict(epsilon=1e-2, tau=0.0, tolerance=1e-8, solver_control_type="l1norm",
+              max_iterations=1000, use_log_sum_exp_trick=True, verbose_output=False)
+    s = make(src, tgt, **kw)
+    psi = s.solve()
+    info = s.get_last_solve_info()
+    prm = O.params(epsilon=1e-2, tau=0.0, tolerance=1e-8, control="l1norm", max_iterations=1000,
+                   use_lse=1)
+    psi_o, info_o = O.solve(src, tgt, prm)
+    say(f"gpu {info['status']} it={info['iterations']} cv={info['check_value']:.3e}; oracle "
+        f"{info_o['status']} it={info_o['iterations']} cv={info_o['check_value']:.3e}")
+    assert info["status"] == info_o["status"]
+    assert abs(info["iterations"] - info_o["iterations"]) <= 5
+    assert np.max(np.abs((psi - psi.mean()) - (psi_o - psi_o.mean()))) < 1e-6
+    assert abs(info["functional"] - info_o["functional"]) <= 1e-12 * abs(info_o["functional"])
+    s.configure(sd.SolverParameters(line_search="noise_aware", **kw))
+    psi2 = s.solve()
+    info2 = s.get_last_solve_info()
+    _, g, _ = O.evaluate_R(src, tgt, prm, DMAX, psi2)
+    say(f"noise-aware: {info2['status']} it={info2['iterations']} oracle ||g||_1 {np.abs(g).sum():.3e}")
+    assert info2["status"] == "converged" and np.abs(g).sum() <= 1e-8 * 1.01
+    s.close()
