@@ -417,16 +417,23 @@ def tets_leg(args):
         cost = sx2 - float(np.sum(wl.nu[:, None] * y * b)) + float(np.sum(m * 0.5 * np.sum(y * y, 1)))
         s.eval_device(R, d_psi.address, d_grad.address, d_fc.address)
         s.synchronize()
+        s.profile_enable(True)
+        s.synchronize()
         ta = time.perf_counter()
         for _ in range(args.tets_steps):
             s.eval_device(R, d_psi.address, d_grad.address, d_fc.address)
         s.synchronize()
         ms = (time.perf_counter() - ta) * 1e3 / args.tets_steps
-        vals[prec] = (F, g, C, cost, ms)
+        prof = s.profile_get()
+        s.profile_enable(False)
+        kern = {k: prof[k] / max(1, prof["evaluations"])
+                for k in ("ms_lse_max", "ms_lse_sum", "ms_qfin", "ms_grad")}
+        vals[prec] = (F, g, C, cost, ms, kern)
         log(f"tets eval {prec}: {ms:.2f} ms, {pairs / (ms * 1e-3):.3e} pair-evals/s, F={F:.15e} "
-            f"C={C:.10e} cost={cost:.10e}")
-    F64, g64, C64, c64, ms64 = vals["fp64"]
-    F32, g32, C32, c32, ms32 = vals["fp32_lse"]
+            f"C={C:.10e} cost={cost:.10e}; kernels "
+            + ", ".join(f"{k}={v:.3f}ms" for k, v in kern.items()))
+    F64, g64, C64, c64, ms64, k64 = vals["fp64"]
+    F32, g32, C32, c32, ms32, k32 = vals["fp32_lse"]
     psi32 = sol["fp32_lse"][0]
     a, b = psi64 - psi64.mean(), psi32 - psi32.mean()
     s.configure(sd.SolverParameters(precision="fp64", **base))
@@ -435,7 +442,7 @@ def tets_leg(args):
     out = {
         "workload": wl.name, "n_q": n_q, "n_targets": int(wl.nu.size), "epsilon": wl.epsilon,
         "R": R, "discarded_mass": dm, "pairs_per_eval": int(pairs),
-        "fp64_eval_ms": ms64, "fp32_eval_ms": ms32,
+        "fp64_eval_ms": ms64, "fp32_eval_ms": ms32, "fp64_kernel_ms": k64, "fp32_kernel_ms": k32,
         "fp64_pair_evals_per_s": pairs / (ms64 * 1e-3), "fp32_pair_evals_per_s": pairs / (ms32 * 1e-3),
         "solve": {p: {"status": sol[p][1]["status"], "iterations": sol[p][1]["iterations"],
                       "evaluations": sol[p][1]["evaluations"], "seconds": sol[p][2]}

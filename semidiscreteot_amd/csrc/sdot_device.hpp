@@ -13,7 +13,6 @@
 
 namespace sdot {
 
-
 // ---------------------------------------------------------------------------
 // helpers
 // ---------------------------------------------------------------------------
@@ -176,6 +175,98 @@ __device__ __forceinline__ void for_rows(const Grid &g, const int32_t *__restric
   }
 }
 
+// Item stream of a one-wave workgroup staged through LDS, for record sets
+// that outgrow the XCD's L2 (there the scalar record loads of the plain sweeps
+// wait out a memory round trip each).  The runs of item indices that for_rows
+// hands out are buffered (start, end, tag), and their items are streamed in
+// tiles of kWave: lane l fetches item l of the next tile (W doubles, with
+// coalesced 16-byte loads) into registers while the wave computes the current
+// tile from LDS, so the fetch latency hides behind a tile of compute.
+// fn(item, tested) sees the items in run order, as the runs were added, so
+// per-lane sums keep the plain sweeps' order.
+constexpr int kStageRuns = 128;
+
+template <int W>
+struct StageLds {
+  double item[kWave * W];
+  int tag[kWave];  // 1: the item needs the candidate test
+  int rs[kStageRuns], rt[kStageRuns], cum[kStageRuns + 1];
+};
+
+template <int W, class Fetch, class Fn>
+__device__ __forceinline__ void stage_flush(StageLds<W> &L, int nr, Fetch &&fetch_item, Fn &&fn) {
+  static_assert(W % 2 == 0, "items are moved as double2");
+  const int lane = threadIdx.x;
+  const int total = __builtin_amdgcn_readfirstlane(L.cum[nr]);
+  if (total <= 0) return;
+  double2 buf[W / 2];
+  int btag = 0;
+  int khint = 0;  // run holding the tile's first item (wave-uniform)
+  auto fetch = [&](int f0) {
+    const int f = f0 + lane;
+    if (f < total) {
+      int k = khint;
+      while (L.cum[k + 1] <= f) ++k;
+      const int j = L.rs[k] + (f - L.cum[k]);
+      fetch_item(j, buf);
+      btag = L.rt[k];
+    }
+  };
+  fetch(0);
+  for (int f0 = 0; f0 < total; f0 += kWave) {
+    __syncthreads();  // the previous tile has been read
+    double2 *dst = reinterpret_cast<double2 *>(&L.item[lane * W]);
+#pragma unroll
+    for (int i = 0; i < W / 2; ++i) dst[i] = buf[i];
+    L.tag[lane] = btag;
+    __syncthreads();
+    const int m = min(kWave, total - f0);
+    if (f0 + kWave < total) {
+      while (__builtin_amdgcn_readfirstlane(L.cum[khint + 1]) <= f0 + kWave) ++khint;
+      fetch(f0 + kWave);
+    }
+    for (int r = 0; r < m; ++r) fn(&L.item[r * W], L.tag[r] != 0);
+  }
+}
+
+// Streams the items of the runs produce(add) adds (add(start, end, tested))
+// through L, flushing every kStageRuns runs.
+template <int W, class Produce, class Fetch, class Fn>
+__device__ __forceinline__ void stage_stream(StageLds<W> &L, Produce &&produce, Fetch &&fetch_item,
+                                             Fn &&fn) {
+  int nr = 0;
+  if (threadIdx.x == 0) L.cum[0] = 0;
+  auto add = [&](int s, int e, bool tested) {
+    if (threadIdx.x == 0) {
+      L.rs[nr] = s;
+      L.rt[nr] = tested ? 1 : 0;
+      L.cum[nr + 1] = L.cum[nr] + (e - s);
+    }
+    ++nr;
+    if (nr == kStageRuns) {
+      __syncthreads();
+      stage_flush(L, nr, fetch_item, fn);
+      __syncthreads();
+      nr = 0;
+      if (threadIdx.x == 0) L.cum[0] = 0;
+    }
+  };
+  produce(add);
+  __syncthreads();
+  stage_flush(L, nr, fetch_item, fn);
+  __syncthreads();
+}
+
+// Item fetch of the q-major sweeps: one target record.
+struct FetchRec {
+  const double *__restrict__ rec;
+  __device__ __forceinline__ void operator()(int j, double2 *buf) const {
+    const double2 *src = reinterpret_cast<const double2 *>(rec + (int64_t)j * kRec);
+#pragma unroll
+    for (int i = 0; i < kRec / 2; ++i) buf[i] = src[i];
+  }
+};
+
 // Fixed-pattern block sum / max (blockDim.x == 256). Result valid in all threads.
 __device__ __forceinline__ double block_sum_256(double v, double *sh) {
   sh[threadIdx.x] = v;
@@ -302,7 +393,8 @@ __device__ __forceinline__ void chunkext_body(const double *dpsi_r, int64_t N, d
 // k_pass1: q-major sweeps
 // ---------------------------------------------------------------------------
 
-template <int DIM, int G, int MODE, int PASS>
+// STAGED: the records stream through LDS (record sets larger than an L2).
+template <int DIM, int G, int MODE, int PASS, bool STAGED = false>
 __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg) {
   const int lane = threadIdx.x;
   const int64_t grp = blk * kWave + lane;
@@ -336,10 +428,8 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
     lo[d] = a.blkbox[blk * 2 * DIM + d];
     hi[d] = a.blkbox[blk * 2 * DIM + DIM + d];
   }
-  auto one = [&](const TRec<DIM> &r, auto tag) {
-    if constexpr (decltype(tag)::value) {
-      if (!in_set<DIM>(r.y, c, a.smax)) return;
-    }
+  auto one = [&](const TRec<DIM> &r, bool tested) {
+    if (tested && !in_set<DIM>(r.y, c, a.smax)) return;
     if (PASS == 0) {
 #pragma unroll
       for (int k = 0; k < G; ++k) {
@@ -369,18 +459,6 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
       }
     }
   };
-  cdouble *trec = cptr(a.trec);
-  auto body = [&](int j0, int j1, auto tag) {
-    // records are fetched two at a time ahead of their use
-    for (int j = j0; j < j1; j += 2) {
-      const int j2 = (j + 1 < j1) ? j + 1 : j;
-      TRec<DIM> r0, r1;
-      r0.load(trec + (int64_t)j * kRec);
-      r1.load(trec + (int64_t)j2 * kRec);
-      one(r0, tag);
-      if (j + 1 < j1) one(r1, tag);
-    }
-  };
   auto sweep = [&](auto &&f) {
     if (a.dense) {
       const int64_t j0 = a.N * seg / a.K1, j1 = a.N * (seg + 1) / a.K1;
@@ -388,6 +466,43 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
     } else {
       for_rows<DIM>(a.tg, a.tstart, lo, hi, a.Rq, a.s_in, seg, a.K1, f);
     }
+  };
+  // the item's candidate records: scalar loads (records resident in L2), or
+  // staged through LDS (record sets larger than an XCD's L2)
+  cdouble *trec = cptr(a.trec);
+  auto records = [&](auto &&fn) {
+    if constexpr (STAGED) {
+      __shared__ StageLds<kRec> stage;
+      stage_stream(
+          stage,
+          [&](auto &add) {
+            sweep([&](int j0, int j1, auto tag) { add(j0, j1, decltype(tag)::value); });
+          },
+          FetchRec{a.trec}, fn);
+    } else {
+      sweep([&](int j0, int j1, auto tag) {
+        // records are fetched two at a time ahead of their use
+        for (int j = j0; j < j1; j += 2) {
+          const int j2 = (j + 1 < j1) ? j + 1 : j;
+          TRec<DIM> r0, r1;
+          r0.load(trec + (int64_t)j * kRec);
+          r1.load(trec + (int64_t)j2 * kRec);
+          fn(r0, decltype(tag)::value);
+          if (j + 1 < j1) fn(r1, decltype(tag)::value);
+        }
+      });
+    }
+  };
+  auto as_rec = [](auto &&f) {
+    return [f](const auto &src, bool tested) {
+      if constexpr (std::is_same_v<std::decay_t<decltype(src)>, TRec<DIM>>) {
+        f(src, tested);
+      } else {
+        TRec<DIM> r;
+        r.load(src);
+        f(r, tested);
+      }
+    };
   };
 
   if (PASS == 2) {
@@ -424,22 +539,16 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
       for (int k = 0; k < G; ++k) M[k] = ceil(a.Tout[(int64_t)seg * a.nQ + q0 + k] + dmx);
     } else {
       // exact maximum over the item's candidates first
-      sweep([&](int j0, int j1, auto tag) {
-        for (int j = j0; j < j1; ++j) {
-          TRec<DIM> r;
-          r.load(trec + (int64_t)j * kRec);
-          if constexpr (decltype(tag)::value) {
-            if (!in_set<DIM>(r.y, c, a.smax)) continue;
-          }
+      records(as_rec([&](const TRec<DIM> &r, bool tested) {
+        if (tested && !in_set<DIM>(r.y, c, a.smax)) return;
 #pragma unroll
-          for (int k = 0; k < G; ++k) {
-            double t = r.a;
+        for (int k = 0; k < G; ++k) {
+          double t = r.a;
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) t = fma(x[k][d], r.yl[d], t);
-            T[k] = vmax64(T[k], t);
-          }
+          for (int d = 0; d < DIM; ++d) t = fma(x[k][d], r.yl[d], t);
+          T[k] = vmax64(T[k], t);
         }
-      });
+      }));
 #pragma unroll
       for (int k = 0; k < G; ++k) {
         M[k] = ceil(T[k]);
@@ -447,7 +556,7 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
       }
     }
   }
-  sweep(body);
+  records(as_rec(one));
   if (active) {
 #pragma unroll
     for (int k = 0; k < G; ++k) {
@@ -478,7 +587,7 @@ __device__ __forceinline__ void qfin_body(const QFArgs &a, int64_t blk, double *
 #pragma unroll
     for (int d = 0; d < DIM; ++d) x[d] = a.qx[qm * DIM + d];
     const double x2 = a.qx2[qm];
-    double M, S = 0.0;
+    double M, S = 0.0, T = 0.0;
     if (MODE == kF64Plain) {
       M = a.plain_coef * x2;
       for (int s = 0; s < a.K1; ++s) S += a.Spart[(int64_t)s * a.nQ + qm];
@@ -490,7 +599,7 @@ __device__ __forceinline__ void qfin_body(const QFArgs &a, int64_t blk, double *
       // segments carry integer reference values: S = sum_s S_s 2^(M_s - M),
       // then renormalised to M = ceil(largest exponent) for the gradient pass
       M = -INFINITY;
-      double T = -INFINITY;
+      T = -INFINITY;
       for (int s = 0; s < a.K1; ++s) {
         M = fmax(M, a.Mpart[(int64_t)s * a.nQ + qm]);
         T = fmax(T, a.Tpart[(int64_t)s * a.nQ + qm]);
@@ -516,7 +625,9 @@ __device__ __forceinline__ void qfin_body(const QFArgs &a, int64_t blk, double *
     if (S > 0.0 && M > -INFINITY) {
       Fq = w * (a.eps * (M * a.ln_scale + log(S)) - 0.5 * x2);
       scale = w / S;
-      Cq = scale;
+      // C_global sums w_q / S_q with S_q taken relative to the largest term
+      // (SotSolver.cc:290); the FP32 sum is relative to 2^M = 2^ceil(T) instead
+      Cq = MODE == kF32Lse ? scale * exp2(T - M) : scale;
       Mrec = M;
     }
     double *r = a.qrec_r + qr * kRec;
@@ -606,12 +717,15 @@ __device__ __forceinline__ void grad_body(const GArgs &a, int64_t blk, int seg) 
       for (int k0 = 0; k0 < nq; k0 += G) chunk(qp + (int64_t)k0 * kRec);
     }
   };
-  if (a.dense) {
-    const int64_t c0 = a.n_cells * seg / a.K2, c1 = a.n_cells * (seg + 1) / a.K2;
-    body((int)c0, (int)c1, Inside{});
-  } else {
-    for_rows<DIM>(a.cg, a.cstart, lo, hi, a.Rq, a.s_in, seg, a.K2, body);
-  }
+  auto cells = [&](auto &&f) {
+    if (a.dense) {
+      const int64_t c0 = a.n_cells * seg / a.K2, c1 = a.n_cells * (seg + 1) / a.K2;
+      f((int)c0, (int)c1, Inside{});
+    } else {
+      for_rows<DIM>(a.cg, a.cstart, lo, hi, a.Rq, a.s_in, seg, a.K2, f);
+    }
+  };
+  cells(body);
   if (active) {
     a.Gpart[(int64_t)seg * a.N + j] = acc;
     if constexpr (BARY) {

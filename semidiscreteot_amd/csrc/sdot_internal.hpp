@@ -121,6 +121,9 @@ struct Plan {
   double s_in = 0.0;    // inner runs: every pair closer than sqrt(s_in) passes the rule
   int hist = 0;        // FP32 mode: seed the online sweep with the previous evaluation
   int dense = 0;       // every pair is a candidate (no enumeration, no test)
+  // the q-major sweeps stream the target records through LDS (record sets
+  // larger than one XCD's L2)
+  int staged1 = 0;
 };
 
 

@@ -52,9 +52,9 @@ __global__ __launch_bounds__(256) void k_chunkext(const double *dpsi_r, int64_t 
   chunkext_body(dpsi_r, N, dcx, dcn, (int64_t)blockIdx.x * (256 / kWave) + threadIdx.x / kWave);
 }
 
-template <int DIM, int G, int MODE, int PASS>
+template <int DIM, int G, int MODE, int PASS, bool STAGED>
 __global__ __launch_bounds__(64) void k_pass1(P1Args a) {
-  pass1_body<DIM, G, MODE, PASS>(a, blockIdx.x, blockIdx.y);
+  pass1_body<DIM, G, MODE, PASS, STAGED>(a, blockIdx.x, blockIdx.y);
 }
 
 template <int DIM, int MODE>
@@ -313,14 +313,26 @@ hipError_t launch_hessian_final(const Plan &p, const double *d_A, int64_t Npad, 
 // launchers
 // ---------------------------------------------------------------------------
 
-template <int DIM, int G, int MODE, int PASS>
+template <int DIM, int G, int MODE, int PASS, bool STAGED = false>
 static void launch_pass1_t(const P1Args &a, int64_t nb, int K1, hipStream_t s) {
-  hipLaunchKernelGGL((k_pass1<DIM, G, MODE, PASS>), dim3((unsigned)nb, (unsigned)K1), dim3(64),
-                     0, s, a);
+  hipLaunchKernelGGL((k_pass1<DIM, G, MODE, PASS, STAGED>), dim3((unsigned)nb, (unsigned)K1),
+                     dim3(64), 0, s, a);
 }
 
+// staged: the truncated sweeps' records stream through LDS (their groupings
+// have at most 4 points per lane)
 template <int DIM, int MODE, int PASS>
-static hipError_t launch_pass1_g(int G, const P1Args &a, int64_t nb, int K1, hipStream_t s) {
+static hipError_t launch_pass1_g(int G, const P1Args &a, int64_t nb, int K1, hipStream_t s,
+                                 bool staged = false) {
+  if (staged && G <= 4) {
+    switch (G) {
+      case 1: launch_pass1_t<DIM, 1, MODE, PASS, true>(a, nb, K1, s); break;
+      case 2: launch_pass1_t<DIM, 2, MODE, PASS, true>(a, nb, K1, s); break;
+      case 3: launch_pass1_t<DIM, 3, MODE, PASS, true>(a, nb, K1, s); break;
+      default: launch_pass1_t<DIM, 4, MODE, PASS, true>(a, nb, K1, s); break;
+    }
+    return hipGetLastError();
+  }
   switch (G) {
     case 1: launch_pass1_t<DIM, 1, MODE, PASS>(a, nb, K1, s); break;
     case 2: launch_pass1_t<DIM, 2, MODE, PASS>(a, nb, K1, s); break;
@@ -336,36 +348,28 @@ static hipError_t launch_pass1_g(int G, const P1Args &a, int64_t nb, int K1, hip
   return hipGetLastError();
 }
 
-template <int DIM, int MODE>
+template <int DIM, int MODE, bool BARY>
 static void launch_grad_g(int G, const GArgs &g, int64_t ntw, int K2, hipStream_t s) {
   const dim3 gg((unsigned)(ntw * K2));
+#define SDOT_GRAD(GG)                                                               \
+  do {                                                                              \
+    if constexpr (BARY)                                                             \
+      hipLaunchKernelGGL((k_grad_bary<DIM, MODE, GG>), gg, dim3(64), 0, s, g);      \
+    else                                                                            \
+      hipLaunchKernelGGL((k_grad<DIM, MODE, GG>), gg, dim3(64), 0, s, g);           \
+  } while (0)
   switch (G) {
-    case 1: hipLaunchKernelGGL((k_grad<DIM, MODE, 1>), gg, dim3(64), 0, s, g); break;
-    case 2: hipLaunchKernelGGL((k_grad<DIM, MODE, 2>), gg, dim3(64), 0, s, g); break;
-    case 3: hipLaunchKernelGGL((k_grad<DIM, MODE, 3>), gg, dim3(64), 0, s, g); break;
-    case 4: hipLaunchKernelGGL((k_grad<DIM, MODE, 4>), gg, dim3(64), 0, s, g); break;
-    case 5: hipLaunchKernelGGL((k_grad<DIM, MODE, 5>), gg, dim3(64), 0, s, g); break;
-    case 6: hipLaunchKernelGGL((k_grad<DIM, MODE, 6>), gg, dim3(64), 0, s, g); break;
-    case 7: hipLaunchKernelGGL((k_grad<DIM, MODE, 7>), gg, dim3(64), 0, s, g); break;
-    case 8: hipLaunchKernelGGL((k_grad<DIM, MODE, 8>), gg, dim3(64), 0, s, g); break;
-    default: hipLaunchKernelGGL((k_grad<DIM, MODE, 9>), gg, dim3(64), 0, s, g); break;
+    case 1: SDOT_GRAD(1); break;
+    case 2: SDOT_GRAD(2); break;
+    case 3: SDOT_GRAD(3); break;
+    case 4: SDOT_GRAD(4); break;
+    case 5: SDOT_GRAD(5); break;
+    case 6: SDOT_GRAD(6); break;
+    case 7: SDOT_GRAD(7); break;
+    case 8: SDOT_GRAD(8); break;
+    default: SDOT_GRAD(9); break;
   }
-}
-
-template <int DIM, int MODE>
-static void launch_grad_bary_g(int G, const GArgs &g, int64_t ntw, int K2, hipStream_t s) {
-  const dim3 gg((unsigned)(ntw * K2));
-  switch (G) {
-    case 1: hipLaunchKernelGGL((k_grad_bary<DIM, MODE, 1>), gg, dim3(64), 0, s, g); break;
-    case 2: hipLaunchKernelGGL((k_grad_bary<DIM, MODE, 2>), gg, dim3(64), 0, s, g); break;
-    case 3: hipLaunchKernelGGL((k_grad_bary<DIM, MODE, 3>), gg, dim3(64), 0, s, g); break;
-    case 4: hipLaunchKernelGGL((k_grad_bary<DIM, MODE, 4>), gg, dim3(64), 0, s, g); break;
-    case 5: hipLaunchKernelGGL((k_grad_bary<DIM, MODE, 5>), gg, dim3(64), 0, s, g); break;
-    case 6: hipLaunchKernelGGL((k_grad_bary<DIM, MODE, 6>), gg, dim3(64), 0, s, g); break;
-    case 7: hipLaunchKernelGGL((k_grad_bary<DIM, MODE, 7>), gg, dim3(64), 0, s, g); break;
-    case 8: hipLaunchKernelGGL((k_grad_bary<DIM, MODE, 8>), gg, dim3(64), 0, s, g); break;
-    default: hipLaunchKernelGGL((k_grad_bary<DIM, MODE, 9>), gg, dim3(64), 0, s, g); break;
-  }
+#undef SDOT_GRAD
 }
 
 template <int DIM, int MODE>
@@ -430,15 +434,16 @@ static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_g
   a.Tout = d.Tpart;
   hipError_t e;
   // exact maxima first (FP64 LSE; the FP32 sweep finds its own bounds)
+  const bool st1 = p.staged1 != 0;
   if (MODE == kF64Lse) {
-    e = launch_pass1_g<DIM, MODE, 0>(p.G, a, p.n_blocks, p.K1, s);
+    e = launch_pass1_g<DIM, MODE, 0>(p.G, a, p.n_blocks, p.K1, s, st1);
     if (e != hipSuccess) return e;
   }
   if (timing) (void)hipEventRecord(prof->ev[1], s);
   if (MODE == kF32Lse)
-    e = launch_pass1_g<DIM, MODE, 2>(p.G, a, p.n_blocks, p.K1, s);
+    e = launch_pass1_g<DIM, MODE, 2>(p.G, a, p.n_blocks, p.K1, s, st1);
   else
-    e = launch_pass1_g<DIM, MODE, 1>(p.G, a, p.n_blocks, p.K1, s);
+    e = launch_pass1_g<DIM, MODE, 1>(p.G, a, p.n_blocks, p.K1, s, st1);
   if (e != hipSuccess) return e;
   if (timing) (void)hipEventRecord(prof->ev[2], s);
 
@@ -481,9 +486,9 @@ static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_g
   g.s_in = p.s_in;
   g.Bpart = d_Bpart;
   if (d_bary)
-    launch_grad_bary_g<DIM, MODE>(p.G, g, p.n_twaves, p.K2, s);
+    launch_grad_g<DIM, MODE, true>(p.G, g, p.n_twaves, p.K2, s);
   else
-    launch_grad_g<DIM, MODE>(p.G, g, p.n_twaves, p.K2, s);
+    launch_grad_g<DIM, MODE, false>(p.G, g, p.n_twaves, p.K2, s);
   if (timing) (void)hipEventRecord(prof->ev[4], s);
   GFArgs gf{d.Gpart, d.trec_m, d.tm2o, d_grad, p.N, p.K2};
   hipLaunchKernelGGL((k_gfin<DIM>), dim3((unsigned)((p.N + 255) / 256)), dim3(256), 0, s, gf);

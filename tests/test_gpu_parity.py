@@ -115,6 +115,8 @@ def test_truncated_3d_sets_and_values(precision):
     else:
         assert abs(F - Fo) <= 1e-8 * abs(Fo)
         assert np.max(np.abs(g - go) / tgt.nu) <= 1e-6
+        # C_global relative to each q-point's largest term, as the reference sums it
+        assert abs(C - Co) <= 1e-6 * Co
 
 
 def test_fp32_small_eps_error():
@@ -302,32 +304,73 @@ def test_plain_2d_config_solve():
     """The reference's CPU-runnable configuration end to end: 64x64 Q1 / QGauss(3)
     on the unit square (36,864 q), 1,000 uniform targets (seed 0), eps = 1e-2,
     no truncation, FP64, l1-norm tolerance 1e-8 from psi = 0.  With the
-    reference's line search the GPU solve and the oracle's end the same way
-    (the functional's rounding noise stops both near ||g||_1 ~ 5e-8) after
-    nearly the same iterations at nearly the same potential; the noise-aware
-    search reaches the tolerance, confirmed on the oracle's gradient."""
+    reference's line search the GPU solve takes the oracle's decisions for the
+    first 60 iterations (same evaluation count, potentials and functional to
+    rounding).  Near ||g||_1 ~ 5e-8 the functional's rounding noise decides
+    the reference search on both sides (the oracle stops there with a failed
+    line search); the noise-aware search reaches the tolerance, confirmed on
+    the oracle's gradient."""
     say("plain 2D configuration")
     src = O.structured_source(2, [64, 64], [0.0, 0.0], [1.0, 1.0], 3)
     y = np.random.default_rng(0).random((1000, 2))
     tgt = O.Target(y.ravel(), np.full(1000, 1e-3))
     kw = dict(epsilon=1e-2, tau=0.0, tolerance=1e-8, solver_control_type="l1norm",
-              max_iterations=1000, use_log_sum_exp_trick=True, verbose_output=False)
-    s = make(src, tgt, **kw)
+              use_log_sum_exp_trick=True, verbose_output=False)
+    s = make(src, tgt, max_iterations=60, **kw)
     psi = s.solve()
     info = s.get_last_solve_info()
-    prm = O.params(epsilon=1e-2, tau=0.0, tolerance=1e-8, control="l1norm", max_iterations=1000,
+    prm = O.params(epsilon=1e-2, tau=0.0, tolerance=1e-8, control="l1norm", max_iterations=60,
                    use_lse=1)
     psi_o, info_o = O.solve(src, tgt, prm)
-    say(f"gpu {info['status']} it={info['iterations']} cv={info['check_value']:.3e}; oracle "
-        f"{info_o['status']} it={info_o['iterations']} cv={info_o['check_value']:.3e}")
-    assert info["status"] == info_o["status"]
-    assert abs(info["iterations"] - info_o["iterations"]) <= 5
-    assert np.max(np.abs((psi - psi.mean()) - (psi_o - psi_o.mean()))) < 1e-6
-    assert abs(info["functional"] - info_o["functional"]) <= 1e-12 * abs(info_o["functional"])
-    s.configure(sd.SolverParameters(line_search="noise_aware", **kw))
+    say(f"60 its: gpu {info['status']} ev={info['evaluations']} cv={info['check_value']:.3e}; "
+        f"oracle {info_o['status']} ev={info_o['evaluations']} cv={info_o['check_value']:.3e}")
+    assert info["status"] == info_o["status"] == "max_iterations"
+    assert info["evaluations"] == info_o["evaluations"]
+    assert np.max(np.abs(psi - psi_o)) < 1e-9
+    assert abs(info["functional"] - info_o["functional"]) <= 1e-13 * abs(info_o["functional"])
+    s.configure(sd.SolverParameters(line_search="noise_aware", max_iterations=1000, **kw))
     psi2 = s.solve()
     info2 = s.get_last_solve_info()
     _, g, _ = O.evaluate_R(src, tgt, prm, DMAX, psi2)
     say(f"noise-aware: {info2['status']} it={info2['iterations']} oracle ||g||_1 {np.abs(g).sum():.3e}")
     assert info2["status"] == "converged" and np.abs(g).sum() <= 1e-8 * 1.01
     s.close()
+
+
+@pytest.mark.parametrize("which", ["targets", "large_source"])
+def test_staged_sweeps_match_oracle(which):
+    """Target-record sets larger than an XCD's L2 stream through LDS in the
+    q-major sweeps: 70,000 targets, truncated; and a large source (551,368
+    quadrature points, whose q records outgrow the L2 of the target-major
+    sweep).  FP64 to 1e-10 (bitwise repeatable) and FP32 to 1e-6 of each
+    target's mass against the oracle, barycenters in FP64 to 1e-10."""
+    say(f"staged sweeps: {which}")
+    if which == "targets":
+        src, tgt = structured_case(3, 8, 2, 70000, seed=51, gmm_source=True, gmm_targets=True,
+                                   dirichlet=5)
+        eps, R = 1e-2, 0.15
+    else:
+        src, tgt = structured_case(3, 41, 2, 3000, seed=52, gmm_source=True, gmm_targets=True,
+                                   dirichlet=5)
+        eps, R = 5e-3, 0.1
+    psi = 0.002 * np.random.default_rng(5).standard_normal(tgt.n)
+    prm = O.params(epsilon=eps, use_lse=1)
+    Fo, go, Co = O.evaluate_R(src, tgt, prm, R, psi)
+    for precision in ("fp64", "fp32_lse"):
+        s = make(src, tgt, epsilon=eps, use_log_sum_exp_trick=True, precision=precision)
+        F, g, C = s.evaluate_functional_R(R, psi)
+        F2, g2, _ = s.evaluate_functional_R(R, psi)   # seeded FP32 evaluation
+        if precision == "fp64":
+            assert_fp64(F, g, Fo, go, tgt.nu)
+            assert F2 == F and np.array_equal(g2, g)
+            b = s.compute_weighted_barycenters_R(R, psi)
+            bo = O.barycenters_R(src, tgt, prm, R, psi)
+            assert np.max(np.abs(b - bo)) <= 1e-10
+        else:
+            # away from the solution a target's mass m_j can be many times nu_j;
+            # the FP32 error is relative to the mass (m = nu at the solution)
+            scale = np.maximum(tgt.nu, go + tgt.nu)
+            for FF, gg in ((F, g), (F2, g2)):
+                assert abs(FF - Fo) <= 1e-8 * abs(Fo)
+                assert np.max(np.abs(gg - go) / scale) <= 1e-6
+        s.close()
