@@ -39,36 +39,49 @@ def pass_peak_pairs(dim):
     return 1.0 / max(1.0 / EXP_PER_S, (dim + 3) / FMA_PER_S)
 
 
-def cpu_baseline(wl, psi, R, target_seconds=15.0):
+def cpu_baseline(wl, psi, R, target_seconds=15.0, extra=None):
     """Time the FP64 CPU restatement (oracle, reference algorithm structure:
     per-cell O(N) candidate scan, dense per-cell buffers, sqrt-then-square
-    distances, two-pass LSE) on a bounded sample of the same workload."""
+    distances, two-pass LSE) on a bounded sample of the same workload; with
+    `extra` = (workload, psi, R) also on a sample of that one (the large
+    unstructured workload)."""
     from oracle import oracle as O
-    src = wl.source
-    dim, nq = src.dim, src.nq
-    n_cells = src.n_cells
     cores = min(os.cpu_count() or 1, 16)
 
-    def sample(k):
-        idx = np.linspace(0, n_cells - 1, k).astype(np.int64)
-        x = src.x.reshape(n_cells, nq * dim)[idx].ravel()
-        w = src.w.reshape(n_cells, nq)[idx].ravel()
-        c = src.centers.reshape(n_cells, dim)[idx].ravel()
-        return O.Source(dim, nq, x, w, np.ones_like(w), c)
+    def sample(k, src_):
+        n_ = src_.n_cells
+        dim_, nq_ = src_.dim, src_.nq
+        idx = np.linspace(0, n_ - 1, k).astype(np.int64)
+        x = src_.x.reshape(n_, nq_ * dim_)[idx].ravel()
+        w = src_.w.reshape(n_, nq_)[idx].ravel()
+        c = src_.centers.reshape(n_, dim_)[idx].ravel()
+        return O.Source(dim_, nq_, x, w, np.ones_like(w), c)
 
-    tgt = O.Target(wl.y, wl.nu)
-    prm = O.params(epsilon=wl.epsilon, use_lse=1, n_threads=cores, faithful=True)
-    k = 64
-    while True:
-        s = sample(k)
-        t0 = time.perf_counter()
-        O.evaluate_R(s, tgt, prm, R, psi)
-        dt = time.perf_counter() - t0
-        if dt > target_seconds / 3 or k >= n_cells:
-            break
-        k = min(n_cells, int(k * max(2.0, target_seconds / max(dt, 1e-3) * 0.5)))
-    cnt, _ = O.candidate_stats(s, tgt, R)
-    pairs = int(cnt.sum()) * nq
+    def timed(wl_, psi_, R_):
+        src_ = wl_.source
+        n_ = src_.n_cells
+        tgt = O.Target(wl_.y, wl_.nu)
+        prm = O.params(epsilon=wl_.epsilon, use_lse=1, n_threads=cores, faithful=True)
+        k = 64
+        while True:
+            s = sample(k, src_)
+            t0 = time.perf_counter()
+            O.evaluate_R(s, tgt, prm, R_, psi_)
+            dt = time.perf_counter() - t0
+            if dt > target_seconds / 3 or k >= n_:
+                break
+            k = min(n_, int(k * max(2.0, target_seconds / max(dt, 1e-3) * 0.5)))
+        cnt, _ = O.candidate_stats(s, tgt, R_)
+        return k, n_, int(cnt.sum()) * src_.nq, dt
+
+    k, n_cells, pairs, dt = timed(wl, psi, R)
+    more = {}
+    if extra is not None:
+        k2, n2, pairs2, dt2 = timed(*extra)
+        more = {"large_unstructured": {
+            "value": pairs2 / dt2, "unit": "pair-evals/s",
+            "sample": f"{k2} of {n2} cells (evenly spaced), one FP64 evaluation, "
+                      f"{pairs2} pairs in {dt2:.2f} s"}}
     # the reference's CPU-runnable configuration solved end to end by the port
     # (the same L-BFGS and line search as the GPU leg)
     p2 = PLAIN_2D
@@ -85,7 +98,7 @@ def cpu_baseline(wl, psi, R, target_seconds=15.0):
             "plain_2d_solve": {"seconds": t_solve, "status": info2["status"],
                                "iterations": info2["iterations"],
                                "evaluations": info2["evaluations"],
-                               "check_value": info2["check_value"]}}
+                               "check_value": info2["check_value"]}, **more}
 
 
 # the reference's CPU-runnable configuration: 64x64 Q1 / QGauss(3), 1,000
@@ -460,7 +473,7 @@ def tets_leg(args):
     }
     log(f"tets: {json.dumps(out['fp32_vs_fp64'])}")
     s.close()
-    return out
+    return out, (wl, psi64, R)
 
 
 def main():
@@ -596,14 +609,14 @@ def main():
         log("multilevel leg ...")
         multilevel = multilevel_leg(args)
     plain2d = plain_2d_leg()
-    tets = None
+    tets, tets_cpu = None, None
     if args.tets:
         log("tets leg ...")
-        tets = tets_leg(args)
+        tets, tets_cpu = tets_leg(args)
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
-        cpu = cpu_baseline(wl, psi, R, args.cpu_seconds)
+        cpu = cpu_baseline(wl, psi, R, args.cpu_seconds, tets_cpu)
         log(f"cpu: {cpu['value']:.3e} pair-evals/s on {cpu['cores']} threads ({cpu['sample']})")
     out = {
         "metric": "pair-evals/s",
