@@ -221,7 +221,8 @@ def serving_leg(s, args):
               epsilon_scaling_steps=args.serve_eps_steps, epsilon_scaling_factor=2.0,
               preconditioner=args.serve_precond)
 
-    srv = sd.SotServer(s, precision="fp32_lse", max_slots=256, hbm_budget_bytes=96 << 30)
+    srv = sd.SotServer(s, precision="fp32_lse", max_slots=256, hbm_budget_bytes=96 << 30,
+                       polish_factor=args.serve_polish)
     t0 = srv.clock()
     pending = sorted(reqs, key=lambda r: r.arrival)
     inflight, done = {}, {}
@@ -298,6 +299,7 @@ def serving_leg(s, args):
                     "N logU[256,65536], eps {1e-2,3e-3,1e-3}, tol 1e-6 componentwise) on the "
                     "resident 32^3 Q1 / QGauss(2) mesh",
         "max_iterations": args.serve_max_it, "epsilon_scaling_steps": args.serve_eps_steps,
+        "polish_factor": args.serve_polish,
         "batched_fp32": {"solves_per_s": len(reqs) / span_b, "p50_s": float(np.percentile(lat_b, 50)),
                          "p99_s": float(np.percentile(lat_b, 99)), "within_tol": int(ok_b),
                          "max_fp64_rel_marginal_err": max(errs),
@@ -491,6 +493,8 @@ def main():
     ap.add_argument("--serve-max-it", type=int, default=1000)
     ap.add_argument("--serve-precond", default="diagonal", choices=["none", "diagonal"],
                     help="initial inverse Hessian of both serving legs")
+    ap.add_argument("--serve-polish", type=float, default=2.0,
+                    help="batched FP32 solves switch to FP64 within this factor of the tolerance")
     ap.add_argument("--serve-identity-leg", type=int, default=1,
                     help="also time the one-at-a-time FP64 path with the identity")
     ap.add_argument("--serve-eps-steps", type=int, default=1,

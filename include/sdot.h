@@ -287,7 +287,7 @@ typedef struct {
   double polish_factor;      /* FP32_LSE: a solve switches to FP64 evaluations once its
                                 stopping value is within this factor of the tolerance (or
                                 an FP32 line search fails), so that it stops on an FP64
-                                evaluation; 0: stay FP32 (default 10) */
+                                evaluation; 0: stay FP32 (default 2) */
 } sdot_server_config;
 
 typedef struct {

@@ -319,10 +319,17 @@ int prepare(sdot_ctx *c, double R) {
     p.n_groups = gr.n_groups;
     p.n_blocks = gr.n_blocks;
   }
-  // enough waves to fill 256 CUs: the q-major kernel runs 4 waves per SIMD, the
-  // target-major one up to 12
-  int64_t K1 = std::max<int64_t>(1, 8192 / std::max<int64_t>(1, p.n_blocks));
-  int64_t K2 = std::max<int64_t>(1, 16384 / std::max<int64_t>(1, p.n_twaves));
+  // enough one-wave workgroups for several rounds on the 1,024 SIMDs: the
+  // work per wave varies several-fold with the wave's position (candidate
+  // balls clipped by the domain), and only later rounds balance it (measured
+  // on the 32^3 / 50,000-target workload: 8 -> 32 q-major segments took the
+  // FP32 sum sweep from 4.0 to 3.4 ms, 16 -> 32 target-major segments the
+  // gradient sweep from 3.4 to 2.9 ms); the q-major PASS 1 reads every
+  // segment's maximum, which caps K1
+  int64_t K1 = std::min<int64_t>(32, (65536 + p.n_blocks - 1) / std::max<int64_t>(1, p.n_blocks));
+  int64_t K2 = (24576 + p.n_twaves - 1) / std::max<int64_t>(1, p.n_twaves);
+  K1 = std::max<int64_t>(1, K1);
+  K2 = std::max<int64_t>(1, K2);
   if (p.dense) {
     K1 = std::min<int64_t>(K1, std::max<int64_t>(1, p.N / 32));
     K2 = std::min<int64_t>(K2, std::max<int64_t>(1, p.n_cells / 8));

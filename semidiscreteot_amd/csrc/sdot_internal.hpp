@@ -249,7 +249,7 @@ constexpr double kNoiseRelF64 = 2e-15;
 constexpr double kNoiseRelF32 = 2e-9;
 // FP32_LSE solves switch to FP64 evaluations once their stopping value is
 // within this factor of the tolerance (the serving queue's default).
-constexpr double kPolishFactor = 10.0;
+constexpr double kPolishFactor = 2.0;
 
 // Softmax potential transfer from a coarse to a fine target level
 // (SoftmaxRefinement::compute_refinement, SoftmaxRefinement.cc:61-280), FP64.

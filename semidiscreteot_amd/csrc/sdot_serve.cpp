@@ -576,7 +576,7 @@ extern "C" int sdot_server_config_default(sdot_server_config *cfg) {
   cfg->use_graph = 1;
   cfg->pass_targets = 0;
   cfg->grad_qpoints = 0;
-  cfg->polish_factor = 10.0;
+  cfg->polish_factor = kPolishFactor;
   return SDOT_OK;
 }
 

@@ -46,7 +46,7 @@ class SotServer:
 
     def __init__(self, solver: SotSolver, precision: str = "fp32_lse", max_slots: int = 256,
                  hbm_budget_bytes: int = 16 << 30, use_graph: bool = True,
-                 pass_targets: int = 0, grad_qpoints: int = 0, polish_factor: float = 10.0):
+                 pass_targets: int = 0, grad_qpoints: int = 0, polish_factor: float = 2.0):
         self.lib = L.load()
         self.dim = solver.dim
         self._solver = solver  # keeps the source context alive

@@ -193,7 +193,7 @@ def test_server_config_defaults():
     cfg = L.ServerConfig()
     assert lib.sdot_server_config_default(C.byref(cfg)) == 0
     assert cfg.precision == 1 and cfg.max_slots == 256 and cfg.use_graph == 1
-    assert cfg.hbm_budget_bytes == 16 << 30 and cfg.polish_factor == 10.0
+    assert cfg.hbm_budget_bytes == 16 << 30 and cfg.polish_factor == 2.0
     out = C.c_void_p()
     assert lib.sdot_server_create(None, C.byref(cfg), C.byref(out)) != 0
     assert not out.value
