@@ -137,7 +137,8 @@ def plain_2d_leg():
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_eval_gmm3d_32cube_N50000.md")
 # kernel names of the FP32 evaluation's sweeps (any number of points per lane)
-KERNEL_KEYS = {"lse_max": r"k_pass1<3, \d, 2, 0>", "lse_sum": r"k_pass1<3, \d, 2, 2>",
+KERNEL_KEYS = {"lse_max": r"k_pass1<3, \d, 2, 0(, (false|true))?>",
+               "lse_sum": r"k_pass1<3, \d, 2, 2(, (false|true))?>",
                "grad": r"k_grad<3, 2, \d>"}
 
 

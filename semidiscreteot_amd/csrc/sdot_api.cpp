@@ -365,6 +365,7 @@ int prepare(sdot_ctx *c, double R) {
   }
   if (c->cap_qrec < (size_t)p.nQ * kRec) {
     HIP_TRY(dalloc(d.qrec_r, (size_t)p.nQ * kRec));
+    HIP_TRY(dalloc(d.Mglob, (size_t)p.nQ));
     c->cap_qrec = (size_t)p.nQ * kRec;
   }
   if (c->cap_F < (size_t)p.nFpart) {
@@ -629,7 +630,7 @@ extern "C" int sdot_ctx_destroy(sdot_ctx *c) {
                   d.cell_m2o, d.ty_o, d.tnu_o, d.tpos_r, d.tpos_m, d.tr2o, d.tm2o, d.tstart,
                   d.twbox, d.ty_m, d.trec_r, d.trec_m, d.qrec_r, d.Mpart, d.Spart, d.Gpart,
                   d.Fpart, d.Cpart, d.PNpart, d.psi, d.grad, d.out, d.scratch, d.i64scratch, d.Tpart,
-                  d.Upart, d.psi_prev, d.dpsi_r, d.dpsi_cx, d.dpsi_cn, d.ucount};
+                  d.Upart, d.psi_prev, d.dpsi_r, d.dpsi_cx, d.dpsi_cn, d.ucount, d.Mglob};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto &set : c->ev_pool)

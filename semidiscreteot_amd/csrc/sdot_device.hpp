@@ -418,7 +418,10 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
       M[k] = a.plain_coef * a.qx2[q0 + k];
     } else {
       double m = -INFINITY;
-      for (int s = 0; s < a.K1; ++s) m = fmax(m, a.Mpart[(int64_t)s * a.nQ + q0 + k]);
+      if (a.Mglob)
+        m = a.Mglob[q0 + k];
+      else
+        for (int s = 0; s < a.K1; ++s) m = fmax(m, a.Mpart[(int64_t)s * a.nQ + q0 + k]);
       M[k] = m;
     }
   }

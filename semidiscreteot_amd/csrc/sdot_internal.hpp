@@ -74,6 +74,7 @@ struct Dev {
   double *qrec_r = nullptr;    // [nQ*kRec] row cell order
   double *Mpart = nullptr;     // [K1*nQ]
   double *Spart = nullptr;     // [K1*nQ]
+  double *Mglob = nullptr;     // [nQ] FP64 LSE: the maximum over the segments
   double *Tpart = nullptr;     // [K1*nQ] FP32: segment maxima of the exponent
   double *Gpart = nullptr;     // [K2*N]
   double *Fpart = nullptr;     // [nFpart]
@@ -152,6 +153,7 @@ struct P1Args {
   const double *trec;    // [N*kRec] row order
   const int32_t *tstart;
   const double *Mpart;   // [K1*nQ]  maxima of PASS 0, read by PASS 1
+  const double *Mglob;   // [nQ]     their maximum over the segments (PASS 1; null: Mpart)
   const double *dpsi;    // [N]      FP32 history: exponent change per target, row order
   const double *dcx, *dcn;  // its maximum / minimum per chunk of 64 records (null: scan dpsi)
   double *Mout;          // [K1*nQ]  PASS 0: segment maxima; PASS 2: reference value used

@@ -57,6 +57,17 @@ __global__ __launch_bounds__(64) void k_pass1(P1Args a) {
   pass1_body<DIM, G, MODE, PASS, STAGED>(a, blockIdx.x, blockIdx.y);
 }
 
+// Maximum of the K1 segment maxima per quadrature point (FP64 LSE), read once
+// by each item of PASS 1 instead of every segment's.
+__global__ __launch_bounds__(256) void k_mmerge(const double *Mpart, int64_t nQ, int K1,
+                                                double *Mglob) {
+  const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (q >= nQ) return;
+  double m = -INFINITY;
+  for (int s = 0; s < K1; ++s) m = fmax(m, Mpart[(int64_t)s * nQ + q]);
+  Mglob[q] = m;
+}
+
 template <int DIM, int MODE>
 __global__ __launch_bounds__(256) void k_qfin(QFArgs a) {
   __shared__ double sh[256];
@@ -438,6 +449,9 @@ static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_g
   if (MODE == kF64Lse) {
     e = launch_pass1_g<DIM, MODE, 0>(p.G, a, p.n_blocks, p.K1, s, st1);
     if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_mmerge, dim3((unsigned)((p.nQ + 255) / 256)), dim3(256), 0, s,
+                       d.Mpart, p.nQ, p.K1, d.Mglob);
+    a.Mglob = d.Mglob;
   }
   if (timing) (void)hipEventRecord(prof->ev[1], s);
   if (MODE == kF32Lse)
