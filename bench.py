@@ -334,6 +334,7 @@ def multilevel_leg(args):
                               solver_control_type="componentwise",
                               max_iterations=args.ml_max_it, use_log_sum_exp_trick=True,
                               precision="fp32_lse", line_search="noise_aware",
+                              preconditioner=args.ml_precond,
                               use_epsilon_scaling=True, epsilon_scaling_factor=wl.epsilon_factor,
                               epsilon_scaling_steps=wl.epsilon_steps, quadrature_order=2,
                               verbose_output=True)
@@ -363,7 +364,8 @@ def multilevel_leg(args):
     dm = s.discarded_mass(R_last, psi, wl.sources[-1].w)
     log(f"multilevel: {total:.2f}s total, final FP64 relative marginal error {err:.3e} at R = "
         f"{R_last:.4f} (untruncated {err_full:.3e}, discarded mass {dm:.2e})")
-    return {"workload": wl.name, "seconds": total, "fp64_rel_marginal_err": err,
+    return {"workload": wl.name, "seconds": total, "preconditioner": args.ml_precond,
+            "fp64_rel_marginal_err": err,
             "final_R": R_last, "fp64_rel_marginal_err_untruncated": err_full,
             "discarded_mass": dm,
             "levels": [{"target_level": r.level, "n_targets": r.n_targets, "n_cells": r.n_cells,
@@ -504,6 +506,8 @@ def main():
                     help="also run the combined multilevel workload end to end")
     ap.add_argument("--ml-max-it", type=int, default=3000,
                     help="iteration budget of each multilevel solve")
+    ap.add_argument("--ml-precond", default="diagonal", choices=["none", "diagonal"],
+                    help="initial inverse Hessian of the multilevel solves")
     ap.add_argument("--tets", action="store_true",
                     help="also run the large unstructured (Delaunay ball) workload")
     ap.add_argument("--tets-max-it", type=int, default=2000)
