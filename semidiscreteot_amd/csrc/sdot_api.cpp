@@ -1112,6 +1112,9 @@ static int host_points(sdot_ctx *c, int op, const double *psi, double eps, doubl
   if (!c->have_target) return fail(SDOT_ERR_STATE, "Invalid measures configuration: no target");
   if (n < 0 || n >= (1ll << 40)) return fail(SDOT_ERR_INVALID, "point count out of range");
   if (n_idx < 0) return fail(SDOT_ERR_INVALID, "index count < 0");
+  // the density table has n * n_idx entries, one thread each
+  if (n_idx > 0 && n > ((1ll << 40) - 1) / n_idx)
+    return fail(SDOT_ERR_INVALID, "points x indices out of range");
   if (!psi || (n > 0 && (!points || !out)) || (op == 0 && n > 0 && (!density || (n_idx > 0 && !indices))))
     return fail(SDOT_ERR_INVALID, "null argument");
   if (!(eps > 0) || !std::isfinite(eps))

@@ -237,6 +237,14 @@ __device__ __forceinline__ void stage_stream(StageLds<W> &L, Produce &&produce, 
   int nr = 0;
   if (threadIdx.x == 0) L.cum[0] = 0;
   auto add = [&](int s, int e, bool tested) {
+    // the stream's offsets are int: flush before they could pass 2^30
+    if (nr > 0 && __builtin_amdgcn_readfirstlane(L.cum[nr]) > (1 << 30) - (e - s)) {
+      __syncthreads();
+      stage_flush(L, nr, fetch_item, fn);
+      __syncthreads();
+      nr = 0;
+      if (threadIdx.x == 0) L.cum[0] = 0;
+    }
     if (threadIdx.x == 0) {
       L.rs[nr] = s;
       L.rt[nr] = tested ? 1 : 0;
