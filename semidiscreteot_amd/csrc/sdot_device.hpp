@@ -193,6 +193,15 @@ struct StageLds {
   int rs[kStageRuns], rt[kStageRuns], cum[kStageRuns + 1];
 };
 
+// The workgroup's one staging area per item width: every stream of a kernel
+// shares it (a __shared__ object declared inside each caller would be
+// allocated once per caller).
+template <int W>
+__device__ __forceinline__ StageLds<W> &stage_lds() {
+  __shared__ StageLds<W> lds;
+  return lds;
+}
+
 template <int W, class Fetch, class Fn>
 __device__ __forceinline__ void stage_flush(StageLds<W> &L, int nr, Fetch &&fetch_item, Fn &&fn) {
   static_assert(W % 2 == 0, "items are moved as double2");
@@ -483,9 +492,8 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
   cdouble *trec = cptr(a.trec);
   auto records = [&](auto &&fn) {
     if constexpr (STAGED) {
-      __shared__ StageLds<kRec> stage;
       stage_stream(
-          stage,
+          stage_lds<kRec>(),
           [&](auto &add) {
             sweep([&](int j0, int j1, auto tag) { add(j0, j1, decltype(tag)::value); });
           },
