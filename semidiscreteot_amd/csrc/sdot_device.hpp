@@ -339,6 +339,20 @@ struct TRec {
     }
     nu = r[1 + 2 * DIM];
   }
+  // the whole record in one 64-byte scalar load, issued ahead of its use: the
+  // field loads of load() are placed at their uses, which puts one scalar
+  // round trip before the candidate test and another after it
+  __device__ __forceinline__ void load_whole(cdouble *r) {
+    typedef double d8 __attribute__((ext_vector_type(8)));
+    const d8 v = *reinterpret_cast<const __attribute__((address_space(4))) d8 *>(r);
+    a = v[0];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      yl[d] = v[1 + d];
+      y[d] = v[1 + DIM + d];
+    }
+    nu = v[1 + 2 * DIM];
+  }
 };
 
 // ---------------------------------------------------------------------------
@@ -500,14 +514,15 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
           FetchRec{a.trec}, fn);
     } else {
       sweep([&](int j0, int j1, auto tag) {
-        // records are fetched two at a time ahead of their use
-        for (int j = j0; j < j1; j += 2) {
-          const int j2 = (j + 1 < j1) ? j + 1 : j;
-          TRec<DIM> r0, r1;
-          r0.load(trec + (int64_t)j * kRec);
-          r1.load(trec + (int64_t)j2 * kRec);
-          fn(r0, decltype(tag)::value);
-          if (j + 1 < j1) fn(r1, decltype(tag)::value);
+        if (j0 >= j1) return;
+        // each record is fetched one record ahead of its use
+        TRec<DIM> cur;
+        cur.load_whole(trec + (int64_t)j0 * kRec);
+        for (int j = j0; j < j1; ++j) {
+          TRec<DIM> nxt;
+          nxt.load_whole(trec + (int64_t)min(j + 1, j1 - 1) * kRec);
+          fn(cur, decltype(tag)::value);
+          cur = nxt;
         }
       });
     }
