@@ -284,6 +284,33 @@ struct FetchRec {
   }
 };
 
+// Fold f over the K segment partials p[k * stride], k = 0 .. K-1, in index
+// order.  The loads of 16 partials are issued together: a plain loop waits
+// out one memory round trip per partial, which dominates the finishing
+// kernels of small grids.  Same operations in the same order as the plain
+// loop, so the result is bitwise the same.
+template <class F>
+__device__ __forceinline__ double fold_partials(const double *__restrict__ p, int64_t stride,
+                                                int K, double init, F &&f) {
+  constexpr int B = 16;
+  double r = init;
+  for (int k0 = 0; k0 < K; k0 += B) {
+    double v[B];
+#pragma unroll
+    for (int i = 0; i < B; ++i) v[i] = (k0 + i < K) ? p[(int64_t)(k0 + i) * stride] : 0.0;
+#pragma unroll
+    for (int i = 0; i < B; ++i)
+      if (k0 + i < K) r = f(r, v[i]);
+  }
+  return r;
+}
+struct FoldAdd {
+  __device__ __forceinline__ double operator()(double a, double b) const { return a + b; }
+};
+struct FoldMax {
+  __device__ __forceinline__ double operator()(double a, double b) const { return fmax(a, b); }
+};
+
 // Fixed-pattern block sum / max (blockDim.x == 256). Result valid in all threads.
 __device__ __forceinline__ double block_sum_256(double v, double *sh) {
   sh[threadIdx.x] = v;
@@ -624,20 +651,15 @@ __device__ __forceinline__ void qfin_body(const QFArgs &a, int64_t blk, double *
     double M, S = 0.0, T = 0.0;
     if (MODE == kF64Plain) {
       M = a.plain_coef * x2;
-      for (int s = 0; s < a.K1; ++s) S += a.Spart[(int64_t)s * a.nQ + qm];
+      S = fold_partials(a.Spart + qm, a.nQ, a.K1, 0.0, FoldAdd{});
     } else if (MODE == kF64Lse) {
-      M = -INFINITY;
-      for (int s = 0; s < a.K1; ++s) M = fmax(M, a.Mpart[(int64_t)s * a.nQ + qm]);
-      for (int s = 0; s < a.K1; ++s) S += a.Spart[(int64_t)s * a.nQ + qm];
+      M = fold_partials(a.Mpart + qm, a.nQ, a.K1, -INFINITY, FoldMax{});
+      S = fold_partials(a.Spart + qm, a.nQ, a.K1, 0.0, FoldAdd{});
     } else {
       // segments carry integer reference values: S = sum_s S_s 2^(M_s - M),
       // then renormalised to M = ceil(largest exponent) for the gradient pass
-      M = -INFINITY;
-      T = -INFINITY;
-      for (int s = 0; s < a.K1; ++s) {
-        M = fmax(M, a.Mpart[(int64_t)s * a.nQ + qm]);
-        T = fmax(T, a.Tpart[(int64_t)s * a.nQ + qm]);
-      }
+      M = fold_partials(a.Mpart + qm, a.nQ, a.K1, -INFINITY, FoldMax{});
+      T = fold_partials(a.Tpart + qm, a.nQ, a.K1, -INFINITY, FoldMax{});
       if (M > -INFINITY && T > -INFINITY) {
         for (int s = 0; s < a.K1; ++s) {
           const double ss = a.Spart[(int64_t)s * a.nQ + qm];
@@ -777,8 +799,7 @@ template <int DIM>
 __device__ __forceinline__ void gfin_body(const GFArgs &a, int64_t blk) {
   const int64_t j = blk * 256 + threadIdx.x;
   if (j >= a.N) return;
-  double s = 0.0;
-  for (int k = 0; k < a.K2; ++k) s += a.Gpart[(int64_t)k * a.N + j];
+  const double s = fold_partials(a.Gpart + j, a.N, a.K2, 0.0, FoldAdd{});
   const double nu = a.trec_m[j * kRec + 1 + 2 * DIM];
   a.grad[a.tm2o[j]] = nu * s - nu;
 }
@@ -791,14 +812,10 @@ __device__ __forceinline__ void bfin_body(const double *Gpart, const double *Bpa
                                           int K2, double *bary, int64_t blk) {
   const int64_t j = blk * 256 + threadIdx.x;
   if (j >= N) return;
-  double g = 0.0, b[DIM];
+  const double g = fold_partials(Gpart + j, N, K2, 0.0, FoldAdd{});
+  double b[DIM];
 #pragma unroll
-  for (int d = 0; d < DIM; ++d) b[d] = 0.0;
-  for (int k = 0; k < K2; ++k) {
-    g += Gpart[(int64_t)k * N + j];
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) b[d] += Bpart[((int64_t)k * DIM + d) * N + j];
-  }
+  for (int d = 0; d < DIM; ++d) b[d] = fold_partials(Bpart + d * N + j, DIM * N, K2, 0.0, FoldAdd{});
   const int64_t jo = tm2o[j];
 #pragma unroll
   for (int d = 0; d < DIM; ++d) bary[jo * DIM + d] = fma(o[d], g, b[d]);

@@ -63,9 +63,7 @@ __global__ __launch_bounds__(256) void k_mmerge(const double *Mpart, int64_t nQ,
                                                 double *Mglob) {
   const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (q >= nQ) return;
-  double m = -INFINITY;
-  for (int s = 0; s < K1; ++s) m = fmax(m, Mpart[(int64_t)s * nQ + q]);
-  Mglob[q] = m;
+  Mglob[q] = fold_partials(Mpart + q, nQ, K1, -INFINITY, FoldMax{});
 }
 
 template <int DIM, int MODE>

@@ -54,8 +54,7 @@ __global__ __launch_bounds__(64) void k_ref_coarse(RefArgs a) {
     hi[d] = wave_max(c[d]);
   }
   double M = -INFINITY, S = 0.0;
-  if (PASS == 1)
-    for (int s = 0; s < a.K1; ++s) M = fmax(M, a.Mpart[(int64_t)s * a.nQ + qq]);
+  if (PASS == 1) M = fold_partials(a.Mpart + qq, a.nQ, a.K1, -INFINITY, FoldMax{});
   cdouble *crec = cptr(a.crec);
   auto body = [&](int j0, int j1, auto) {
     for (int j = j0; j < j1; ++j) {
@@ -92,11 +91,8 @@ __global__ __launch_bounds__(256) void k_ref_qfin(RefArgs a) {
   if (qr >= a.nQ) return;
   const int64_t cr = qr / a.nq, k = qr - cr * a.nq;
   const int64_t qm = a.cell_r2m[cr] * a.nq + k;
-  double M = -INFINITY, S = 0.0;
-  for (int s = 0; s < a.K1; ++s) {
-    if (a.lse) M = fmax(M, a.Mpart[(int64_t)s * a.nQ + qm]);
-    S += a.Spart[(int64_t)s * a.nQ + qm];
-  }
+  const double M = a.lse ? fold_partials(a.Mpart + qm, a.nQ, a.K1, -INFINITY, FoldMax{}) : -INFINITY;
+  const double S = fold_partials(a.Spart + qm, a.nQ, a.K1, 0.0, FoldAdd{});
   // q-points whose sum is <= 0 contribute nothing (SoftmaxRefinement.cc:181)
   double scale = 0.0;
   if (!(S <= 0.0)) {
