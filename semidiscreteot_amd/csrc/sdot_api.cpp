@@ -343,10 +343,6 @@ int prepare(sdot_ctx *c, double R) {
   p.K2 = (int)std::min<int64_t>(K2, kMaxSeg);
   // the gradient grid gives each XCD the segments s = xcd (mod 8)
   if (p.K2 >= 8) p.K2 -= p.K2 % 8;
-  // truncated q-major sweeps whose target records outgrow an XCD's 4 MB L2
-  // stream them through LDS
-  constexpr double kL2 = 4.0 * 1024 * 1024;
-  p.staged1 = (!p.dense && (double)p.N * kRec * 8 > kL2) ? 1 : 0;
   p.nFpart = (p.nQ + 255) / 256;
   p.nPpart = (p.N + 255) / 256;
   Dev &d = c->dev;

@@ -175,115 +175,6 @@ __device__ __forceinline__ void for_rows(const Grid &g, const int32_t *__restric
   }
 }
 
-// Item stream of a one-wave workgroup staged through LDS, for record sets
-// that outgrow the XCD's L2 (there the scalar record loads of the plain sweeps
-// wait out a memory round trip each).  The runs of item indices that for_rows
-// hands out are buffered (start, end, tag), and their items are streamed in
-// tiles of kWave: lane l fetches item l of the next tile (W doubles, with
-// coalesced 16-byte loads) into registers while the wave computes the current
-// tile from LDS, so the fetch latency hides behind a tile of compute.
-// fn(item, tested) sees the items in run order, as the runs were added, so
-// per-lane sums keep the plain sweeps' order.
-constexpr int kStageRuns = 128;
-
-template <int W>
-struct StageLds {
-  double item[kWave * W];
-  int tag[kWave];  // 1: the item needs the candidate test
-  int rs[kStageRuns], rt[kStageRuns], cum[kStageRuns + 1];
-};
-
-// The workgroup's one staging area per item width: every stream of a kernel
-// shares it (a __shared__ object declared inside each caller would be
-// allocated once per caller).
-template <int W>
-__device__ __forceinline__ StageLds<W> &stage_lds() {
-  __shared__ StageLds<W> lds;
-  return lds;
-}
-
-template <int W, class Fetch, class Fn>
-__device__ __forceinline__ void stage_flush(StageLds<W> &L, int nr, Fetch &&fetch_item, Fn &&fn) {
-  static_assert(W % 2 == 0, "items are moved as double2");
-  const int lane = threadIdx.x;
-  const int total = __builtin_amdgcn_readfirstlane(L.cum[nr]);
-  if (total <= 0) return;
-  double2 buf[W / 2];
-  int btag = 0;
-  int khint = 0;  // run holding the tile's first item (wave-uniform)
-  auto fetch = [&](int f0) {
-    const int f = f0 + lane;
-    if (f < total) {
-      int k = khint;
-      while (L.cum[k + 1] <= f) ++k;
-      const int j = L.rs[k] + (f - L.cum[k]);
-      fetch_item(j, buf);
-      btag = L.rt[k];
-    }
-  };
-  fetch(0);
-  for (int f0 = 0; f0 < total; f0 += kWave) {
-    __syncthreads();  // the previous tile has been read
-    double2 *dst = reinterpret_cast<double2 *>(&L.item[lane * W]);
-#pragma unroll
-    for (int i = 0; i < W / 2; ++i) dst[i] = buf[i];
-    L.tag[lane] = btag;
-    __syncthreads();
-    const int m = min(kWave, total - f0);
-    if (f0 + kWave < total) {
-      while (__builtin_amdgcn_readfirstlane(L.cum[khint + 1]) <= f0 + kWave) ++khint;
-      fetch(f0 + kWave);
-    }
-    for (int r = 0; r < m; ++r) fn(&L.item[r * W], L.tag[r] != 0);
-  }
-}
-
-// Streams the items of the runs produce(add) adds (add(start, end, tested))
-// through L, flushing every kStageRuns runs.
-template <int W, class Produce, class Fetch, class Fn>
-__device__ __forceinline__ void stage_stream(StageLds<W> &L, Produce &&produce, Fetch &&fetch_item,
-                                             Fn &&fn) {
-  int nr = 0;
-  if (threadIdx.x == 0) L.cum[0] = 0;
-  auto add = [&](int s, int e, bool tested) {
-    // the stream's offsets are int: flush before they could pass 2^30
-    if (nr > 0 && __builtin_amdgcn_readfirstlane(L.cum[nr]) > (1 << 30) - (e - s)) {
-      __syncthreads();
-      stage_flush(L, nr, fetch_item, fn);
-      __syncthreads();
-      nr = 0;
-      if (threadIdx.x == 0) L.cum[0] = 0;
-    }
-    if (threadIdx.x == 0) {
-      L.rs[nr] = s;
-      L.rt[nr] = tested ? 1 : 0;
-      L.cum[nr + 1] = L.cum[nr] + (e - s);
-    }
-    ++nr;
-    if (nr == kStageRuns) {
-      __syncthreads();
-      stage_flush(L, nr, fetch_item, fn);
-      __syncthreads();
-      nr = 0;
-      if (threadIdx.x == 0) L.cum[0] = 0;
-    }
-  };
-  produce(add);
-  __syncthreads();
-  stage_flush(L, nr, fetch_item, fn);
-  __syncthreads();
-}
-
-// Item fetch of the q-major sweeps: one target record.
-struct FetchRec {
-  const double *__restrict__ rec;
-  __device__ __forceinline__ void operator()(int j, double2 *buf) const {
-    const double2 *src = reinterpret_cast<const double2 *>(rec + (int64_t)j * kRec);
-#pragma unroll
-    for (int i = 0; i < kRec / 2; ++i) buf[i] = src[i];
-  }
-};
-
 // Fold f over the K segment partials p[k * stride], k = 0 .. K-1, in index
 // order.  The loads of 16 partials are issued together: a plain loop waits
 // out one memory round trip per partial, which dominates the finishing
@@ -451,8 +342,7 @@ __device__ __forceinline__ void chunkext_body(const double *dpsi_r, int64_t N, d
 // k_pass1: q-major sweeps
 // ---------------------------------------------------------------------------
 
-// STAGED: the records stream through LDS (record sets larger than an L2).
-template <int DIM, int G, int MODE, int PASS, bool STAGED = false>
+template <int DIM, int G, int MODE, int PASS>
 __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg) {
   const int lane = threadIdx.x;
   const int64_t grp = blk * kWave + lane;
@@ -528,42 +418,21 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
       for_rows<DIM>(a.tg, a.tstart, lo, hi, a.Rq, a.s_in, seg, a.K1, f);
     }
   };
-  // the item's candidate records: scalar loads (records resident in L2), or
-  // staged through LDS (record sets larger than an XCD's L2)
+  // the item's candidate records, with scalar loads
   cdouble *trec = cptr(a.trec);
   auto records = [&](auto &&fn) {
-    if constexpr (STAGED) {
-      stage_stream(
-          stage_lds<kRec>(),
-          [&](auto &add) {
-            sweep([&](int j0, int j1, auto tag) { add(j0, j1, decltype(tag)::value); });
-          },
-          FetchRec{a.trec}, fn);
-    } else {
-      sweep([&](int j0, int j1, auto tag) {
-        if (j0 >= j1) return;
-        // each record is fetched one record ahead of its use
-        TRec<DIM> cur;
-        cur.load_whole(trec + (int64_t)j0 * kRec);
-        for (int j = j0; j < j1; ++j) {
-          TRec<DIM> nxt;
-          nxt.load_whole(trec + (int64_t)min(j + 1, j1 - 1) * kRec);
-          fn(cur, decltype(tag)::value);
-          cur = nxt;
-        }
-      });
-    }
-  };
-  auto as_rec = [](auto &&f) {
-    return [f](const auto &src, bool tested) {
-      if constexpr (std::is_same_v<std::decay_t<decltype(src)>, TRec<DIM>>) {
-        f(src, tested);
-      } else {
-        TRec<DIM> r;
-        r.load(src);
-        f(r, tested);
+    sweep([&](int j0, int j1, auto tag) {
+      if (j0 >= j1) return;
+      // each record is fetched one record ahead of its use
+      TRec<DIM> cur;
+      cur.load_whole(trec + (int64_t)j0 * kRec);
+      for (int j = j0; j < j1; ++j) {
+        TRec<DIM> nxt;
+        nxt.load_whole(trec + (int64_t)min(j + 1, j1 - 1) * kRec);
+        fn(cur, decltype(tag)::value);
+        cur = nxt;
       }
-    };
+    });
   };
 
   if (PASS == 2) {
@@ -600,7 +469,7 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
       for (int k = 0; k < G; ++k) M[k] = ceil(a.Tout[(int64_t)seg * a.nQ + q0 + k] + dmx);
     } else {
       // exact maximum over the item's candidates first
-      records(as_rec([&](const TRec<DIM> &r, bool tested) {
+      records([&](const TRec<DIM> &r, bool tested) {
         if (tested && !in_set<DIM>(r.y, c, a.smax)) return;
 #pragma unroll
         for (int k = 0; k < G; ++k) {
@@ -609,7 +478,7 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
           for (int d = 0; d < DIM; ++d) t = fma(x[k][d], r.yl[d], t);
           T[k] = vmax64(T[k], t);
         }
-      }));
+      });
 #pragma unroll
       for (int k = 0; k < G; ++k) {
         M[k] = ceil(T[k]);
@@ -617,7 +486,7 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
       }
     }
   }
-  records(as_rec(one));
+  records(one);
   if (active) {
 #pragma unroll
     for (int k = 0; k < G; ++k) {

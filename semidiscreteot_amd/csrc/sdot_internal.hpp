@@ -124,7 +124,6 @@ struct Plan {
   int dense = 0;       // every pair is a candidate (no enumeration, no test)
   // the q-major sweeps stream the target records through LDS (record sets
   // larger than one XCD's L2)
-  int staged1 = 0;
 };
 
 

@@ -52,9 +52,9 @@ __global__ __launch_bounds__(256) void k_chunkext(const double *dpsi_r, int64_t 
   chunkext_body(dpsi_r, N, dcx, dcn, (int64_t)blockIdx.x * (256 / kWave) + threadIdx.x / kWave);
 }
 
-template <int DIM, int G, int MODE, int PASS, bool STAGED>
+template <int DIM, int G, int MODE, int PASS>
 __global__ __launch_bounds__(64) void k_pass1(P1Args a) {
-  pass1_body<DIM, G, MODE, PASS, STAGED>(a, blockIdx.x, blockIdx.y);
+  pass1_body<DIM, G, MODE, PASS>(a, blockIdx.x, blockIdx.y);
 }
 
 // Maximum of the K1 segment maxima per quadrature point (FP64 LSE), read once
@@ -322,26 +322,14 @@ hipError_t launch_hessian_final(const Plan &p, const double *d_A, int64_t Npad, 
 // launchers
 // ---------------------------------------------------------------------------
 
-template <int DIM, int G, int MODE, int PASS, bool STAGED = false>
+template <int DIM, int G, int MODE, int PASS>
 static void launch_pass1_t(const P1Args &a, int64_t nb, int K1, hipStream_t s) {
-  hipLaunchKernelGGL((k_pass1<DIM, G, MODE, PASS, STAGED>), dim3((unsigned)nb, (unsigned)K1),
-                     dim3(64), 0, s, a);
+  hipLaunchKernelGGL((k_pass1<DIM, G, MODE, PASS>), dim3((unsigned)nb, (unsigned)K1), dim3(64), 0,
+                     s, a);
 }
 
-// staged: the truncated sweeps' records stream through LDS (their groupings
-// have at most 4 points per lane)
 template <int DIM, int MODE, int PASS>
-static hipError_t launch_pass1_g(int G, const P1Args &a, int64_t nb, int K1, hipStream_t s,
-                                 bool staged = false) {
-  if (staged && G <= 4) {
-    switch (G) {
-      case 1: launch_pass1_t<DIM, 1, MODE, PASS, true>(a, nb, K1, s); break;
-      case 2: launch_pass1_t<DIM, 2, MODE, PASS, true>(a, nb, K1, s); break;
-      case 3: launch_pass1_t<DIM, 3, MODE, PASS, true>(a, nb, K1, s); break;
-      default: launch_pass1_t<DIM, 4, MODE, PASS, true>(a, nb, K1, s); break;
-    }
-    return hipGetLastError();
-  }
+static hipError_t launch_pass1_g(int G, const P1Args &a, int64_t nb, int K1, hipStream_t s) {
   switch (G) {
     case 1: launch_pass1_t<DIM, 1, MODE, PASS>(a, nb, K1, s); break;
     case 2: launch_pass1_t<DIM, 2, MODE, PASS>(a, nb, K1, s); break;
@@ -443,9 +431,8 @@ static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_g
   a.Tout = d.Tpart;
   hipError_t e;
   // exact maxima first (FP64 LSE; the FP32 sweep finds its own bounds)
-  const bool st1 = p.staged1 != 0;
   if (MODE == kF64Lse) {
-    e = launch_pass1_g<DIM, MODE, 0>(p.G, a, p.n_blocks, p.K1, s, st1);
+    e = launch_pass1_g<DIM, MODE, 0>(p.G, a, p.n_blocks, p.K1, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_mmerge, dim3((unsigned)((p.nQ + 255) / 256)), dim3(256), 0, s,
                        d.Mpart, p.nQ, p.K1, d.Mglob);
@@ -453,9 +440,9 @@ static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_g
   }
   if (timing) (void)hipEventRecord(prof->ev[1], s);
   if (MODE == kF32Lse)
-    e = launch_pass1_g<DIM, MODE, 2>(p.G, a, p.n_blocks, p.K1, s, st1);
+    e = launch_pass1_g<DIM, MODE, 2>(p.G, a, p.n_blocks, p.K1, s);
   else
-    e = launch_pass1_g<DIM, MODE, 1>(p.G, a, p.n_blocks, p.K1, s, st1);
+    e = launch_pass1_g<DIM, MODE, 1>(p.G, a, p.n_blocks, p.K1, s);
   if (e != hipSuccess) return e;
   if (timing) (void)hipEventRecord(prof->ev[2], s);
 

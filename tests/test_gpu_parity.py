@@ -353,8 +353,8 @@ def test_plain_2d_config_solve():
 
 @pytest.mark.parametrize("which", ["targets", "large_source"])
 def test_staged_sweeps_match_oracle(which):
-    """Target-record sets larger than an XCD's L2 stream through LDS in the
-    q-major sweeps: 70,000 targets, truncated; and a large source (551,368
+    """Record sets larger than an XCD's L2: 70,000 targets (4.5 MB of target
+    records for the q-major sweeps), truncated; and a large source (551,368
     quadrature points, whose q records outgrow the L2 of the target-major
     sweep).  FP64 to 1e-10 (bitwise repeatable) and FP32 to 1e-6 of each
     target's mass against the oracle, barycenters in FP64 to 1e-10."""
