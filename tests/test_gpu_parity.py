@@ -352,13 +352,13 @@ def test_plain_2d_config_solve():
 
 
 @pytest.mark.parametrize("which", ["targets", "large_source"])
-def test_staged_sweeps_match_oracle(which):
+def test_large_record_sets_match_oracle(which):
     """Record sets larger than an XCD's L2: 70,000 targets (4.5 MB of target
     records for the q-major sweeps), truncated; and a large source (551,368
     quadrature points, whose q records outgrow the L2 of the target-major
     sweep).  FP64 to 1e-10 (bitwise repeatable) and FP32 to 1e-6 of each
     target's mass against the oracle, barycenters in FP64 to 1e-10."""
-    say(f"staged sweeps: {which}")
+    say(f"large record sets: {which}")
     if which == "targets":
         src, tgt = structured_case(3, 8, 2, 70000, seed=51, gmm_source=True, gmm_targets=True,
                                    dirichlet=5)
