@@ -530,9 +530,19 @@ __device__ __forceinline__ void qfin_body(const QFArgs &a, int64_t blk, double *
       M = fold_partials(a.Mpart + qm, a.nQ, a.K1, -INFINITY, FoldMax{});
       T = fold_partials(a.Tpart + qm, a.nQ, a.K1, -INFINITY, FoldMax{});
       if (M > -INFINITY && T > -INFINITY) {
-        for (int s = 0; s < a.K1; ++s) {
-          const double ss = a.Spart[(int64_t)s * a.nQ + qm];
-          if (ss > 0.0) S += ldexp(ss, (int)(a.Mpart[(int64_t)s * a.nQ + qm] - M));
+        // segment order, with the loads of 8 segments issued together
+        constexpr int B = 8;
+        for (int s0 = 0; s0 < a.K1; s0 += B) {
+          double sv[B], mv[B];
+#pragma unroll
+          for (int i = 0; i < B; ++i) {
+            const bool in = s0 + i < a.K1;
+            sv[i] = in ? a.Spart[(int64_t)(s0 + i) * a.nQ + qm] : 0.0;
+            mv[i] = in ? a.Mpart[(int64_t)(s0 + i) * a.nQ + qm] : 0.0;
+          }
+#pragma unroll
+          for (int i = 0; i < B; ++i)
+            if (sv[i] > 0.0) S += ldexp(sv[i], (int)(mv[i] - M));
         }
         const double Mt = ceil(T);
         S = ldexp(S, (int)(M - Mt));
