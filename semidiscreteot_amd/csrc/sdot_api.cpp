@@ -329,12 +329,14 @@ int prepare(sdot_ctx *c, double R) {
   // items (5.4M q-points / 500,000 targets: 4 -> 16 q-major segments took the
   // FP32 sum sweep from 26.9 to 21.6 ms, 4 -> 64 target-major segments the
   // gradient sweep from 50.8 to 24.1 ms), hence the floors of 16 and 64; the
-  // q-major partials (3 K1 nQ doubles) stay below 2^28 entries per array
+  // segment partials (3 K1 nQ and K2 N doubles) stay below 2^28 entries per
+  // array
   int64_t K1 = std::min<int64_t>(32, (65536 + p.n_blocks - 1) / std::max<int64_t>(1, p.n_blocks));
   int64_t K2 = (24576 + p.n_twaves - 1) / std::max<int64_t>(1, p.n_twaves);
   K1 = std::max<int64_t>(16, K1);
   K1 = std::max<int64_t>(1, std::min<int64_t>(K1, (1ll << 28) / std::max<int64_t>(1, p.nQ)));
   K2 = std::max<int64_t>(64, K2);
+  K2 = std::max<int64_t>(1, std::min<int64_t>(K2, (1ll << 28) / std::max<int64_t>(1, p.N)));
   if (p.dense) {
     K1 = std::min<int64_t>(K1, std::max<int64_t>(1, p.N / 32));
     K2 = std::min<int64_t>(K2, std::max<int64_t>(1, p.n_cells / 8));
