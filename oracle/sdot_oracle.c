@@ -25,8 +25,18 @@
  *                             (tests/golden), plain vs LSE mode agreement.
  *   orc_candidate_stats       brute-force Python scan with the same rule.
  *   orc_distance_threshold    closed-form evaluation of the three formulas.
+ *   orc_covering_radius       numpy brute force; one central target (closed
+ *                             form sqrt(3)(1/2 - 1/2n)); targets at the centers.
+ *   orc_discarded_fraction    numpy brute force; R = 10 and R < 0 limits; two
+ *                             symmetric targets (closed-form logistic share).
  *   orc_hessian               FD of the gradient, H*1 = 0, symmetry, PSD.
- *   orc_bfgs_generic/solve    quadratic (-> A^-1 b) and Rosenbrock minima,
+ *   orc_bfgs_generic/solve    hand-derived line-search traces on scripted
+ *                             objectives (acceptance, growth 9, safeguards
+ *                             0.1/0.5, cubic / quadratic / bisection, first
+ *                             step 2.02, eta 0.9, mu 1e-3, 20-evaluation cap)
+ *                             and the two-loop direction against the explicit
+ *                             BFGS matrix update (tests/test_line_search_pins.py);
+ *                             quadratic (-> A^-1 b) and Rosenbrock minima,
  *                             converged SOT gradients below tolerance.
  *   orc_barycenters_R         one target: the mean of the source (sum w x);
  *                             a shift of every point shifts the barycenters;

@@ -9,7 +9,7 @@ from ._lib import LIB_PATH, SdotError, load  # noqa: F401
 from .mesh import SourceArrays, simplex_source, structured_source  # noqa: F401
 from .solver import SolverParameters, SotSolver  # noqa: F401
 from .server import ServeResult, SotServer  # noqa: F401
-from .epsilon_scaling import EpsilonScalingHandler  # noqa: F401
+from .epsilon_scaling import epsilon_schedule, level_schedules  # noqa: F401
 from .lloyd import run_lloyd  # noqa: F401
 from .transport_plan import OptimalTransportPlan  # noqa: F401
 from .multilevel import (MultilevelParameters, SoftmaxRefinement,  # noqa: F401
@@ -17,7 +17,7 @@ from .multilevel import (MultilevelParameters, SoftmaxRefinement,  # noqa: F401
                          run_target_multilevel)
 
 __all__ = ["SotSolver", "SolverParameters", "SotServer", "ServeResult", "SourceArrays", "structured_source",
-           "simplex_source", "SdotError", "load", "LIB_PATH", "EpsilonScalingHandler",
+           "simplex_source", "SdotError", "load", "LIB_PATH", "epsilon_schedule", "level_schedules",
            "SoftmaxRefinement", "MultilevelParameters", "generate_target_hierarchy",
            "run_target_multilevel", "run_combined_multilevel", "run_lloyd",
            "OptimalTransportPlan"]

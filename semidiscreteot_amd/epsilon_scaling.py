@@ -1,92 +1,59 @@
-"""EpsilonScalingHandler (include/SemiDiscreteOT/solvers/EpsilonScalingHandler.h,
-src/solvers/EpsilonScalingHandler.cc): the epsilon schedule of a solve and its
-distribution over the levels of a multilevel run.  Host-side bookkeeping; the
-solves themselves run on the GPU."""
+"""Epsilon schedules of a regularized solve and their split over the levels of
+a multilevel run (the schedule SemiDiscreteOT::run_sot and the multilevel
+drivers take from EpsilonScalingHandler, src/solvers/EpsilonScalingHandler.cc:
+6-100).  Host-side arithmetic only; the solves run on the GPU.
+
+Two rules fix the numbers and are kept for parity:
+  * a schedule of n stages ends at the requested epsilon and grows by the
+    factor towards the first stage: stage i solves at eps * factor^(n-1-i);
+    a factor <= 1 means 2 and 0 stages mean 1 (EpsilonScalingHandler.cc:18-27);
+  * over L levels (coarsest first) a schedule of at most L values gives one
+    value per level, the last value repeated on the levels beyond it; a longer
+    one gives each level in turn ceil(values left / levels left) values
+    (EpsilonScalingHandler.cc:44-90).
+"""
 from __future__ import annotations
 
-import sys
+
+def normalized_scaling(factor: float, steps: int) -> tuple[float, int]:
+    """(factor, steps) after the fallbacks: factor <= 1 -> 2, steps 0 -> 1."""
+    f = float(factor)
+    n = int(steps)
+    return (f if f > 1.0 else 2.0), (n if n != 0 else 1)
 
 
-def _warn(msg: str):
-    print(msg, file=sys.stderr)
+def epsilon_schedule(epsilon: float, factor: float, steps: int) -> list[float]:
+    """Stage values from the largest to the requested epsilon."""
+    f, n = normalized_scaling(factor, steps)
+    eps = float(epsilon)
+    return [eps * f ** float(n - 1 - i) for i in range(n)]
 
 
-class EpsilonScalingHandler:
-    def __init__(self, initial_epsilon: float, scaling_factor: float, num_steps: int):
-        """EpsilonScalingHandler.cc:6-28: a factor <= 1 falls back to 2 and
-        zero steps to 1, with a warning."""
-        self.initial_epsilon = float(initial_epsilon)
-        self.scaling_factor = float(scaling_factor)
-        self.num_steps = int(num_steps)
-        if self.scaling_factor <= 1.0:
-            _warn("Warning: Epsilon scaling factor should be > 1.0. Using default value of 2.0.")
-            self.scaling_factor = 2.0
-        if self.num_steps == 0:
-            _warn("Warning: Number of epsilon scaling steps cannot be 0. Using default value of 1.")
-            self.num_steps = 1
-        self.epsilon_distribution: list[list[float]] = []
+def split_schedule(schedule, n_levels: int) -> list[list[float]]:
+    """The stages each level solves, coarsest level first."""
+    vals = [float(v) for v in schedule]
+    if n_levels <= 0:
+        return []
+    if not vals:
+        return [[] for _ in range(n_levels)]
+    if len(vals) <= n_levels:
+        return [[vals[min(k, len(vals) - 1)]] for k in range(n_levels)]
+    out, pos = [], 0
+    for k in range(n_levels):
+        levels_left = n_levels - k
+        take = -(-(len(vals) - pos) // levels_left)
+        out.append(vals[pos:pos + take])
+        pos += take
+    return out
 
-    def generate_epsilon_sequence(self) -> list[float]:
-        """EpsilonScalingHandler.cc:30-42: largest to smallest,
-        initial_epsilon * factor^(steps - 1 - i)."""
-        n = self.num_steps
-        return [self.initial_epsilon * self.scaling_factor ** float(n - 1 - i) for i in range(n)]
 
-    def distribute_epsilon_values(self, epsilon_sequence, num_levels: int) -> list[list[float]]:
-        """EpsilonScalingHandler.cc:44-90: one value per level from the
-        coarsest (the smallest repeated on the remaining levels) when there
-        are no more values than levels; otherwise ceil(remaining values /
-        remaining levels) values per level, coarsest first."""
-        dist: list[list[float]] = [[] for _ in range(num_levels)]
-        if num_levels == 0:
-            _warn("Warning: No levels provided for epsilon distribution.")
-            return dist
-        seq = list(epsilon_sequence)
-        if len(seq) <= num_levels:
-            for level in range(num_levels):
-                dist[level].append(seq[level] if level < len(seq) else seq[-1])
-        else:
-            remaining_eps, remaining_levels, k = len(seq), num_levels, 0
-            for level in range(num_levels):
-                n_here = (remaining_eps + remaining_levels - 1) // remaining_levels
-                for _ in range(n_here):
-                    if k < len(seq):
-                        dist[level].append(seq[k])
-                        k += 1
-                remaining_eps -= n_here
-                remaining_levels -= 1
-        return dist
+def level_schedules(epsilon: float, factor: float, steps: int, n_levels: int) -> list[list[float]]:
+    """epsilon_schedule split over n_levels levels."""
+    return split_schedule(epsilon_schedule(epsilon, factor, steps), n_levels)
 
-    def compute_epsilon_distribution(self, num_levels: int) -> list[list[float]]:
-        """EpsilonScalingHandler.cc:92-100."""
-        self.epsilon_distribution = self.distribute_epsilon_values(
-            self.generate_epsilon_sequence(), num_levels)
-        return self.epsilon_distribution
 
-    def get_epsilon_values_for_level(self, level_index: int) -> list[float]:
-        """EpsilonScalingHandler.cc:102-114: empty (with a warning) for an
-        index beyond the distribution."""
-        if level_index >= len(self.epsilon_distribution):
-            _warn(f"Warning: Requested epsilon values for invalid level index: {level_index}")
-            return []
-        return self.epsilon_distribution[level_index]
-
-    def print_epsilon_distribution(self, out=sys.stdout):
-        """EpsilonScalingHandler.cc:116-159."""
-        if not self.epsilon_distribution:
-            print("Epsilon distribution has not been computed yet.", file=out)
-            return
-        n = len(self.epsilon_distribution)
-        print("\n----------------------------------------", file=out)
-        print("Epsilon Scaling Distribution:", file=out)
-        print("----------------------------------------", file=out)
-        print(f"Initial epsilon: {self.initial_epsilon:g}", file=out)
-        print(f"Scaling factor: {self.scaling_factor:g}", file=out)
-        print(f"Number of steps: {self.num_steps}", file=out)
-        print(f"Number of levels: {n}", file=out)
-        print("----------------------------------------", file=out)
-        for level, eps in enumerate(self.epsilon_distribution):
-            kind = "coarsest" if level == 0 else ("finest" if level == n - 1 else "intermediate")
-            vals = ", ".join(f"{e:.4e}" for e in eps)
-            print(f"Level {n - 1 - level} ({kind}): {len(eps)} epsilon value(s): {vals}", file=out)
-        print("----------------------------------------", file=out)
+def stages_of_level(schedules: list[list[float]], level: int, fallback: float) -> list[float]:
+    """The stages of one level, or [fallback] where the split gives none."""
+    if 0 <= level < len(schedules) and schedules[level]:
+        return list(schedules[level])
+    return [float(fallback)]

@@ -12,7 +12,7 @@ from dataclasses import dataclass, fields
 
 import numpy as np
 
-from .epsilon_scaling import EpsilonScalingHandler
+from .epsilon_scaling import level_schedules
 from .solver import SolverParameters, SotSolver, _f64
 
 
@@ -36,8 +36,8 @@ def run_lloyd(solver: SotSolver, target_points, target_density, params: SolverPa
     cur = SolverParameters(**{f.name: getattr(params, f.name) for f in fields(params)})
     seq = [params.epsilon]
     if params.use_epsilon_scaling:
-        seq = EpsilonScalingHandler(params.epsilon, params.epsilon_scaling_factor,
-                                    params.epsilon_scaling_steps).compute_epsilon_distribution(1)[0]
+        seq = level_schedules(params.epsilon, params.epsilon_scaling_factor,
+                              params.epsilon_scaling_steps, 1)[0]
     records = []
     psi = None
     for n_iter in range(max_iterations):

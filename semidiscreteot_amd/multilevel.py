@@ -23,7 +23,7 @@ import numpy as np
 
 from . import _lib as L
 from . import formats
-from .epsilon_scaling import EpsilonScalingHandler
+from .epsilon_scaling import level_schedules, stages_of_level
 from .solver import SolverParameters, SotSolver, _f64, _p
 
 
@@ -68,8 +68,8 @@ class SoftmaxRefinement:
             raise ValueError("coarse points, densities and potential differ in size")
         out = np.zeros(n_fine)
         if level < 0 or level >= len(child_indices):
-            print(f"Error: Invalid level {level} for child_indices_ of size {len(child_indices)}",
-                  file=sys.stderr)
+            print(f"softmax transfer: no child lists for level {level} "
+                  f"({len(child_indices)} levels known); fine potentials left at 0", file=sys.stderr)
             return out
         off, flat = children_csr(child_indices[level])
         lib = L.load()
@@ -92,7 +92,8 @@ def assign_potentials_by_hierarchy(solver: SotSolver, coarse_level: int, fine_le
     keep 0)."""
     prev = _f64(prev_potentials).ravel()
     if coarse_level < 0 or fine_level < 0:
-        print("Invalid hierarchy levels for potential assignment", file=sys.stderr)
+        print(f"potential transfer: negative level ({coarse_level} -> {fine_level})",
+              file=sys.stderr)
         return None
     if coarse_level == fine_level:
         return prev.copy()
@@ -273,18 +274,17 @@ def run_target_multilevel(solver: SotSolver, params: SolverParameters,
     the source already set on `solver`: the hierarchy's levels from the
     coarsest, each solved from the potential transferred from the level
     above; with use_epsilon_scaling the epsilon sequence is spread over the
-    levels (EpsilonScalingHandler).  Returns (finest potential, records)."""
+    levels (epsilon_scaling.level_schedules).  Returns (finest potential, records)."""
     files = formats.get_target_hierarchy_files(ml.target_hierarchy_dir)
     if not files:
         raise RuntimeError("No target point cloud hierarchy found")
     n_levels = len(files)
     cur = _copy_params(params)
     original_eps = params.epsilon
-    handler = None
+    schedules = None
     if params.use_epsilon_scaling:
-        handler = EpsilonScalingHandler(params.epsilon, params.epsilon_scaling_factor,
-                                        params.epsilon_scaling_steps)
-        handler.compute_epsilon_distribution(n_levels)
+        schedules = level_schedules(params.epsilon, params.epsilon_scaling_factor,
+                                    params.epsilon_scaling_steps, n_levels)
     dim = solver.dim
     level_psi = None
     threshold = 0.0
@@ -313,10 +313,7 @@ def run_target_multilevel(solver: SotSolver, params: SolverParameters,
         save = None if out_dir is None else \
             (lambda v, name: formats.write_vector(v, os.path.join(out_dir, name), io_coding))
         t0 = time.perf_counter()
-        if handler is not None:
-            eps_list = handler.get_epsilon_values_for_level(idx) or [original_eps]
-        else:
-            eps_list = [original_eps]
+        eps_list = stages_of_level(schedules, idx, original_eps) if schedules else [original_eps]
         psi = _solve_level(solver, psi, cur, eps_list, rec, save)
         rec.seconds = time.perf_counter() - t0
         threshold = solver.get_last_distance_threshold()
@@ -348,11 +345,10 @@ def run_combined_multilevel(solver: SotSolver, source_levels, params: SolverPara
     n_c = max(n_s, n_t)
     cur = _copy_params(params)
     original_eps = params.epsilon
-    handler = None
+    schedules = None
     if params.use_epsilon_scaling:
-        handler = EpsilonScalingHandler(params.epsilon, params.epsilon_scaling_factor,
-                                        params.epsilon_scaling_steps)
-        handler.compute_epsilon_distribution(n_c)
+        schedules = level_schedules(params.epsilon, params.epsilon_scaling_factor,
+                                    params.epsilon_scaling_steps, n_c)
     dim = solver.dim
     cur_psi = None
     prev_t_level, prev_s_idx = -1, -1
@@ -397,10 +393,7 @@ def run_combined_multilevel(solver: SotSolver, source_levels, params: SolverPara
         save = None if out_dir is None else \
             (lambda v, name: formats.write_vector(v, os.path.join(out_dir, name), io_coding))
         t0 = time.perf_counter()
-        if handler is not None:
-            eps_list = handler.get_epsilon_values_for_level(it) or [original_eps]
-        else:
-            eps_list = [original_eps]
+        eps_list = stages_of_level(schedules, it, original_eps) if schedules else [original_eps]
         psi = _solve_level(solver, psi, cur, eps_list, rec, save)
         rec.seconds = time.perf_counter() - t0
         threshold = solver.get_last_distance_threshold()

@@ -13,7 +13,7 @@ from dataclasses import dataclass, fields
 import numpy as np
 
 from . import _lib as L
-from .epsilon_scaling import EpsilonScalingHandler
+from .epsilon_scaling import epsilon_schedule
 
 _THRESH = {"pointwise": 0, "integral": 1, "geometric": 2}
 _CTRL = {"l1norm": 0, "componentwise": 1}
@@ -250,18 +250,16 @@ class SotSolver:
     def solve_epsilon_scaling(self, potential=None, params: SolverParameters | None = None):
         """The epsilon-scaling loop SemiDiscreteOT::run_sot wraps around
         SotSolver::solve (SemiDiscreteOT.cc:877-918): solves at
-        epsilon * factor^(steps-1-i), i = 0..steps-1 (the schedule of
-        EpsilonScalingHandler.cc:30-41; factor <= 1 -> 2 and steps 0 -> 1 as
-        its constructor, :18-27), each to the tolerance, carrying the potential
+        epsilon * factor^(steps-1-i), i = 0..steps-1 (epsilon_scaling.epsilon_schedule;
+        factor <= 1 -> 2 and steps 0 -> 1 as EpsilonScalingHandler.cc:18-27), each to the tolerance, carrying the potential
         over; a stage that does not converge does not stop the loop.  Without
         use_epsilon_scaling it is a single solve.  Returns (psi, [stage info])."""
         if params is not None:
             self.configure(params)
         p = self.current_params
         if p.use_epsilon_scaling:
-            seq = EpsilonScalingHandler(p.epsilon, p.epsilon_scaling_factor,
-                                        p.epsilon_scaling_steps).generate_epsilon_sequence()
-            seq = seq or [p.epsilon]
+            seq = epsilon_schedule(p.epsilon, p.epsilon_scaling_factor,
+                                   p.epsilon_scaling_steps) or [p.epsilon]
         else:
             seq = [p.epsilon]
         psi = None if potential is None else _f64(potential).copy()

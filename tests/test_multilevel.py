@@ -1,5 +1,5 @@
 """Host-side pieces of the multilevel callers (no GPU): the epsilon
-distribution over levels (EpsilonScalingHandler.cc), the vector and
+schedule and its split over levels (EpsilonScalingHandler.cc:6-100), the vector and
 hierarchy file formats (utils.h:55-176, 238-396; PointCloudHierarchy.cc:
 236-300), the hierarchy's level sizes and clustering, and the direct
 coarse-to-fine potential assignment (SemiDiscreteOT.cc:808-826)."""
@@ -9,28 +9,41 @@ import numpy as np
 import pytest
 
 from semidiscreteot_amd import formats
-from semidiscreteot_amd.epsilon_scaling import EpsilonScalingHandler
+from semidiscreteot_amd.epsilon_scaling import (epsilon_schedule, level_schedules,
+                                                normalized_scaling, split_schedule,
+                                                stages_of_level)
 from semidiscreteot_amd import multilevel as ml
 
 
 def test_epsilon_sequence_and_fallbacks():
-    h = EpsilonScalingHandler(1e-3, 10.0, 3)
-    assert h.generate_epsilon_sequence() == [1e-3 * 10.0 ** 2, 1e-3 * 10.0, 1e-3]
-    assert EpsilonScalingHandler(1e-2, 1.0, 2).scaling_factor == 2.0   # factor <= 1 -> 2
-    assert EpsilonScalingHandler(1e-2, 3.0, 0).num_steps == 1          # 0 steps -> 1
+    # stage i at eps * factor^(n-1-i); factor <= 1 -> 2, 0 stages -> 1
+    assert epsilon_schedule(1e-3, 10.0, 3) == [1e-3 * 10.0 ** 2, 1e-3 * 10.0, 1e-3]
+    assert epsilon_schedule(1e-2, 1.0, 2) == [2e-2, 1e-2]
+    assert epsilon_schedule(1e-2, 0.5, 3) == [4e-2, 2e-2, 1e-2]
+    assert epsilon_schedule(1e-2, 3.0, 0) == [1e-2]
+    assert normalized_scaling(1.0, 0) == (2.0, 1)
 
 
 def test_epsilon_distribution_over_levels():
-    h = EpsilonScalingHandler(1.0, 2.0, 5)
-    seq = h.generate_epsilon_sequence()                  # 16, 8, 4, 2, 1
-    # more values than levels: ceil(remaining / levels left), coarsest first
-    assert h.compute_epsilon_distribution(3) == [seq[0:2], seq[2:4], seq[4:5]]
-    assert h.compute_epsilon_distribution(2) == [seq[0:3], seq[3:5]]
+    seq = epsilon_schedule(1.0, 2.0, 5)                  # 16, 8, 4, 2, 1
+    assert seq == [16.0, 8.0, 4.0, 2.0, 1.0]
+    # more values than levels: ceil(values left / levels left), coarsest first
+    assert split_schedule(seq, 3) == [seq[0:2], seq[2:4], seq[4:5]]
+    assert split_schedule(seq, 2) == [seq[0:3], seq[3:5]]
+    assert level_schedules(1.0, 2.0, 7, 3) == [[64.0, 32.0, 16.0], [8.0, 4.0], [2.0, 1.0]]
+    # every value is used exactly once, in order
+    for n in range(1, 9):
+        for L in range(1, 9):
+            parts = split_schedule(list(range(n)), L)
+            assert len(parts) == L
+            if n > L:
+                assert sum(parts, []) == list(range(n))
+                assert all(len(a) >= len(b) for a, b in zip(parts, parts[1:]))
     # fewer values than levels: one each, the smallest repeated
-    h2 = EpsilonScalingHandler(1.0, 2.0, 2)
-    assert h2.compute_epsilon_distribution(4) == [[2.0], [1.0], [1.0], [1.0]]
-    assert h2.get_epsilon_values_for_level(7) == []
-    assert h2.distribute_epsilon_values([3.0], 0) == []
+    assert level_schedules(1.0, 2.0, 2, 4) == [[2.0], [1.0], [1.0], [1.0]]
+    assert split_schedule([3.0], 0) == []
+    assert stages_of_level([[2.0], [1.0]], 7, 5.0) == [5.0]
+    assert stages_of_level([[2.0], [1.0]], 1, 5.0) == [1.0]
 
 
 def test_vector_formats_round_trip(tmp_path):

@@ -334,3 +334,54 @@ def test_point_queries_closed_forms():
     srt = np.sort(sc, 1)
     ok = srt[:, -1] - srt[:, -2] > 50 * e
     assert ok.sum() > 50 and np.max(np.abs(b[ok] - mo[ok])) <= 1e-12
+
+
+# ---- covering radius (SotSolver.cc:431-473) and discarded mass ----
+
+def test_covering_radius_brute_force_and_closed_form():
+    # brute force: max over cells of the distance from the center to the
+    # nearest target, the reference's sqrt of the running sum of squares
+    for dim, n, N, seed in ((2, 7, 40, 1), (3, 5, 60, 2)):
+        src, tgt = structured_case(dim, n, 2, N, seed=seed, gmm_targets=True)
+        c = src.center.reshape(-1, dim)
+        y = tgt.y.reshape(-1, dim)
+        d = np.sqrt(((c[:, None, :] - y[None, :, :]) ** 2).sum(-1))
+        assert O.covering_radius(src, tgt) == pytest.approx(d.min(1).max(), rel=1e-15, abs=0)
+    # closed form: one target at the center of [0,1]^3 cut into n^3 cells: the
+    # farthest cell center is a corner cell's, sqrt(3) (1/2 - 1/(2n)) away
+    n = 6
+    src = O.structured_source(3, [n] * 3, [0.0] * 3, [1.0] * 3, 1)
+    tgt = O.Target(np.array([0.5, 0.5, 0.5]), np.array([1.0]))
+    assert O.covering_radius(src, tgt) == pytest.approx(np.sqrt(3.0) * (0.5 - 0.5 / n), rel=1e-15)
+    # targets at every cell center: every cell is covered at distance 0
+    tgt = O.Target(src.center.copy(), np.full(n ** 3, 1.0 / n ** 3))
+    assert O.covering_radius(src, tgt) == 0.0
+
+
+def test_discarded_fraction_brute_force_and_limits():
+    src, tgt = structured_case(2, 6, 2, 25, seed=11, gmm_targets=True, dirichlet=3)
+    eps, R = 0.02, 0.3
+    psi = 0.01 * np.random.default_rng(3).standard_normal(tgt.n)
+    q = np.arange(0, src.n_cells * src.nq, 7, dtype=np.int64)
+    got = O.discarded_fraction(src, tgt, O.params(epsilon=eps, use_lse=1), R, psi, q)
+    x = src.x.reshape(-1, 2)
+    c = src.center.reshape(-1, 2)
+    y = tgt.y.reshape(-1, 2)
+    for k, qi in enumerate(q):
+        e = (psi - 0.5 * ((x[qi] - y) ** 2).sum(1)) / eps
+        w = tgt.nu * np.exp(e - e.max())
+        out = np.sqrt(((y - c[qi // src.nq]) ** 2).sum(1)) > R
+        assert got[k] == pytest.approx(w[out].sum() / w.sum(), rel=1e-12, abs=1e-300)
+    # no truncation keeps everything; a negative radius drops everything
+    assert np.all(O.discarded_fraction(src, tgt, O.params(epsilon=eps, use_lse=1), 10.0, psi, q) == 0)
+    assert np.all(O.discarded_fraction(src, tgt, O.params(epsilon=eps, use_lse=1), -1.0, psi, q) == 1)
+    # two targets symmetric about a point, equal weights and potentials, a
+    # radius that keeps only the nearer one of a q-point on the axis: the
+    # fraction is the farther one's share, 1 / (1 + exp(d / eps)) with
+    # d = (|x - y_far|^2 - |x - y_near|^2) / 2
+    src1 = O.structured_source(2, [1, 1], [0.0, 0.0], [1.0, 1.0], 1)
+    tgt1 = O.Target(np.array([0.2, 0.5, 0.9, 0.5]), np.array([0.5, 0.5]))
+    d = 0.5 * ((0.5 - 0.9) ** 2 - (0.5 - 0.2) ** 2)
+    got1 = O.discarded_fraction(src1, tgt1, O.params(epsilon=0.05, use_lse=1), 0.35, np.zeros(2),
+                                np.array([0], dtype=np.int64))
+    assert got1[0] == pytest.approx(1.0 / (1.0 + np.exp(d / 0.05)), rel=1e-14)
