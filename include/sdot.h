@@ -132,6 +132,64 @@ int sdot_params_default(sdot_params *p);
  * (ParameterManager.cc:64-95); keys outside it are ignored. */
 int sdot_params_parse_prm(const char *text, sdot_params *p);
 int sdot_params_load_prm(const char *path, sdot_params *p);
+/* The rest of a SotParameterManager parameter file (ParameterManager.cc:
+ * 19-146, ParameterManager.h:42-121) and the `lloyd parameters` subsection
+ * of LloydParameterManager (ParameterManager.cc:412-422, ParameterManager.h:
+ * 202-205): everything the reference's run/ parameter files set.  Strings are
+ * NUL-terminated; a value longer than its buffer is a parse error. */
+typedef struct {
+  uint32_t n_refinements;              /* "number of refinements" (0) */
+  char grid_generator_function[64];    /* "grid generator function" */
+  char grid_generator_arguments[256];  /* "grid generator arguments" */
+  int32_t use_tetrahedral_mesh;        /* "use tetrahedral mesh" (false) */
+  int32_t use_custom_density;          /* "use custom density" (false) */
+  char density_file_path[256];         /* "density file path" */
+  char density_file_format[16];        /* "density file format" ("vtk") */
+  char density_field_name[64];         /* "density field name" ("normalized_density") */
+} sdot_mesh_params;
+
+/* SotParameterManager::MultilevelParameters (ParameterManager.h:78-98). */
+typedef struct {
+  int32_t source_enabled;              /* false */
+  int32_t source_min_vertices;         /* 1000 */
+  int32_t source_max_vertices;         /* 10000 */
+  char source_hierarchy_dir[256];      /* "output/data_multilevel/source_multilevel" */
+  int32_t target_enabled;              /* false */
+  int32_t target_min_points;           /* 100 */
+  int32_t target_max_points;           /* 1000 */
+  char target_hierarchy_dir[256];      /* "output/data_multilevel/target_multilevel" */
+  int32_t use_softmax_potential_transfer; /* true */
+  int32_t use_python_clustering;       /* false */
+  char python_script_name[128];        /* "multilevel_clustering_scipy.py" */
+  char output_prefix[256];             /* "output/multilevel/sot" */
+} sdot_multilevel_params;
+
+#define SDOT_MAX_DENSITY_INDICES 256
+typedef struct {
+  sdot_params solver;                  /* subsection rsot_solver */
+  char selected_task[64];              /* "selected_task" ("") */
+  int32_t io_coding;                   /* "io_coding": 0 = txt (default), 1 = bin */
+  sdot_mesh_params source, target;     /* subsection mesh_generation / source, target */
+  sdot_multilevel_params multilevel;   /* subsection multilevel_parameters */
+  char power_diagram_implementation[16]; /* power_diagram_parameters ("dealii") */
+  double transport_map_truncation_radius; /* transport_map_parameters (-1 = disabled) */
+  int32_t n_density_indices;           /* conditional_density_parameters "indices" */
+  uint32_t density_indices[SDOT_MAX_DENSITY_INDICES];
+  char density_potential_folder[256];  /* "potential_folder" ("") */
+  double density_truncation_radius;    /* "truncation_radius" (-1 = disabled) */
+  uint32_t lloyd_max_iterations;       /* lloyd parameters "max iterations" (1000) */
+  double lloyd_relative_tolerance;     /* "relative tolerance" (1e-8) */
+} sdot_prm_file;
+
+/* Defaults of every field (the declarations cited above). */
+int sdot_prm_file_default(sdot_prm_file *f);
+/* Parse a whole parameter file: the subsections above directly under
+ * `subsection SotParameterManager` (or at the top level), rsot_solver as
+ * sdot_params_parse_prm reads it; keys these subsections do not declare are
+ * a parse error (deal.II rejects them), other subsections are skipped. */
+int sdot_prm_file_parse(const char *text, sdot_prm_file *f);
+int sdot_prm_file_load(const char *path, sdot_prm_file *f);
+
 /* SotSolver::configure (SotSolver.cc:63-70). */
 int sdot_set_params(sdot_ctx *ctx, const sdot_params *p);
 

@@ -12,6 +12,7 @@ from .server import ServeResult, SotServer  # noqa: F401
 from .epsilon_scaling import epsilon_schedule, level_schedules  # noqa: F401
 from .lloyd import run_lloyd  # noqa: F401
 from .transport_plan import OptimalTransportPlan  # noqa: F401
+from .parameters import ParameterFile  # noqa: F401
 from .multilevel import (MultilevelParameters, SoftmaxRefinement,  # noqa: F401
                          generate_target_hierarchy, run_combined_multilevel,
                          run_target_multilevel)
@@ -20,4 +21,4 @@ __all__ = ["SotSolver", "SolverParameters", "SotServer", "ServeResult", "SourceA
            "simplex_source", "SdotError", "load", "LIB_PATH", "epsilon_schedule", "level_schedules",
            "SoftmaxRefinement", "MultilevelParameters", "generate_target_hierarchy",
            "run_target_multilevel", "run_combined_multilevel", "run_lloyd",
-           "OptimalTransportPlan"]
+           "OptimalTransportPlan", "ParameterFile"]

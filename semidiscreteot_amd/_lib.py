@@ -39,6 +39,38 @@ class Params(C.Structure):
                 ("line_search", C.c_int32), ("preconditioner", C.c_int32)]
 
 
+class MeshParams(C.Structure):
+    _fields_ = [("n_refinements", C.c_uint32), ("grid_generator_function", C.c_char * 64),
+                ("grid_generator_arguments", C.c_char * 256), ("use_tetrahedral_mesh", C.c_int32),
+                ("use_custom_density", C.c_int32), ("density_file_path", C.c_char * 256),
+                ("density_file_format", C.c_char * 16), ("density_field_name", C.c_char * 64)]
+
+
+class MultilevelParams(C.Structure):
+    _fields_ = [("source_enabled", C.c_int32), ("source_min_vertices", C.c_int32),
+                ("source_max_vertices", C.c_int32), ("source_hierarchy_dir", C.c_char * 256),
+                ("target_enabled", C.c_int32), ("target_min_points", C.c_int32),
+                ("target_max_points", C.c_int32), ("target_hierarchy_dir", C.c_char * 256),
+                ("use_softmax_potential_transfer", C.c_int32),
+                ("use_python_clustering", C.c_int32), ("python_script_name", C.c_char * 128),
+                ("output_prefix", C.c_char * 256)]
+
+
+MAX_DENSITY_INDICES = 256
+
+
+class PrmFile(C.Structure):
+    _fields_ = [("solver", Params), ("selected_task", C.c_char * 64), ("io_coding", C.c_int32),
+                ("source", MeshParams), ("target", MeshParams), ("multilevel", MultilevelParams),
+                ("power_diagram_implementation", C.c_char * 16),
+                ("transport_map_truncation_radius", C.c_double),
+                ("n_density_indices", C.c_int32),
+                ("density_indices", C.c_uint32 * MAX_DENSITY_INDICES),
+                ("density_potential_folder", C.c_char * 256),
+                ("density_truncation_radius", C.c_double),
+                ("lloyd_max_iterations", C.c_uint32), ("lloyd_relative_tolerance", C.c_double)]
+
+
 class SolveInfo(C.Structure):
     _fields_ = [("status", C.c_int32), ("iterations", C.c_int32), ("evaluations", C.c_int32),
                 ("functional", C.c_double), ("check_value", C.c_double), ("seconds", C.c_double)]
@@ -89,6 +121,9 @@ SIGNATURES = {
     "sdot_params_default": (C.c_int, [C.POINTER(Params)]),
     "sdot_params_parse_prm": (C.c_int, [C.c_char_p, C.POINTER(Params)]),
     "sdot_params_load_prm": (C.c_int, [C.c_char_p, C.POINTER(Params)]),
+    "sdot_prm_file_default": (C.c_int, [C.POINTER(PrmFile)]),
+    "sdot_prm_file_parse": (C.c_int, [C.c_char_p, C.POINTER(PrmFile)]),
+    "sdot_prm_file_load": (C.c_int, [C.c_char_p, C.POINTER(PrmFile)]),
     "sdot_set_params": (C.c_int, [C.c_void_p, C.POINTER(Params)]),
     "sdot_set_source": (C.c_int, [C.c_void_p, C.c_int, C.c_int64, C.c_int, _dp, _dp, _dp]),
     "sdot_set_target": (C.c_int, [C.c_void_p, C.c_int64, _dp, _dp]),

@@ -221,12 +221,25 @@ def generate_target_hierarchy(points, densities=None, output_dir: str | None = N
 
 @dataclass
 class MultilevelParameters:
-    """The fields of SotParameterManager::MultilevelParameters
-    (ParameterManager.h:78-98) the drivers read."""
-    target_hierarchy_dir: str = "output/data_multilevel/target_multilevel"
-    use_softmax_potential_transfer: bool = True
+    """SotParameterManager::MultilevelParameters (ParameterManager.h:78-98),
+    with its defaults; `subsection multilevel_parameters` of a parameter file
+    fills it (parameters.ParameterFile).  The drivers here read the target
+    hierarchy fields; the source hierarchy is passed to
+    run_combined_multilevel as arrays (mesh generation is not part of this
+    library), and the Python clustering switch has no effect (the hierarchy
+    is built by generate_target_hierarchy)."""
+    source_enabled: bool = False
+    source_min_vertices: int = 1000
+    source_max_vertices: int = 10000
+    source_hierarchy_dir: str = "output/data_multilevel/source_multilevel"
+    target_enabled: bool = False
     target_min_points: int = 100
     target_max_points: int = 1000
+    target_hierarchy_dir: str = "output/data_multilevel/target_multilevel"
+    use_softmax_potential_transfer: bool = True
+    use_python_clustering: bool = False
+    python_script_name: str = "multilevel_clustering_scipy.py"
+    output_prefix: str = "output/multilevel/sot"
 
 
 @dataclass

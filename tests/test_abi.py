@@ -197,3 +197,87 @@ def test_server_config_defaults():
     out = C.c_void_p()
     assert lib.sdot_server_create(None, C.byref(cfg), C.byref(out)) != 0
     assert not out.value
+
+
+# ---- whole parameter files (ParameterManager.cc:19-146, 412-422) ----
+
+def test_parameter_file_every_subsection():
+    from semidiscreteot_amd.parameters import ParameterFile
+    pf = ParameterFile.from_prm(path=os.path.join(ROOT, "tests", "golden", "parameters_full.prm"))
+    assert pf.selected_task == "multilevel_sot" and pf.io_coding == "bin"
+    s = pf.solver
+    assert (s.max_iterations, s.epsilon, s.tau, s.tolerance) == (2500, 5e-3, 1e-10, 1e-6)
+    assert s.distance_threshold_type == "geometric" and s.solver_control_type == "componentwise"
+    assert s.quadrature_order == 2 and s.n_threads == 4 and s.use_log_sum_exp_trick
+    assert s.use_epsilon_scaling and s.epsilon_scaling_factor == 4.0 and s.epsilon_scaling_steps == 3
+    src, tgt = pf.source_mesh, pf.target_mesh
+    assert (src.n_refinements, src.grid_generator_function) == (5, "hyper_cube")
+    assert src.grid_generator_arguments == "-1 : 1 : false" and src.use_tetrahedral_mesh
+    assert src.density_file_path == "output/density_field/source_density.vtk"
+    assert (src.density_field_name, src.density_file_format) == ("rho", "h5")
+    # continuation line joined; unset keys keep their defaults
+    assert tgt.grid_generator_arguments.replace(" ", "") == "0,0,0:1:true"
+    assert (tgt.density_file_format, tgt.density_field_name) == ("vtk", "normalized_density")
+    ml = pf.multilevel
+    assert ml.source_enabled and (ml.source_min_vertices, ml.source_max_vertices) == (500, 20000)
+    assert ml.source_hierarchy_dir == "out/src_levels" and ml.target_enabled
+    assert (ml.target_min_points, ml.target_max_points) == (250, 64000)
+    assert ml.target_hierarchy_dir == "out/tgt_levels" and not ml.use_softmax_potential_transfer
+    assert ml.use_python_clustering and ml.python_script_name == "multilevel_clustering_faiss_cpu.py"
+    assert ml.output_prefix == "out/multilevel"
+    assert pf.power_diagram_implementation == "geogram"
+    assert pf.transport_map_truncation_radius == 0.25
+    cd = pf.conditional_density
+    assert cd.indices == [0, 3, 17] and cd.potential_folder == "out/potentials"
+    assert cd.truncation_radius == 0.5
+    assert (pf.lloyd.max_iterations, pf.lloyd.relative_tolerance) == (40, 1e-5)
+
+
+def test_parameter_file_defaults_and_errors():
+    from semidiscreteot_amd.parameters import ParameterFile
+    d = ParameterFile.from_prm(text="")
+    # ParameterManager.h:42-121, 202-205
+    assert d.io_coding == "txt" and d.selected_task == ""
+    assert d.multilevel.target_hierarchy_dir == "output/data_multilevel/target_multilevel"
+    assert d.multilevel.source_hierarchy_dir == "output/data_multilevel/source_multilevel"
+    assert d.multilevel.use_softmax_potential_transfer and not d.multilevel.target_enabled
+    assert (d.multilevel.source_min_vertices, d.multilevel.source_max_vertices) == (1000, 10000)
+    assert (d.multilevel.target_min_points, d.multilevel.target_max_points) == (100, 1000)
+    assert d.multilevel.python_script_name == "multilevel_clustering_scipy.py"
+    assert d.multilevel.output_prefix == "output/multilevel/sot"
+    assert d.transport_map_truncation_radius == -1.0 and d.conditional_density.truncation_radius == -1.0
+    assert d.conditional_density.indices == [] and d.power_diagram_implementation == "dealii"
+    assert (d.lloyd.max_iterations, d.lloyd.relative_tolerance) == (1000, 1e-8)
+    assert d.solver.epsilon == 1e-3 and d.solver.max_iterations == 1000
+    # the manager's keys at the top level (no SotParameterManager subsection)
+    t = ParameterFile.from_prm(text="set io_coding = bin\nsubsection transport_map_parameters\n"
+                                    "  set truncation_radius = 2\nend\n")
+    assert t.io_coding == "bin" and t.transport_map_truncation_radius == 2.0
+    bad = [
+        "subsection SotParameterManager\n set io_codng = txt\nend\n",               # undeclared key
+        "subsection SotParameterManager\n set io_coding = vtk\nend\n",              # bad value
+        "subsection SotParameterManager\n subsection multilevel_parameters\n"
+        "  set target_min_points = many\n end\nend\n",                               # not an integer
+        "subsection SotParameterManager\n subsection no_such_section\n set a = 1\n end\nend\n",
+        "subsection SotParameterManager\n subsection conditional_density_parameters\n"
+        "  set indices = 1, -2\n end\nend\n",                                        # negative index
+        "subsection SotParameterManager\n set selected_task = sot\n",               # missing end
+        "subsection SotParameterManager\n subsection transport_map_parameters\n"
+        "  set radius = 1\n end\nend\n",                                             # undeclared key
+    ]
+    for text in bad:
+        with pytest.raises(L.SdotError):
+            ParameterFile.from_prm(text=text)
+
+
+def test_parameter_file_reads_the_reference_run_files():
+    """Every parameter file the reference ships parses (only in a tree where
+    the reference checkout is present)."""
+    from semidiscreteot_amd.parameters import ParameterFile
+    import glob
+    files = glob.glob("/root/reference/**/*.prm", recursive=True)
+    if not files:
+        pytest.skip("reference checkout not present")
+    for f in files:
+        pf = ParameterFile.from_prm(path=f)
+        assert pf.io_coding in ("txt", "bin")
