@@ -241,6 +241,7 @@ struct sdot_ctx {
   bool hist_valid = false;  // device holds psi and M_q of a previous FP32 evaluation
   double hist_R = 0.0;      // its truncation radius
   int n_servers = 0;        // serving queues reading this context's source arrays
+  double seed_bias = 0.0;   // test hook (sdot_ctx_create)
   Plan plan;
   Dev dev;
   std::vector<double> h_nu;
@@ -307,6 +308,7 @@ int prepare(sdot_ctx *c, double R) {
   p.Rq = R * (1.0 + 1e-9) + 1e-9 * (p.tg.h + p.cg.h + c->coord_scale);
   // the seed needs the previous evaluation's candidate sets: same R
   p.hist = (p.mode == kF32Lse && c->hist_valid && c->hist_R == R) ? 1 : 0;
+  p.seed_bias = c->seed_bias;
   if (!std::isfinite(p.Rq)) p.dense = std::isnan(R) ? 0 : 1;
   if (std::isnan(R)) p.Rq = 0.0;
   // truncated sweeps run the small grouping (same R, same grouping: the FP32
@@ -618,6 +620,10 @@ extern "C" int sdot_ctx_create(int device, sdot_ctx **out) {
     return fail(SDOT_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
   }
   sdot_params_default(&c->params);
+  // test hook: SDOT_TEST_SEED_BIAS raises every seeded reference value of the
+  // FP32 sweep by that many log2 units, which drives the sweep's re-sweep of
+  // items whose seed lies too high (the seed bound itself never does)
+  if (const char *b = std::getenv("SDOT_TEST_SEED_BIAS")) c->seed_bias = std::atof(b);
   *out = c;
   return SDOT_OK;
 }

@@ -435,6 +435,7 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
     });
   };
 
+  bool seedq = false;  // the item's reference value came from the history seed
   if (PASS == 2) {
     // reference value of this item (see kSeedSpread)
     bool seed = false;
@@ -464,9 +465,11 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
       }
       seed = dmx - dmn < kSeedSpread;
     }
+    seedq = seed;
     if (seed) {
 #pragma unroll
-      for (int k = 0; k < G; ++k) M[k] = ceil(a.Tout[(int64_t)seg * a.nQ + q0 + k] + dmx);
+      for (int k = 0; k < G; ++k)
+        M[k] = ceil(a.Tout[(int64_t)seg * a.nQ + q0 + k] + dmx) + a.seed_bias;
     } else {
       // exact maximum over the item's candidates first
       records([&](const TRec<DIM> &r, bool tested) {
@@ -487,6 +490,25 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
     }
   }
   records(one);
+  if (PASS == 2 && seedq) {
+    // The seeded bound holds by construction: the new maximum lies less than
+    // kSeedSpread + 1 below it.  An item whose maximum fell further (history
+    // that does not belong to these potentials) is swept again against its
+    // exact maximum, found by the sweep just done, so that its dominant terms
+    // keep their FP32 argument precision.
+    bool low = false;
+#pragma unroll
+    for (int k = 0; k < G; ++k) low |= T[k] > -INFINITY && T[k] < M[k] - (kSeedSpread + 2.0);
+    if (__ballot(low) != 0) {
+#pragma unroll
+      for (int k = 0; k < G; ++k) {
+        M[k] = ceil(T[k]);
+        S[k] = 0.0;
+        T[k] = -INFINITY;
+      }
+      records(one);
+    }
+  }
   if (active) {
 #pragma unroll
     for (int k = 0; k < G; ++k) {

@@ -122,6 +122,7 @@ struct Plan {
   double s_in = 0.0;    // inner runs: every pair closer than sqrt(s_in) passes the rule
   int hist = 0;        // FP32 mode: seed the online sweep with the previous evaluation
   int dense = 0;       // every pair is a candidate (no enumeration, no test)
+  double seed_bias = 0.0;  // test hook: raises the FP32 sweep's seeded reference values
   // the q-major sweeps stream the target records through LDS (record sets
   // larger than one XCD's L2)
 };
@@ -164,6 +165,8 @@ struct P1Args {
   int nq, gpc, K1, dense, hist;
   double smax, Rq, plain_coef, coef;
   double s_in;            // squared radius of the untested inner runs (0: none)
+  double seed_bias;       // added to every seeded reference value (0; a test hook that
+                          // exercises the re-sweep of an item whose seed is too high)
 };
 
 struct QFArgs {

@@ -424,6 +424,7 @@ static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_g
   a.smax = p.smax;
   a.Rq = p.Rq;
   a.s_in = p.s_in;
+  a.seed_bias = p.seed_bias;
   a.plain_coef = 0.5 / p.eps;
   a.coef = coef;
   a.Mout = d.Mpart;
