@@ -241,7 +241,8 @@ struct sdot_ctx {
   bool hist_valid = false;  // device holds psi and M_q of a previous FP32 evaluation
   double hist_R = 0.0;      // its truncation radius
   int n_servers = 0;        // serving queues reading this context's source arrays
-  double seed_bias = 0.0;   // test hook (sdot_ctx_create)
+  double seed_bias = 0.0;   // test hooks (sdot_ctx_create)
+  int force_K1 = 0, force_K2 = 0;
   Plan plan;
   Dev dev;
   std::vector<double> h_nu;
@@ -347,6 +348,8 @@ int prepare(sdot_ctx *c, double R) {
   p.K2 = (int)std::min<int64_t>(K2, kMaxSeg);
   // the gradient grid gives each XCD the segments s = xcd (mod 8)
   if (p.K2 >= 8) p.K2 -= p.K2 % 8;
+  if (c->force_K1 > 0) p.K1 = std::min(c->force_K1, kMaxSeg);
+  if (c->force_K2 > 0) p.K2 = std::min(c->force_K2, kMaxSeg);
   p.nFpart = (p.nQ + 255) / 256;
   p.nPpart = (p.N + 255) / 256;
   Dev &d = c->dev;
@@ -624,6 +627,16 @@ extern "C" int sdot_ctx_create(int device, sdot_ctx **out) {
   // FP32 sweep by that many log2 units, which drives the sweep's re-sweep of
   // items whose seed lies too high (the seed bound itself never does)
   if (const char *b = std::getenv("SDOT_TEST_SEED_BIAS")) c->seed_bias = std::atof(b);
+  // test hook: SDOT_TEST_SEGMENTS="K1,K2" fixes the segment counts of the
+  // q-major and target-major sweeps (exercises counts the sizing rule picks
+  // only at sizes no test reaches)
+  if (const char *k = std::getenv("SDOT_TEST_SEGMENTS")) {
+    int k1 = 0, k2 = 0;
+    if (std::sscanf(k, "%d,%d", &k1, &k2) == 2 && k1 >= 1 && k2 >= 1) {
+      c->force_K1 = k1;
+      c->force_K2 = k2;
+    }
+  }
   *out = c;
   return SDOT_OK;
 }
