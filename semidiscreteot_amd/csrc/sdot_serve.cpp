@@ -881,6 +881,56 @@ extern "C" int sdot_server_autotune(sdot_server *s, int32_t *pass_targets, int32
       for (int d = 0; d < dim; ++d)
         ys[i][j * dim + d] = s->src.lo[d] + (s->src.hi[d] - s->src.lo[d]) * uni();
   }
+  const int pts[4] = {1024, 2048, 4096, 8192};
+  const int gqs[4] = {2048, 4096, 8192, 16384};
+  const int prior_pt = s->pass_targets, prior_gq = s->grad_qpoints;
+  // the batch must fit (4 slots, the arena) at the sizes with the most work
+  // items, or autotune changes nothing
+  {
+    if (s->max_slots < 4) return set_error(SDOT_ERR_NOMEM, "autotune needs 4 solve slots");
+    s->pass_targets = pts[0];
+    s->grad_qpoints = gqs[0];
+    size_t need = 0;
+    int rc = SDOT_OK;
+    for (int i = 0; i < 4 && rc == SDOT_OK; ++i) {
+      Request r;
+      r.N = sizes[i];
+      r.R = INFINITY;
+      build_target_layout(dim, r.N, ys[i].data(), r.L);
+      rc = fill_shape(s, r);
+      need += r.bytes;
+    }
+    s->pass_targets = prior_pt;
+    s->grad_qpoints = prior_gq;
+    if (rc) return rc;
+    if (need > s->arena.size())
+      return set_error(SDOT_ERR_NOMEM, "autotune batch needs " + std::to_string(need) +
+                                           " device bytes, more than the server's HBM budget");
+  }
+  std::vector<int64_t> synthetic;
+  // on an error: the prior sizes back, the synthetic solves gone from the
+  // queue, the active list (their arena blocks released) and the results
+  auto undo = [&](int rc) {
+    s->pass_targets = prior_pt;
+    s->grad_qpoints = prior_gq;
+    auto ours = [&](int64_t id) {
+      return std::find(synthetic.begin(), synthetic.end(), id) != synthetic.end();
+    };
+    std::lock_guard<std::mutex> g(s->mu);
+    for (auto it = s->queue.begin(); it != s->queue.end();)
+      it = ours((*it)->id) ? s->queue.erase(it) : it + 1;
+    for (size_t i = 0; i < s->active.size();) {
+      if (ours(s->active[i]->req->id)) {
+        s->arena.release_block(s->active[i]->off, s->active[i]->o.total);
+        s->active.erase(s->active.begin() + i);
+        s->dirty = true;
+      } else {
+        ++i;
+      }
+    }
+    for (int64_t id : synthetic) s->done.erase(id);
+    return rc;
+  };
   auto time_config = [&](int pt, int gq, double &ms) -> int {
     s->pass_targets = pt;
     s->grad_qpoints = gq;
@@ -900,6 +950,7 @@ extern "C" int sdot_server_autotune(sdot_server *s, int32_t *pass_targets, int32
         int rc = sdot_server_submit(s, &q, &id);
         if (rc) return rc;
         ids.push_back(id);
+        synthetic.push_back(id);
       }
       int32_t na = 0, nq = 0;
       int rc = sdot_server_step(s, &na, &nq);
@@ -915,13 +966,11 @@ extern "C" int sdot_server_autotune(sdot_server *s, int32_t *pass_targets, int32
     ms = best;
     return SDOT_OK;
   };
-  const int pts[4] = {1024, 2048, 4096, 8192};
-  const int gqs[4] = {2048, 4096, 8192, 16384};
   int best_pt = s->pass_targets, best_gq = s->grad_qpoints;
   double best = INFINITY, ms = 0.0;
   for (int pt : pts) {
     int rc = time_config(pt, best_gq, ms);
-    if (rc) return rc;
+    if (rc) return undo(rc);
     if (ms < best) {
       best = ms;
       best_pt = pt;
@@ -930,7 +979,7 @@ extern "C" int sdot_server_autotune(sdot_server *s, int32_t *pass_targets, int32
   best = INFINITY;
   for (int gq : gqs) {
     int rc = time_config(best_pt, gq, ms);
-    if (rc) return rc;
+    if (rc) return undo(rc);
     if (ms < best) {
       best = ms;
       best_gq = gq;
