@@ -39,6 +39,45 @@ def pass_peak_pairs(dim):
     return 1.0 / max(1.0 / EXP_PER_S, (dim + 3) / FMA_PER_S)
 
 
+def host_cpus():
+    """The host CPUs this process may use: its affinity set, capped by the
+    cgroup CPU quota when one is set (a container's share of a larger host),
+    with nproc (os.cpu_count) and the CPU model for the record."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = max(1, int(float(q) / float(per)))
+    except (OSError, ValueError):
+        pass
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    usable = min(aff, quota) if quota else aff
+    return {"usable": usable, "affinity": aff, "cgroup_quota": quota,
+            "nproc": os.cpu_count(), "model": model}
+
+
+def measured_rates():
+    """Instruction rates measured on this GPU by lib/sdot_rates (v_exp_f32,
+    v_fma_f64, FP64 MFMA, and whether FP64 MFMA and VALU work overlap)."""
+    import subprocess
+    exe = os.path.join(ROOT, "semidiscreteot_amd", "lib", "sdot_rates")
+    if not os.path.exists(exe):
+        return None
+    try:
+        r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+        return json.loads(r.stdout.strip().splitlines()[-1]) if r.returncode == 0 else None
+    except (subprocess.SubprocessError, ValueError, IndexError):
+        return None
+
+
 def cpu_baseline(wl, psi, R, target_seconds=15.0, extra=None):
     """Time the FP64 CPU restatement (oracle, reference algorithm structure:
     per-cell O(N) candidate scan, dense per-cell buffers, sqrt-then-square
@@ -46,7 +85,8 @@ def cpu_baseline(wl, psi, R, target_seconds=15.0, extra=None):
     `extra` = (workload, psi, R) also on a sample of that one (the large
     unstructured workload)."""
     from oracle import oracle as O
-    cores = min(os.cpu_count() or 1, 16)
+    host = host_cpus()
+    cores = host["usable"]
 
     def sample(k, src_):
         n_ = src_.n_cells
@@ -94,6 +134,7 @@ def cpu_baseline(wl, psi, R, target_seconds=15.0, extra=None):
     t_solve = time.perf_counter() - t0
     pairs2 = 36864 * 1000
     return {"value": pairs / dt, "unit": "pair-evals/s", "cores": cores, "kind": "port",
+            "host": host,
             "sample": f"{k} of {n_cells} cells (evenly spaced), one FP64 evaluation, "
                       f"{pairs} pairs in {dt:.2f} s",
             "plain_2d_solve": {"seconds": t_solve, "status": info2["status"],
@@ -223,18 +264,23 @@ def measured_traffic(kernel_key):
 
 def serving_leg(s, args):
     """Serving workload: the first `serve_requests` requests of the seeded
-    trace (Poisson arrivals at 50/s, N ~ logU[256, 65536], eps in {1e-2, 3e-3,
-    1e-3}, componentwise tolerance 1e-6) against the resident source of `s`.
-      batched:  FP32_LSE serving queue (escalation to FP64 for the final
-                iterations), requests submitted at their arrival times;
-      one-at-a-time: the FP64 single-solve path (sdot_solve), requests in
-                arrival order, each started when it has arrived and the
-                previous one has finished.
-    Both use the noise-aware line search and the same initial inverse Hessian
-    of the L-BFGS recursion (`--serve-precond`: diag(eps/nu_j) by default);
-    every batched result is re-checked with an FP64 evaluation at the returned
-    potential.  With `--serve-identity-leg` the one-at-a-time FP64 path also
-    runs with the identity (the reference's SolverBFGS) for comparison."""
+    trace (N ~ logU[256, 65536], eps in {1e-2, 3e-3, 1e-3}, componentwise
+    tolerance 1e-6) against the resident source of `s`.
+      capacity: every request is available at once (offered load above both
+                paths' capacity): solves/s of the batched FP32_LSE serving queue
+                (work-item sizes from sdot_server_autotune; escalation to FP64
+                for the final iterations) and of the one-at-a-time FP64
+                single-solve path (sdot_solve), on the same requests;
+      steady:   the first `serve_steady` requests arriving as a Poisson stream
+                (the trace's seed-4 gaps, rescaled) at `serve_load` x the
+                measured batched capacity: latency p50/p99 of the queue; the
+                one-at-a-time path's latency at the same arrival times follows
+                from its measured per-request service times.
+    Both paths use the noise-aware line search and the same initial inverse
+    Hessian (`--serve-precond`: diag(eps/nu_j) by default); every batched
+    result is re-checked with an FP64 evaluation at the returned potential.
+    `--serve-identity-leg` also times the one-at-a-time path with the
+    identity (the reference's SolverBFGS) on the first 8 requests."""
     import itertools
     import semidiscreteot_amd as sd
     from semidiscreteot_amd import workloads
@@ -244,46 +290,53 @@ def serving_leg(s, args):
               epsilon_scaling_steps=args.serve_eps_steps, epsilon_scaling_factor=2.0,
               preconditioner=args.serve_precond)
 
-    srv = sd.SotServer(s, precision="fp32_lse", max_slots=256, hbm_budget_bytes=96 << 30,
-                       polish_factor=args.serve_polish)
-    t0 = srv.clock()
-    pending = sorted(reqs, key=lambda r: r.arrival)
-    inflight, done = {}, {}
-    while pending or inflight:
-        now = srv.clock() - t0
-        while pending and pending[0].arrival <= now:
-            r = pending.pop(0)
-            inflight[srv.submit(r.y, r.nu, r.epsilon, r.tolerance, **kw)] = r
-        if not inflight:
-            time.sleep(max(0.0, pending[0].arrival - now))
-            continue
-        srv.step()
-        for rid in list(inflight):
-            out = srv.poll(rid)
-            if out is not None:
-                r = inflight.pop(rid)
-                done[r.index] = out
-                log(f"  batched: request {r.index} (N={r.nu.size}, eps={r.epsilon:g}) "
-                    f"{out[0].status} after {out[0].iterations} iterations, "
-                    f"{len(done)}/{len(reqs)} done")
-    srv.close()
-    lat_b = np.array([done[r.index][0].t_done - (t0 + r.arrival) for r in reqs])
-    span_b = max(done[r.index][0].t_done for r in reqs) - (t0 + min(r.arrival for r in reqs))
-    # FP64 check of every batched result
-    errs = []
-    for r in reqs:
-        s.setup_target(r.y, r.nu)
-        s.configure(sd.SolverParameters(epsilon=r.epsilon, tau=0.0, use_log_sum_exp_trick=True))
-        _, g, _ = s.evaluate_functional_R(dmax, done[r.index][1])
-        errs.append(float(np.max(np.abs(g) / r.nu)))
-    ok_b = sum(e <= r.tolerance * 1.001 for e, r in zip(errs, reqs))
-    log(f"serving batched fp32: {len(reqs)} solves in {span_b:.2f}s; iterations "
-        f"{[done[r.index][0].iterations for r in reqs]}; fp64 relerr max {max(errs):.3e}")
+    def batched(rs, arrivals, tag):
+        srv = sd.SotServer(s, precision="fp32_lse", max_slots=256, hbm_budget_bytes=96 << 30,
+                           polish_factor=args.serve_polish)
+        tuned = srv.autotune()
+        t0 = srv.clock()
+        order = sorted(range(len(rs)), key=lambda i: arrivals[i])
+        inflight, done = {}, {}
+        while order or inflight:
+            now = srv.clock() - t0
+            while order and arrivals[order[0]] <= now:
+                i = order.pop(0)
+                r = rs[i]
+                inflight[srv.submit(r.y, r.nu, r.epsilon, r.tolerance, **kw)] = i
+            if not inflight:
+                time.sleep(max(0.0, arrivals[order[0]] - now))
+                continue
+            srv.step()
+            for rid in list(inflight):
+                out = srv.poll(rid)
+                if out is not None:
+                    i = inflight.pop(rid)
+                    done[i] = out
+        srv.close()
+        lat = np.array([done[i][0].t_done - (t0 + arrivals[i]) for i in range(len(rs))])
+        span = max(done[i][0].t_done for i in range(len(rs))) - (t0 + min(arrivals))
+        its = [done[i][0].iterations for i in range(len(rs))]
+        log(f"serving batched fp32 ({tag}): {len(rs)} solves in {span:.2f}s, tuned item sizes "
+            f"{tuned}; iterations {its}")
+        return done, lat, span, tuned, its
 
-    # one-at-a-time FP64 single-solve path
-    def one_at_a_time(precond):
-        end, lat, ok, its = 0.0, [], 0, []
-        for r in sorted(reqs, key=lambda r: r.arrival):
+    def fp64_check(rs, done):
+        errs = []
+        for i, r in enumerate(rs):
+            s.setup_target(r.y, r.nu)
+            s.configure(sd.SolverParameters(epsilon=r.epsilon, tau=0.0,
+                                            use_log_sum_exp_trick=True))
+            _, g, _ = s.evaluate_functional_R(dmax, done[i][1])
+            errs.append(float(np.max(np.abs(g) / r.nu)))
+        return errs, sum(e <= r.tolerance * 1.001 for e, r in zip(errs, rs))
+
+    # capacity: everything available at t = 0
+    done_c, lat_c, span_c, tuned, its_c = batched(reqs, [0.0] * len(reqs), "capacity")
+    errs_c, ok_c = fp64_check(reqs, done_c)
+
+    def one_at_a_time(rs, precond):
+        service, ok, its = [], 0, []
+        for r in rs:
             ta = time.perf_counter()
             s.setup_target(r.y, r.nu)
             prm = sd.SolverParameters(epsilon=r.epsilon, tau=0.0, tolerance=r.tolerance,
@@ -297,41 +350,138 @@ def serving_leg(s, args):
                                       epsilon_scaling_steps=max(1, args.serve_eps_steps),
                                       epsilon_scaling_factor=2.0)
             psi, infos = s.solve_epsilon_scaling(None, prm)
-            dt = time.perf_counter() - ta
-            end = max(end, r.arrival) + dt
-            lat.append(end - r.arrival)
+            service.append(time.perf_counter() - ta)
             ok += infos[-1]["status"] == "converged"
             its.append(sum(i["iterations"] for i in infos))
-            log(f"  one-at-a-time ({precond}): request {r.index} (N={r.nu.size}) "
-                f"{infos[-1]['status']} after {its[-1]} iterations, {dt:.2f}s")
-        span = end - min(r.arrival for r in reqs)
-        lat = np.array(lat)
-        log(f"serving one-at-a-time fp64 ({precond}): {len(reqs)} solves in {span:.2f}s; "
+        log(f"serving one-at-a-time fp64 ({precond}): {len(rs)} solves in {sum(service):.2f}s; "
             f"iterations {its}")
-        return span, {"solves_per_s": len(reqs) / span, "p50_s": float(np.percentile(lat, 50)),
-                      "p99_s": float(np.percentile(lat, 99)), "within_tol": int(ok),
-                      "preconditioner": precond, "iterations": its}
+        return np.array(service), int(ok), its
 
-    span_s, single = one_at_a_time(args.serve_precond)
+    def queue_latency(service, arrivals):
+        end, lat = 0.0, []
+        for t_arr, dt in zip(arrivals, service):
+            end = max(end, t_arr) + dt
+            lat.append(end - t_arr)
+        return np.array(lat), end - min(arrivals)
+
+    serv_s, ok_s, its_s = one_at_a_time(reqs, args.serve_precond)
+    lat_sc, span_sc = queue_latency(serv_s, [0.0] * len(reqs))
+    cap_b, cap_s = len(reqs) / span_c, len(reqs) / span_sc
+
+    # steady state: Poisson arrivals at serve_load x the batched capacity
+    m = min(args.serve_steady, len(reqs))
+    steady = None
+    if m > 0:
+        rate = args.serve_load * cap_b
+        base = np.array([r.arrival for r in reqs[:m]])
+        arr = list((base - base[0]) * (50.0 / rate))  # the trace's gaps are at 50/s
+        done_t, lat_t, span_t, _, its_t = batched(reqs[:m], arr, "steady")
+        errs_t, ok_t = fp64_check(reqs[:m], done_t)
+        lat_st, _ = queue_latency(serv_s[:m], arr)
+        steady = {"requests": m, "arrival_rate_per_s": rate,
+                  "batched_fp32": {"solves_per_s": m / span_t,
+                                   "p50_s": float(np.percentile(lat_t, 50)),
+                                   "p99_s": float(np.percentile(lat_t, 99)),
+                                   "within_tol": int(ok_t),
+                                   "max_fp64_rel_marginal_err": max(errs_t)},
+                  "one_at_a_time_fp64": {"p50_s": float(np.percentile(lat_st, 50)),
+                                         "p99_s": float(np.percentile(lat_st, 99))}}
+        log(f"serving steady state at {rate:.2f}/s: batched p50 {steady['batched_fp32']['p50_s']:.2f}s "
+            f"p99 {steady['batched_fp32']['p99_s']:.2f}s; one-at-a-time p50 "
+            f"{steady['one_at_a_time_fp64']['p50_s']:.2f}s p99 {steady['one_at_a_time_fp64']['p99_s']:.2f}s")
     extra = {}
     if args.serve_identity_leg and args.serve_precond != "none":
-        span_i, ident = one_at_a_time("none")
-        extra = {"one_at_a_time_fp64_identity": ident, "speedup_vs_identity": span_i / span_b}
+        k = min(8, len(reqs))
+        serv_i, ok_i, its_i = one_at_a_time(reqs[:k], "none")
+        extra = {"one_at_a_time_fp64_identity_first8": {
+            "solves_per_s": k / float(np.sum(serv_i)), "within_tol": ok_i, "iterations": its_i,
+            "speedup_of_batched": (k / float(np.sum(serv_i))) ** -1 * cap_b}}
     return {
-        "workload": f"first {len(reqs)} requests of the seeded serving trace (seed 4, Poisson 50/s, "
-                    "N logU[256,65536], eps {1e-2,3e-3,1e-3}, tol 1e-6 componentwise) on the "
-                    "resident 32^3 Q1 / QGauss(2) mesh",
+        "workload": f"first {len(reqs)} requests of the seeded serving trace (seed 4; N logU[256,65536], "
+                    "eps {1e-2,3e-3,1e-3}, tol 1e-6 componentwise) on the resident 32^3 Q1 / "
+                    "QGauss(2) mesh",
         "max_iterations": args.serve_max_it, "epsilon_scaling_steps": args.serve_eps_steps,
-        "polish_factor": args.serve_polish,
-        "batched_fp32": {"solves_per_s": len(reqs) / span_b, "p50_s": float(np.percentile(lat_b, 50)),
-                         "p99_s": float(np.percentile(lat_b, 99)), "within_tol": int(ok_b),
-                         "max_fp64_rel_marginal_err": max(errs),
-                         "preconditioner": args.serve_precond,
-                         "iterations": [done[r.index][0].iterations for r in reqs]},
-        "one_at_a_time_fp64": single,
-        "speedup": span_s / span_b,
+        "polish_factor": args.serve_polish, "preconditioner": args.serve_precond,
+        "autotuned_item_sizes": {"pass_targets": tuned[0], "grad_qpoints": tuned[1]},
+        "capacity": {
+            "batched_fp32": {"solves_per_s": cap_b, "p50_s": float(np.percentile(lat_c, 50)),
+                             "p99_s": float(np.percentile(lat_c, 99)), "within_tol": int(ok_c),
+                             "max_fp64_rel_marginal_err": max(errs_c), "iterations": its_c},
+            "one_at_a_time_fp64": {"solves_per_s": cap_s,
+                                   "p50_s": float(np.percentile(lat_sc, 50)),
+                                   "p99_s": float(np.percentile(lat_sc, 99)),
+                                   "within_tol": ok_s, "iterations": its_s}},
+        "speedup": cap_b / cap_s,
+        "steady": steady,
         **extra,
     }
+
+
+def fp32_vs_fp64_at(s, wl, base, R, psi64, psi32, solves):
+    """FP32_LSE against FP64 at the FP64 solution psi64 (threshold R): the
+    functional, the gradient relative to nu, C_global and the transport cost
+    <c, pi> = sum_q w_q |x_q|^2/2 - sum_j nu_j y_j . b_j + sum_j m_j |y_j|^2/2
+    (b_j the weighted barycenters, m_j = g_j + nu_j), and the two solved
+    potentials (mean-shifted) with their FP64 relative marginal errors."""
+    import semidiscreteot_amd as sd
+    x = wl.source.x.reshape(-1, 3)
+    y = wl.y.reshape(-1, 3)
+    sx2 = float(np.sum(wl.source.w * 0.5 * np.sum(x * x, axis=1)))
+    vals = {}
+    for prec in ("fp64", "fp32_lse"):
+        s.configure(sd.SolverParameters(precision=prec, **base))
+        F, g, C = s.evaluate_functional_R(R, psi64)
+        b = s.compute_weighted_barycenters_R(R, psi64)
+        m = g + wl.nu
+        cost = sx2 - float(np.sum(wl.nu[:, None] * y * b)) + float(np.sum(m * 0.5 * np.sum(y * y, 1)))
+        vals[prec] = (F, g, C, cost)
+    F64, g64, C64, c64 = vals["fp64"]
+    F32, g32, C32, c32 = vals["fp32_lse"]
+    a, b = psi64 - psi64.mean(), psi32 - psi32.mean()
+    s.configure(sd.SolverParameters(precision="fp64", **base))
+    _, g_at32, _ = s.evaluate_functional_R(R, psi32)
+    return {
+        "functional_rel": abs(F32 - F64) / abs(F64),
+        "gradient_max_rel_to_nu": float(np.max(np.abs(g32 - g64) / wl.nu)),
+        "gradient_inf_norm_rel": float(np.max(np.abs(g32 - g64)) / max(np.max(np.abs(g64)), 1e-300)),
+        "C_global_rel": abs(C32 - C64) / abs(C64),
+        "transport_cost_rel": abs(c32 - c64) / abs(c64),
+        "psi_max_abs": float(np.max(np.abs(a - b))),
+        "psi_max_abs_rel_to_range": float(np.max(np.abs(a - b)) / (a.max() - a.min())),
+        "fp64_rel_marginal_err_of_fp32_solution": float(np.max(np.abs(g_at32) / wl.nu)),
+        "fp64_rel_marginal_err_of_fp64_solution": float(np.max(np.abs(g64) / wl.nu)),
+        "transport_cost_fp64": c64, "R": R, "solves": solves}
+
+
+def precision_table(s, wl, psi0, args):
+    """The 3D Gaussian-mixture workload solved twice from psi0 (componentwise
+    1e-6, untruncated, noise-aware search, diagonal initial inverse Hessian):
+    FP64, and FP32_LSE finished on FP64 evaluations; then fp32_vs_fp64_at the
+    FP64 solution.  Untruncated (tau = 0): the pointwise threshold follows
+    max_j psi_j from one evaluation to the next, and on this workload (target
+    clusters where the source density is ~1e-20 of its peak) its radius
+    discards mass that the componentwise criterion resolves, so the truncated
+    problem changes under the line search."""
+    import semidiscreteot_amd as sd
+    base = dict(epsilon=wl.epsilon, tau=0.0, tolerance=1e-6,
+                solver_control_type="componentwise", max_iterations=args.table_max_it,
+                use_log_sum_exp_trick=True, line_search="noise_aware",
+                preconditioner="diagonal", verbose_output=False)
+    sol, solves = {}, {}
+    for prec in ("fp64", "fp32_lse"):
+        s.configure(sd.SolverParameters(precision=prec, **base))
+        ta = time.perf_counter()
+        psi = s.solve(psi0.copy())
+        info = s.get_last_solve_info()
+        sol[prec] = psi
+        solves[prec] = {"status": info["status"], "iterations": info["iterations"],
+                        "evaluations": info["evaluations"], "seconds": time.perf_counter() - ta}
+        log(f"  {prec} solve: {info['status']} after {info['iterations']} iterations in "
+            f"{solves[prec]['seconds']:.2f}s")
+    R = float(np.finfo(float).max)
+    t = fp32_vs_fp64_at(s, wl, base, R, sol["fp64"], sol["fp32_lse"], solves)
+    log(f"  fp32 vs fp64: {json.dumps({k: v for k, v in t.items() if k != 'solves'})}")
+    return t
 
 
 def multilevel_leg(args):
@@ -513,25 +663,32 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--modes", default="fp64,fp32_lse",
                     help="comma list of precisions to time (fp32_lse is the reported one)")
-    ap.add_argument("--serve-requests", type=int, default=8,
-                    help="requests of the serving trace to replay (0: skip the serving leg)")
+    ap.add_argument("--serve-requests", type=int, default=64,
+                    help="requests of the serving trace (capacity leg; 0: skip the serving leg)")
+    ap.add_argument("--serve-steady", type=int, default=32,
+                    help="requests of the steady-state (Poisson arrival) serving leg")
+    ap.add_argument("--serve-load", type=float, default=0.7,
+                    help="steady-state arrival rate as a fraction of the measured batched capacity")
     ap.add_argument("--serve-max-it", type=int, default=1000)
     ap.add_argument("--serve-precond", default="diagonal", choices=["none", "diagonal"],
                     help="initial inverse Hessian of both serving legs")
     ap.add_argument("--serve-polish", type=float, default=2.0,
                     help="batched FP32 solves switch to FP64 within this factor of the tolerance")
-    ap.add_argument("--serve-identity-leg", type=int, default=1,
+    ap.add_argument("--serve-identity-leg", type=int, default=0,
                     help="also time the one-at-a-time FP64 path with the identity")
     ap.add_argument("--serve-eps-steps", type=int, default=1,
                     help="epsilon-scaling stages per request (1: none)")
-    ap.add_argument("--multilevel", action="store_true",
-                    help="also run the combined multilevel workload end to end")
+    ap.add_argument("--no-multilevel", action="store_true",
+                    help="skip the combined multilevel workload")
     ap.add_argument("--ml-max-it", type=int, default=3000,
                     help="iteration budget of each multilevel solve")
     ap.add_argument("--ml-precond", default="diagonal", choices=["none", "diagonal"],
                     help="initial inverse Hessian of the multilevel solves")
-    ap.add_argument("--tets", action="store_true",
-                    help="also run the large unstructured (Delaunay ball) workload")
+    ap.add_argument("--no-tets", action="store_true",
+                    help="skip the large unstructured (Delaunay ball) workload")
+    ap.add_argument("--no-solve-table", action="store_true",
+                    help="skip the FP32-vs-FP64 solves of the 3D workload")
+    ap.add_argument("--table-max-it", type=int, default=2000)
     ap.add_argument("--tets-max-it", type=int, default=2000)
     ap.add_argument("--tets-steps", type=int, default=5)
     ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_BYTES"),
@@ -623,6 +780,33 @@ def main():
             + ", ".join(f"{k}={prof[k] / max(1, prof['evaluations']):.3f}ms"
                         for k in ("ms_lse_max", "ms_lse_sum", "ms_qfin", "ms_grad")))
 
+    # cold FP32 evaluations: potentials far apart, so that no work item can be
+    # seeded from the previous evaluation's maxima (every item runs its max loop)
+    psi_far = psi + 2.0 * wl.epsilon * np.random.default_rng(8).standard_normal(psi.size)
+    d_far = devmem.DeviceArray.from_host(psi_far)
+    params.precision = "fp32_lse"
+    s.configure(params)
+    s.eval_device(R, d_far.address, d_grad.address, d_fc.address)
+    s.synchronize()
+    s.profile_enable(True)
+    s.synchronize()
+    ta = time.perf_counter()
+    n_cold = max(2, args.steps // 2)
+    for i in range(n_cold):
+        s.eval_device(R, (d_psi[0] if i % 2 else d_far).address, d_grad.address, d_fc.address)
+    s.synchronize()
+    ms_cold = (time.perf_counter() - ta) * 1e3 / n_cold
+    prof_c = s.profile_get()
+    s.profile_enable(False)
+    cold = {"ms_per_eval": ms_cold, "pair_evals_per_s": pairs / (ms_cold * 1e-3),
+            "kernel_ms": {k: prof_c[k] / max(1, prof_c["evaluations"])
+                          for k in ("ms_lse_max", "ms_lse_sum", "ms_qfin", "ms_grad")}}
+    log(f"fp32_lse cold (unseeded): {ms_cold:.3f} ms/eval; kernels {cold['kernel_ms']}")
+    solve_table = None
+    if not args.no_solve_table:
+        log("fp32 vs fp64 solves ...")
+        solve_table = precision_table(s, wl, psi, args)
+
     r = results["fp32_lse"]
     ev = max(1, r["prof"]["evaluations"])
     kern = {"lse_max": r["prof"]["ms_lse_max"] / ev, "lse_sum": r["prof"]["ms_lse_sum"] / ev,
@@ -631,17 +815,20 @@ def main():
     achieved = pairs / (kern[dom] * 1e-3)
     peak = pass_peak_pairs(3)
     eval_frac = (pairs / (r["ms"] * 1e-3)) / (peak / 2.0)
+    rates = measured_rates()
+    if rates:
+        log(f"measured rates: {json.dumps(rates)}")
     serving = None
     if args.serve_requests > 0:
         log("serving leg ...")
         serving = serving_leg(s, args)
     multilevel = None
-    if args.multilevel:
+    if not args.no_multilevel:
         log("multilevel leg ...")
         multilevel = multilevel_leg(args)
     plain2d = plain_2d_leg()
     tets, tets_cpu = None, None
-    if args.tets:
+    if not args.no_tets:
         log("tets leg ...")
         tets, tets_cpu = tets_leg(args)
     cpu = None
@@ -673,6 +860,9 @@ def main():
                      "peak_basis": "per pair sweep 1/max(1/R_exp,(d+3)/R_fma) with R_fma = 3.93e13 FMA/s "
                                    "(published 78.6 TFLOP/s FP64/FP32 vector) and R_exp >= R_fma/4; "
                                    "an evaluation is two sweeps (eval_frac)"},
+        "fp32_cold": cold,
+        "fp32_vs_fp64": solve_table,
+        "measured_rates": rates,
         "fp64": None if "fp64" not in results else {
             "ms_per_step": results["fp64"]["ms"],
             "value": pairs / (results["fp64"]["ms"] * 1e-3),
