@@ -215,6 +215,18 @@ int sdot_eval_R(sdot_ctx *ctx, double R, const double *psi, double *grad, double
  * d_psi, d_grad: N doubles in original target order; d_FC: 2 doubles (F, C).
  * Asynchronous on the context stream (sdot_synchronize to wait). */
 int sdot_eval_device(sdot_ctx *ctx, double R, const double *d_psi, double *d_grad, double *d_FC);
+/* No counterpart in the reference: sdot_eval_device for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and results; it calls no
+ * hipMalloc, hipFree or other allocating routine.  Every per-evaluation
+ * buffer comes from one device array of 128 MiB per GPU (held until the
+ * process ends) shared by every call on that GPU, from every context: calls
+ * that overlap on one GPU race.  The FP32_LSE sweep runs unseeded; the segment
+ * counts shrink until the buffers fit, and an evaluation that does not fit
+ * with one segment per sweep (about 160 N + 96 nQ bytes) returns
+ * SDOT_ERR_INVALID.  Nothing in the array has to start at zero. */
+int sdot_eval_device_noalloc(sdot_ctx *ctx, double R, const double *d_psi, double *d_grad,
+                             double *d_FC);
 int sdot_synchronize(sdot_ctx *ctx);
 
 /* SotSolver::evaluate_weighted_barycenters / compute_weighted_barycenters_euclidean

@@ -130,6 +130,8 @@ SIGNATURES = {
     "sdot_eval": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp]),
     "sdot_eval_R": (C.c_int, [C.c_void_p, C.c_double, _dp, _dp, _dp, _dp]),
     "sdot_eval_device": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "sdot_eval_device_noalloc": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p,
+                                           C.c_void_p]),
     "sdot_synchronize": (C.c_int, [C.c_void_p]),
     "sdot_hessian": (C.c_int, [C.c_void_p, _dp, _dp, C.c_int64]),
     "sdot_barycenters": (C.c_int, [C.c_void_p, _dp, _dp]),

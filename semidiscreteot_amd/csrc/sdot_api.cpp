@@ -243,6 +243,7 @@ struct sdot_ctx {
   int n_servers = 0;        // serving queues reading this context's source arrays
   double seed_bias = 0.0;   // test hooks (sdot_ctx_create)
   int force_K1 = 0, force_K2 = 0;
+  double *noalloc = nullptr;  // this device's no-allocation scratch (kNoAllocBytes)
   Plan plan;
   Dev dev;
   std::vector<double> h_nu;
@@ -285,8 +286,9 @@ int resolve_profile(sdot_ctx *c) {
   return SDOT_OK;
 }
 
-// Choose segment counts and make sure every per-evaluation buffer fits.
-int prepare(sdot_ctx *c, double R) {
+// The evaluation plan at radius R: arithmetic mode, candidate rule, grouping
+// and segment counts.
+int plan_eval(sdot_ctx *c, double R) {
   if (!c->have_source || !c->have_target)
     return fail(SDOT_ERR_STATE, "Invalid measures configuration: source and target must be set");
   Plan &p = c->plan;
@@ -352,6 +354,15 @@ int prepare(sdot_ctx *c, double R) {
   if (c->force_K2 > 0) p.K2 = std::min(c->force_K2, kMaxSeg);
   p.nFpart = (p.nQ + 255) / 256;
   p.nPpart = (p.N + 255) / 256;
+  return SDOT_OK;
+}
+
+// The plan at radius R with every per-evaluation buffer of the context large
+// enough for it.
+int prepare(sdot_ctx *c, double R) {
+  int rc = plan_eval(c, R);
+  if (rc) return rc;
+  Plan &p = c->plan;
   Dev &d = c->dev;
   const size_t needM = (size_t)p.K1 * p.nQ, needG = (size_t)p.K2 * p.N;
   if (c->cap_M < needM) {
@@ -423,6 +434,68 @@ int run_eval(sdot_ctx *c, double R, const double *d_psi, double *d_grad, double 
     c->hist_valid = true;
     c->hist_R = R;
   }
+  return SDOT_OK;
+}
+
+// The per-evaluation buffers of plan p laid out in one block at `base`
+// (sub-arrays on 256-byte boundaries): d takes the context's source and target
+// arrays from `ctx` and the evaluation's own arrays from the block.  Returns
+// the block's size in bytes; base == nullptr sizes it only.
+size_t carve_eval(const Plan &p, const Dev &ctx, double *base, Dev &d) {
+  size_t off = 0;  // doubles
+  auto take = [&](double *&dst, size_t n) {
+    dst = base ? base + off : nullptr;
+    off += (std::max<size_t>(n, 1) + 31) / 32 * 32;
+  };
+  const size_t N = (size_t)p.N, nQ = (size_t)p.nQ, M = (size_t)p.K1 * nQ;
+  d = ctx;
+  take(d.trec_r, N * kRec);
+  take(d.trec_m, N * kRec);
+  take(d.qrec_r, nQ * kRec);
+  take(d.Mpart, M);
+  take(d.Spart, M);
+  take(d.Tpart, M);
+  take(d.Mglob, nQ);
+  take(d.Gpart, (size_t)p.K2 * N);
+  take(d.Fpart, (size_t)p.nFpart);
+  take(d.Cpart, (size_t)p.nFpart);
+  take(d.Upart, (size_t)p.nFpart);
+  take(d.PNpart, (size_t)p.nPpart);
+  take(d.psi_prev, N);
+  take(d.dpsi_r, N);
+  take(d.dpsi_cx, (N + 63) / 64);
+  take(d.dpsi_cn, (N + 63) / 64);
+  take(d.out, 4);
+  take(d.ucount, 1);
+  return off * sizeof(double);
+}
+
+// sdot_eval_device without device-memory allocation: the evaluation's buffers
+// are carved from this device's scratch array.  The FP32 sweep runs unseeded
+// (the scratch keeps no history that belongs to this context), and the segment
+// counts halve, the larger partial array first, until the buffers fit.
+int run_eval_noalloc(sdot_ctx *c, double R, const double *d_psi, double *d_grad, double *d_FC) {
+  if (!c->noalloc) return fail(SDOT_ERR_STATE, "no-allocation scratch is not set up on this device");
+  int rc = plan_eval(c, R);
+  if (rc) return rc;
+  Plan p = c->plan;
+  p.hist = 0;
+  Dev d;
+  size_t need = carve_eval(p, c->dev, nullptr, d);
+  while (need > kNoAllocBytes && (p.K1 > 1 || p.K2 > 1)) {
+    const size_t m = 3 * (size_t)p.K1 * (size_t)p.nQ, g = (size_t)p.K2 * (size_t)p.N;
+    if (p.K1 > 1 && (m >= g || p.K2 == 1))
+      p.K1 = (p.K1 + 1) / 2;
+    else
+      p.K2 = (p.K2 + 1) / 2;
+    need = carve_eval(p, c->dev, nullptr, d);
+  }
+  if (need > kNoAllocBytes)
+    return fail(SDOT_ERR_INVALID,
+                "evaluation needs " + std::to_string(need) + " bytes of scratch with one segment per "
+                "sweep; the no-allocation entry point holds " + std::to_string(kNoAllocBytes));
+  carve_eval(p, c->dev, c->noalloc, d);
+  HIP_TRY(launch_eval(p, d, d_psi, d_grad, d_FC, c->stream, nullptr));
   return SDOT_OK;
 }
 
@@ -621,6 +694,13 @@ extern "C" int sdot_ctx_create(int device, sdot_ctx **out) {
   if (e != hipSuccess) {
     delete c;
     return fail(SDOT_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  // the no-allocation entry point's scratch: its address on this device
+  e = noalloc_scratch(device, &c->noalloc);
+  if (e != hipSuccess) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return fail(SDOT_ERR_HIP, std::string("no-allocation scratch lookup: ") + hipGetErrorString(e));
   }
   sdot_params_default(&c->params);
   // test hook: SDOT_TEST_SEED_BIAS raises every seeded reference value of the
@@ -903,6 +983,14 @@ extern "C" int sdot_eval_device(sdot_ctx *c, double R, const double *d_psi, doub
   if (!d_psi || !d_grad || !d_FC) return fail(SDOT_ERR_INVALID, "null device pointer");
   HIP_TRY(hipSetDevice(c->device));
   return run_eval(c, R, d_psi, d_grad, d_FC);
+}
+
+extern "C" int sdot_eval_device_noalloc(sdot_ctx *c, double R, const double *d_psi,
+                                        double *d_grad, double *d_FC) {
+  if (ctx_check(c)) return SDOT_ERR_INVALID;
+  if (!d_psi || !d_grad || !d_FC) return fail(SDOT_ERR_INVALID, "null device pointer");
+  HIP_TRY(hipSetDevice(c->device));
+  return run_eval_noalloc(c, R, d_psi, d_grad, d_FC);
 }
 
 extern "C" int sdot_synchronize(sdot_ctx *c) {

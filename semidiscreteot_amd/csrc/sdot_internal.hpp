@@ -323,6 +323,14 @@ struct PtArgs {
 // op 0: conditional densities, 1: barycentric map, 2: modal map.
 hipError_t launch_points(int dim, const PtArgs &a, int op, hipStream_t s);
 
+// The evaluation that allocates nothing (sdot_eval_device_noalloc) carves its
+// per-evaluation buffers from one file-scope device array of this many bytes
+// per GPU (sdot_kernels.hip), which each GPU holds until the process ends.
+constexpr size_t kNoAllocBytes = size_t(128) << 20;
+// Device address of that array on `device`, which must be the current device;
+// looked up with hipGetSymbolAddress on the first call per device and kept.
+hipError_t noalloc_scratch(int device, double **ptr);
+
 // Kernel launchers (sdot_kernels.hip).
 hipError_t launch_eval(const Plan &p, Dev &d, const double *d_psi, double *d_grad,
                        double *d_FC, hipStream_t s, Profile *prof);

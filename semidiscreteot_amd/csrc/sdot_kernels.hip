@@ -33,10 +33,41 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
+#include <vector>
 
 #include "sdot_device.hpp"
 
 namespace sdot {
+
+// ---------------------------------------------------------------------------
+// scratch of the evaluation that allocates nothing (sdot_eval_device_noalloc)
+// ---------------------------------------------------------------------------
+// Each GPU holds its own copy at its own address.  In host code the name is a
+// host-side stand-in, never a device address: host code reaches the array
+// only through the address noalloc_scratch looks up.
+static __device__ __align__(256) double scratch[kNoAllocBytes / sizeof(double)];
+
+hipError_t noalloc_scratch(int device, double **ptr) {
+  static std::mutex mu;
+  static std::vector<double *> addr;  // per device; null until looked up
+  if (device < 0 || !ptr) return hipErrorInvalidValue;
+  std::lock_guard<std::mutex> lock(mu);
+  if ((size_t)device >= addr.size()) addr.resize((size_t)device + 1, nullptr);
+  if (!addr[device]) {
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device) return hipErrorInvalidDevice;
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    addr[device] = static_cast<double *>(p);
+  }
+  *ptr = addr[device];
+  return hipSuccess;
+}
 
 // ---------------------------------------------------------------------------
 // single-solve entry points (grid = the solve's own blocks)

@@ -339,6 +339,12 @@ class SotSolver:
         L.check(self.lib.sdot_eval_device(self._h, float(R), C.c_void_p(d_psi),
                                           C.c_void_p(d_grad), C.c_void_p(d_FC)))
 
+    def eval_device_noalloc(self, R, d_psi: int, d_grad: int, d_FC: int):
+        """eval_device with every per-evaluation buffer taken from the
+        library's fixed 128 MiB device array (no device allocation)."""
+        L.check(self.lib.sdot_eval_device_noalloc(self._h, float(R), C.c_void_p(d_psi),
+                                                  C.c_void_p(d_grad), C.c_void_p(d_FC)))
+
     def synchronize(self):
         L.check(self.lib.sdot_synchronize(self._h))
 
