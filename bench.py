@@ -546,6 +546,55 @@ def multilevel_leg(args):
                         "threshold": r.threshold} for r in recs]}
 
 
+HESSIAN_PEAK = 78.6e12  # published MI355X FP64 matrix (MFMA) rate, FLOP/s
+
+
+def hessian_leg(ns):
+    """Dense Hessian (sdot_hessian: P-build + FP64 MFMA SYRK) on the 16^3
+    QGauss(2) mixture source (32,768 q-points), untruncated, for each N in
+    ns: wall time of the call (including the N x N device-to-host copy) and
+    the device time of its chunks from the context's profile.  Algorithmic
+    flops N (N + 1) nQ (the upper triangle and diagonal of P^T P, two flops
+    per product); executed flops count the padding to 128-column blocks and
+    16-row stages."""
+    import semidiscreteot_amd as sd
+    from semidiscreteot_amd import workloads
+    out = []
+    for N in ns:
+        wl = workloads.gmm_3d(n_cells=16, n_targets=N)
+        n_q = wl.source.x.size // 3
+        s = sd.SotSolver(3)
+        s.setup_source(wl.source.x, wl.source.w, wl.source.centers, wl.source.nq)
+        s.setup_target(wl.y, wl.nu)
+        s.configure(sd.SolverParameters(epsilon=1e-2, tau=0.0, use_log_sum_exp_trick=True,
+                                        verbose_output=False))
+        psi = 1e-3 * np.random.default_rng(N).standard_normal(N)
+        s.compute_hessian(psi)
+        s.profile_enable(True)
+        ta = time.perf_counter()
+        H = s.compute_hessian(psi)
+        wall = time.perf_counter() - ta
+        prof = s.profile_get()
+        s.profile_enable(False)
+        rowsum = float(np.max(np.abs(H.sum(1))) / np.max(np.abs(H)))
+        del H
+        s.close()
+        npad = (N + 127) // 128 * 128
+        nb = npad // 128
+        rows = min(max(16, (1 << 30) // (8 * npad) // 16 * 16), (n_q + 15) // 16 * 16)
+        executed = nb * (nb + 1) / 2 * 128 * 128 * 2.0 * rows * ((n_q + rows - 1) // rows)
+        alg = float(N) * (N + 1) * n_q
+        ms = prof["ms_hessian"] / max(1, prof["hessians"])
+        rec = {"n_targets": N, "n_q": n_q, "wall_ms": wall * 1e3, "device_ms": ms,
+               "algorithmic_tflops": alg / (ms * 1e-3) / 1e12,
+               "executed_tflops": executed / (ms * 1e-3) / 1e12,
+               "frac_of_fp64_mfma_peak": executed / (ms * 1e-3) / HESSIAN_PEAK,
+               "max_rowsum_rel": rowsum}
+        log(f"hessian N={N}: {json.dumps(rec)}")
+        out.append(rec)
+    return out
+
+
 def tets_leg(args):
     """Large unstructured workload (Delaunay tetrahedra of 200,000 points in
     the unit ball, P1 density 1 + 0.5 sin(4 pi x0), 4-point rule, 500,000
@@ -691,6 +740,10 @@ def main():
     ap.add_argument("--table-max-it", type=int, default=2000)
     ap.add_argument("--tets-max-it", type=int, default=2000)
     ap.add_argument("--tets-steps", type=int, default=5)
+    ap.add_argument("--hessian", default="16384",
+                    help="comma list of target counts of the dense-Hessian leg ('' to skip)")
+    ap.add_argument("--hessian-only", action="store_true",
+                    help="run the dense-Hessian leg alone and print its JSON")
     ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_BYTES"),
                     help="summarise two rocprofv3 --pmc runs into per-kernel HBM traffic and exit")
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -705,6 +758,11 @@ def main():
 
     import semidiscreteot_amd as sd
     from semidiscreteot_amd import devmem, workloads
+
+    hess_ns = [int(n) for n in args.hessian.split(",") if n.strip()]
+    if args.hessian_only:
+        print(json.dumps({"hessian": hessian_leg(hess_ns)}), flush=True)
+        return
 
     t0 = time.perf_counter()
     wl = workloads.gmm_3d()
@@ -831,6 +889,10 @@ def main():
     if not args.no_tets:
         log("tets leg ...")
         tets, tets_cpu = tets_leg(args)
+    hessian = None
+    if hess_ns:
+        log("hessian leg ...")
+        hessian = hessian_leg(hess_ns)
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -872,6 +934,7 @@ def main():
         "multilevel": multilevel,
         "plain_2d_solve": plain2d,
         "large_unstructured": tets,
+        "hessian": hessian,
         "cpu_baseline": cpu,
     }
     if args.copy_bytes > 0:

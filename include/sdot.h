@@ -111,10 +111,14 @@ typedef struct {
   int64_t last_pair_count;       /* (q, candidate) pairs of the last evaluation, when counted */
 } sdot_state;
 
-/* Kernel timing gathered with HIP events on the context stream. */
+/* Kernel timing gathered with HIP events on the context stream.  The dense
+ * Hessian (sdot_hessian) adds its P-build + SYRK chunks, from the first chunk's
+ * start to the last one's end, to ms_hessian and counts the call. */
 typedef struct {
   int64_t evaluations;
   double ms_lse_max, ms_lse_sum, ms_qfin, ms_grad, ms_total;
+  int64_t hessians;
+  double ms_hessian;
 } sdot_profile;
 
 /* ---- library / context ---------------------------------------------- */
@@ -239,7 +243,8 @@ int sdot_barycenters_R(sdot_ctx *ctx, double R, const double *psi, double *bary)
 
 /* SotSolver::compute_hessian (declared at SotSolver.h:338-344, no reference
  * definition): dense H (row-major, leading dimension ld >= N) of the
- * functional at psi with the current threshold rule. */
+ * functional at psi with the current threshold rule.  N <= 65536
+ * (SDOT_ERR_UNSUPPORTED above). */
 int sdot_hessian(sdot_ctx *ctx, const double *psi, double *H, int64_t ld);
 
 /* SotSolver::solve (SotSolver.h:211-212, SotSolver.cc:105-215): L-BFGS with

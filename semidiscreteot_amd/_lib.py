@@ -86,7 +86,7 @@ class State(C.Structure):
 class Profile(C.Structure):
     _fields_ = [("evaluations", C.c_int64), ("ms_lse_max", C.c_double),
                 ("ms_lse_sum", C.c_double), ("ms_qfin", C.c_double), ("ms_grad", C.c_double),
-                ("ms_total", C.c_double)]
+                ("ms_total", C.c_double), ("hessians", C.c_int64), ("ms_hessian", C.c_double)]
 
 
 class ServerConfig(C.Structure):

@@ -1006,7 +1006,8 @@ extern "C" int sdot_hessian(sdot_ctx *c, const double *psi, double *H, int64_t l
   if (!psi || !H) return fail(SDOT_ERR_INVALID, "null argument");
   const int64_t N = c->plan.N;
   if (ld < N) return fail(SDOT_ERR_INVALID, "ld < N");
-  if (N > 32768) return fail(SDOT_ERR_UNSUPPORTED, "dense Hessian limited to N <= 32768");
+  // two N x N FP64 matrices on the device (69 GB at the limit) and one on the host
+  if (N > 65536) return fail(SDOT_ERR_UNSUPPORTED, "dense Hessian limited to N <= 65536");
   HIP_TRY(hipSetDevice(c->device));
   double R;
   int rc = threshold(c, psi, &R);
@@ -1018,27 +1019,43 @@ extern "C" int sdot_hessian(sdot_ctx *c, const double *psi, double *H, int64_t l
   c->force_mode = -1;
   if (rc) return rc;
   const Plan &p = c->plan;
-  const int64_t Npad = (N + 63) / 64 * 64;
-  const int64_t budget = (int64_t)1 << 28;  // bytes of one P chunk
-  int64_t rows = std::max<int64_t>(4, budget / (8 * Npad) / 4 * 4);
-  rows = std::min<int64_t>(rows, (p.nQ + 3) / 4 * 4);
+  // the SYRK kernel takes 128-column blocks and stages of 16 quadrature rows
+  const int64_t Npad = (N + 127) / 128 * 128;
+  const int64_t budget = (int64_t)1 << 30;  // bytes of one P chunk
+  int64_t rows = std::max<int64_t>(16, budget / (8 * Npad) / 16 * 16);
+  rows = std::min<int64_t>(rows, (p.nQ + 15) / 16 * 16);
   double *dA = nullptr, *dP = nullptr, *dH = nullptr;
   auto cleanup = [&]() {
     if (dA) (void)hipFree(dA);
     if (dP) (void)hipFree(dP);
     if (dH) (void)hipFree(dH);
   };
+  hipEvent_t ev[2] = {nullptr, nullptr};
   hipError_t e = hipMalloc((void **)&dA, sizeof(double) * Npad * Npad);
   if (e == hipSuccess) e = hipMalloc((void **)&dP, sizeof(double) * rows * Npad);
   if (e == hipSuccess) e = hipMalloc((void **)&dH, sizeof(double) * N * N);
   if (e == hipSuccess) e = hipMemsetAsync(dA, 0, sizeof(double) * Npad * Npad, c->stream);
+  if (e == hipSuccess && c->prof_on) {
+    e = hipEventCreate(&ev[0]);
+    if (e == hipSuccess) e = hipEventCreate(&ev[1]);
+    if (e == hipSuccess) e = hipEventRecord(ev[0], c->stream);
+  }
   for (int64_t q0 = 0; e == hipSuccess && q0 < p.nQ; q0 += rows)
     e = launch_hessian_chunk(p, c->dev, q0, rows, dP, dA, Npad, c->stream);
+  if (e == hipSuccess && ev[1]) e = hipEventRecord(ev[1], c->stream);
   if (e == hipSuccess) e = launch_hessian_final(p, dA, Npad, c->dev.grad, c->dev.tnu_o, dH, c->stream);
   if (e == hipSuccess)
     e = hipMemcpy2DAsync(H, sizeof(double) * ld, dH, sizeof(double) * N, sizeof(double) * N, N,
                          hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess && ev[1]) {
+    float ms = 0.0f;
+    e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+    c->prof_acc.ms_hessian += ms;
+    c->prof_acc.hessians += 1;
+  }
+  for (hipEvent_t x : ev)
+    if (x) (void)hipEventDestroy(x);
   cleanup();
   if (e != hipSuccess) return fail(SDOT_ERR_HIP, std::string("hessian: ") + hipGetErrorString(e));
   return SDOT_OK;

@@ -297,6 +297,27 @@ def test_hessian_matches_oracle(dim, tau):
     assert np.max(np.abs(H.sum(1))) <= 1e-10 * scale
 
 
+@pytest.mark.parametrize("tau", [0.0, 1e-6])
+def test_hessian_large_n_matches_oracle(tau):
+    """compute_hessian at N = 2,050 (17 SYRK column blocks of 128, the last
+    one padded; 4,096 q-points in one chunk of 16-row stages) against the
+    oracle's assembly, untruncated and truncated."""
+    say(f"hessian N=2050 tau={tau}")
+    src, tgt = structured_case(3, 8, 2, 2050, seed=23, gmm_source=True, gmm_targets=True,
+                               dirichlet=5)
+    psi = 0.005 * np.random.default_rng(5).standard_normal(tgt.n)
+    s = make(src, tgt, epsilon=1e-2, tau=tau, use_log_sum_exp_trick=True)
+    R = s.distance_threshold(psi)
+    H = s.compute_hessian(psi)
+    Ho = O.hessian(src, tgt, O.params(epsilon=1e-2, use_lse=1), R, psi)
+    scale = np.max(np.abs(Ho))
+    err = np.max(np.abs(H - Ho)) / scale
+    say(f"R {R:.4g}: max |H - H_ref| / max |H_ref| = {err:.3e}")
+    assert err <= 1e-10
+    assert np.max(np.abs(H - H.T)) <= 1e-12 * scale
+    assert np.max(np.abs(H.sum(1))) <= 1e-10 * scale
+
+
 def test_discarded_mass_audit():
     """sdot_q_log_partition: the discarded mass of a radius matches the oracle's
     direct count on sampled quadrature points."""
