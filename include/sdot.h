@@ -207,6 +207,13 @@ int sdot_set_source(sdot_ctx *ctx, int dim, int64_t n_cells, int nq,
                     const double *x, const double *w, const double *centers);
 /* SotSolver::setup_target (SotSolver.h:197-198, SotSolver.cc:55-61). */
 int sdot_set_target(sdot_ctx *ctx, int64_t n, const double *y, const double *nu);
+/* No counterpart in the reference (audit, like sdot_candidate_stats): the
+ * target layout sdot_set_target builds on the device.  pos_r / pos_m: N
+ * positions of each target (caller order) in row-major bin order and in Morton
+ * order; tstart: n_bins + 1 bin offsets; grid: origin[3], bin width, bins per
+ * axis[3].  Any output may be NULL. */
+int sdot_target_layout(sdot_ctx *ctx, int32_t *pos_r, int32_t *pos_m, int32_t *tstart,
+                       int64_t *n_bins, double *grid);
 
 /* ---- the hot path --------------------------------------------------------- */
 /* SotSolver::evaluate_functional (SotSolver.h:335-336, SotSolver.cc:217-377):

@@ -127,6 +127,8 @@ SIGNATURES = {
     "sdot_set_params": (C.c_int, [C.c_void_p, C.POINTER(Params)]),
     "sdot_set_source": (C.c_int, [C.c_void_p, C.c_int, C.c_int64, C.c_int, _dp, _dp, _dp]),
     "sdot_set_target": (C.c_int, [C.c_void_p, C.c_int64, _dp, _dp]),
+    "sdot_target_layout": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.POINTER(C.c_int64), _dp]),
     "sdot_eval": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp]),
     "sdot_eval_R": (C.c_int, [C.c_void_p, C.c_double, _dp, _dp, _dp, _dp]),
     "sdot_eval_device": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]),

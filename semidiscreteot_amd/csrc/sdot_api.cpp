@@ -62,22 +62,6 @@ bool all_finite(const double *a, int64_t n) {
 }
 
 // Morton (Z-order) key of a point inside [lo, hi].
-uint64_t morton_key(const double *p, const double *lo, const double *hi, int dim) {
-  const int bits = dim == 3 ? 21 : 31;
-  const double scale = (double)((1u << (bits - 1)) - 1) * 2.0;
-  uint64_t key = 0;
-  uint64_t q[3] = {0, 0, 0};
-  for (int d = 0; d < dim; ++d) {
-    const double ext = hi[d] - lo[d];
-    double u = ext > 0 ? (p[d] - lo[d]) / ext : 0.0;
-    u = std::min(1.0, std::max(0.0, u));
-    q[d] = (uint64_t)(u * scale);
-  }
-  for (int b = bits - 1; b >= 0; --b)
-    for (int d = dim - 1; d >= 0; --d) key = (key << 1) | ((q[d] >> b) & 1u);
-  return key;
-}
-
 // Uniform grid over [lo, hi] with about `per_bin` points per bin.
 Grid make_grid(const double *lo, const double *hi, int dim, int64_t n, double per_bin) {
   Grid g{};
@@ -103,12 +87,6 @@ Grid make_grid(const double *lo, const double *hi, int dim, int64_t n, double pe
     g.n[d] = d < dim ? (int)std::max(1.0, std::ceil(ext[d] / h)) : 1;
   }
   return g;
-}
-
-int bin_linear(const Grid &g, const double *p, int dim) {
-  int b[3] = {0, 0, 0};
-  for (int d = 0; d < dim; ++d) b[d] = bin_of(p[d], g.o[d], g.inv_h, g.n[d]);
-  return (b[2] * g.n[1] + b[1]) * g.n[0] + b[0];
 }
 
 constexpr int kMaxSeg = 128;
@@ -162,12 +140,12 @@ double sdot::s2max_for(double R) {
   return s;
 }
 
-// Orders of the target points (Morton for the gradient waves, row-major bins
-// for the q-major sweeps), the bin grid and the per-wave bounding boxes.
-void sdot::build_target_layout(int dim, int64_t n, const double *y, TargetLayout &L) {
-  L.n = n;
-  L.n_twaves = (n + kWave - 1) / kWave;
-  double lo[3], hi[3];
+// Bounding box, coordinate scale and bin grid of a target set: one pass over
+// the points.
+void sdot::target_shape(int dim, int64_t n, const double *y, TargetShape &S) {
+  S.n = n;
+  S.n_twaves = (n + kWave - 1) / kWave;
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
   for (int d = 0; d < dim; ++d) {
     lo[d] = INFINITY;
     hi[d] = -INFINITY;
@@ -177,54 +155,38 @@ void sdot::build_target_layout(int dim, int64_t n, const double *y, TargetLayout
       lo[d] = std::min(lo[d], y[j * dim + d]);
       hi[d] = std::max(hi[d], y[j * dim + d]);
     }
-  L.scale = 0.0;
+  S.scale = 0.0;
   for (int d = 0; d < 3; ++d) {
-    L.lo[d] = d < dim ? lo[d] : 0.0;
-    L.hi[d] = d < dim ? hi[d] : 0.0;
-    if (d < dim) L.scale = std::max({L.scale, std::fabs(lo[d]), std::fabs(hi[d])});
+    S.lo[d] = d < dim ? lo[d] : 0.0;
+    S.hi[d] = d < dim ? hi[d] : 0.0;
+    if (d < dim) S.scale = std::max({S.scale, std::fabs(lo[d]), std::fabs(hi[d])});
   }
-  // Morton order (gradient waves)
-  std::vector<uint64_t> key(n);
-  for (int64_t j = 0; j < n; ++j) key[j] = morton_key(y + j * dim, lo, hi, dim);
-  std::vector<int32_t> &m2o = L.m2o, &pos_m = L.pos_m;
-  m2o.resize(n);
-  pos_m.resize(n);
-  std::iota(m2o.begin(), m2o.end(), 0);
-  std::stable_sort(m2o.begin(), m2o.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
-  for (int64_t m = 0; m < n; ++m) pos_m[m2o[m]] = (int32_t)m;
-  std::vector<double> &box = L.twbox, &tym = L.tym;
-  box.assign((size_t)L.n_twaves * 2 * dim, 0.0);
-  tym.assign((size_t)n * 4, 0.0);
-  for (int64_t w = 0; w < L.n_twaves; ++w) {
-    double bl[3] = {INFINITY, INFINITY, INFINITY}, bh[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int64_t m = w * kWave; m < std::min<int64_t>(n, (w + 1) * kWave); ++m)
-      for (int d = 0; d < dim; ++d) {
-        bl[d] = std::min(bl[d], y[(int64_t)m2o[m] * dim + d]);
-        bh[d] = std::max(bh[d], y[(int64_t)m2o[m] * dim + d]);
-      }
-    for (int d = 0; d < dim; ++d) {
-      box[w * 2 * dim + d] = bl[d];
-      box[w * 2 * dim + dim + d] = bh[d];
-    }
-  }
-  for (int64_t m = 0; m < n; ++m)
-    for (int d = 0; d < dim; ++d) tym[m * 4 + d] = y[(int64_t)m2o[m] * dim + d];
-  // row (bin) order (q-major passes stream contiguous bin runs)
-  L.tg = make_grid(lo, hi, dim, n, 8.0);
-  const int64_t ntb = (int64_t)L.tg.n[0] * L.tg.n[1] * L.tg.n[2];
-  std::vector<int> tb(n);
-  for (int64_t j = 0; j < n; ++j) tb[j] = bin_linear(L.tg, y + j * dim, dim);
-  std::vector<int32_t> &r2o = L.r2o, &pos_r = L.pos_r;
-  r2o.resize(n);
-  pos_r.resize(n);
-  std::iota(r2o.begin(), r2o.end(), 0);
-  std::stable_sort(r2o.begin(), r2o.end(), [&](int32_t a, int32_t b) { return tb[a] < tb[b]; });
-  for (int64_t r = 0; r < n; ++r) pos_r[r2o[r]] = (int32_t)r;
-  std::vector<int32_t> &tstart = L.tstart;
-  tstart.assign(ntb + 1, 0);
-  for (int64_t j = 0; j < n; ++j) tstart[tb[j] + 1]++;
-  for (int64_t b = 0; b < ntb; ++b) tstart[b + 1] += tstart[b];
+  S.tg = make_grid(lo, hi, dim, n, 8.0);
+  S.n_bins = (int64_t)S.tg.n[0] * S.tg.n[1] * S.tg.n[2];
+}
 
+// The layout of device_target_layout on the host: stable sorts by Morton key
+// and by row-major bin.
+void sdot::build_target_layout(int dim, int64_t n, const double *y, TargetLayout &L) {
+  target_shape(dim, n, y, L.shape);
+  const TargetShape &S = L.shape;
+  std::vector<uint64_t> key(n);
+  for (int64_t j = 0; j < n; ++j) key[j] = morton_key(y + j * dim, S.lo, S.hi, dim);
+  L.m2o.resize(n);
+  L.pos_m.resize(n);
+  std::iota(L.m2o.begin(), L.m2o.end(), 0);
+  std::stable_sort(L.m2o.begin(), L.m2o.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+  for (int64_t m = 0; m < n; ++m) L.pos_m[L.m2o[m]] = (int32_t)m;
+  std::vector<int> tb(n);
+  for (int64_t j = 0; j < n; ++j) tb[j] = bin_linear(S.tg, y + j * dim, dim);
+  L.r2o.resize(n);
+  L.pos_r.resize(n);
+  std::iota(L.r2o.begin(), L.r2o.end(), 0);
+  std::stable_sort(L.r2o.begin(), L.r2o.end(), [&](int32_t a, int32_t b) { return tb[a] < tb[b]; });
+  for (int64_t r = 0; r < n; ++r) L.pos_r[L.r2o[r]] = (int32_t)r;
+  L.tstart.assign(S.n_bins + 1, 0);
+  for (int64_t j = 0; j < n; ++j) L.tstart[tb[j] + 1]++;
+  for (int64_t b = 0; b < S.n_bins; ++b) L.tstart[b + 1] += L.tstart[b];
 }
 
 struct sdot_ctx {
@@ -244,6 +206,7 @@ struct sdot_ctx {
   double seed_bias = 0.0;   // test hooks (sdot_ctx_create)
   int force_K1 = 0, force_K2 = 0;
   double *noalloc = nullptr;  // this device's no-allocation scratch (kNoAllocBytes)
+  int64_t n_tbins = 0;        // bins of the target grid
   Plan plan;
   Dev dev;
   std::vector<double> h_nu;
@@ -909,34 +872,67 @@ extern "C" int sdot_set_target(sdot_ctx *c, int64_t n, const double *y, const do
   if (!all_finite(y, n * dim) || !all_finite(nu, n)) return fail(SDOT_ERR_INVALID, "non-finite target values");
   HIP_TRY(hipSetDevice(c->device));
   Plan &p = c->plan;
-  TargetLayout L;
-  build_target_layout(dim, n, y, L);
+  TargetShape S;
+  target_shape(dim, n, y, S);
   p.N = n;
-  p.n_twaves = L.n_twaves;
-  p.tg = L.tg;
+  p.n_twaves = S.n_twaves;
+  p.tg = S.tg;
   for (int d = 0; d < 3; ++d) {
-    c->tgt_lo[d] = L.lo[d];
-    c->tgt_hi[d] = L.hi[d];
+    c->tgt_lo[d] = S.lo[d];
+    c->tgt_hi[d] = S.hi[d];
   }
-  c->coord_scale = std::max(c->coord_scale, L.scale);
-  const std::vector<int32_t> &pos_r = L.pos_r, &pos_m = L.pos_m, &r2o = L.r2o, &m2o = L.m2o,
-                             &tstart = L.tstart;
-  const std::vector<double> &box = L.twbox, &tym = L.tym;
+  c->coord_scale = std::max(c->coord_scale, S.scale);
+  c->n_tbins = S.n_bins;
+  c->have_target = false;  // until every target array is in place
+  // the orders, bin offsets and wave boxes are built on the device
   Dev &dv = c->dev;
   std::vector<double> yv(y, y + n * dim), nv(nu, nu + n);
   HIP_TRY(upload(dv.ty_o, yv));
   HIP_TRY(upload(dv.tnu_o, nv));
-  HIP_TRY(upload(dv.tpos_r, pos_r));
-  HIP_TRY(upload(dv.tpos_m, pos_m));
-  HIP_TRY(upload(dv.tr2o, r2o));
-  HIP_TRY(upload(dv.tm2o, m2o));
-  HIP_TRY(upload(dv.tstart, tstart));
-  HIP_TRY(upload(dv.twbox, box));
-  HIP_TRY(upload(dv.ty_m, tym));
+  HIP_TRY(dalloc(dv.tpos_r, (size_t)n));
+  HIP_TRY(dalloc(dv.tpos_m, (size_t)n));
+  HIP_TRY(dalloc(dv.tr2o, (size_t)n));
+  HIP_TRY(dalloc(dv.tm2o, (size_t)n));
+  HIP_TRY(dalloc(dv.tstart, (size_t)S.n_bins + 1));
+  HIP_TRY(dalloc(dv.twbox, (size_t)S.n_twaves * 2 * dim));
+  HIP_TRY(dalloc(dv.ty_m, (size_t)n * 4));
+  const size_t sb = layout_scratch_bytes(S);
+  if (sb == SIZE_MAX) return fail(SDOT_ERR_HIP, "target layout: no sort storage size");
+  void *scratch = nullptr;
+  HIP_TRY(hipMalloc(&scratch, sb));
+  hipError_t e = device_target_layout(dim, S, dv.ty_o, dv.tpos_r, dv.tr2o, dv.tpos_m, dv.tm2o,
+                                      dv.tstart, dv.twbox, dv.ty_m, scratch, sb, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(scratch);
+  if (e != hipSuccess) return fail(SDOT_ERR_HIP, std::string("target layout: ") + hipGetErrorString(e));
   c->h_nu = nv;
   c->cap_G = c->cap_trec = c->cap_P = c->cap_N = 0;
   c->hist_valid = false;
   c->have_target = true;
+  return SDOT_OK;
+}
+
+extern "C" int sdot_target_layout(sdot_ctx *c, int32_t *pos_r, int32_t *pos_m, int32_t *tstart,
+                                  int64_t *n_bins, double *grid) {
+  if (ctx_check(c)) return SDOT_ERR_INVALID;
+  if (!c->have_target) return fail(SDOT_ERR_STATE, "target measure not set");
+  HIP_TRY(hipSetDevice(c->device));
+  if (n_bins) *n_bins = c->n_tbins;
+  if (grid) {
+    const Grid &g = c->plan.tg;
+    for (int d = 0; d < 3; ++d) {
+      grid[d] = g.o[d];
+      grid[4 + d] = (double)g.n[d];
+    }
+    grid[3] = g.h;
+  }
+  const size_t N = (size_t)c->plan.N;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (pos_r) HIP_TRY(hipMemcpy(pos_r, c->dev.tpos_r, N * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (pos_m) HIP_TRY(hipMemcpy(pos_m, c->dev.tpos_m, N * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (tstart)
+    HIP_TRY(hipMemcpy(tstart, c->dev.tstart, (size_t)(c->n_tbins + 1) * sizeof(int32_t),
+                      hipMemcpyDeviceToHost));
   return SDOT_OK;
 }
 
@@ -1135,7 +1131,7 @@ extern "C" int sdot_softmax_refinement(sdot_ctx *c, int64_t n_fine, const double
   }
 
   RefArgs a{};
-  a.tg = L.tg;
+  a.tg = L.shape.tg;
   a.cg = p.cg;
   a.nQ = p.nQ;
   a.n_cells = p.n_cells;
@@ -1146,10 +1142,10 @@ extern "C" int sdot_softmax_refinement(sdot_ctx *c, int64_t n_fine, const double
   a.lse = use_log_sum_exp_trick ? 1 : 0;
   a.smax = smax_for(distance_threshold);
   a.s2max = s2max_for(distance_threshold);
-  const double scale = std::max(c->coord_scale, L.scale);
+  const double scale = std::max(c->coord_scale, L.shape.scale);
   a.Rq = std::isnan(distance_threshold)
              ? 0.0
-             : distance_threshold * (1.0 + 1e-9) + 1e-9 * (L.tg.h + p.cg.h + scale);
+             : distance_threshold * (1.0 + 1e-9) + 1e-9 * (L.shape.tg.h + p.cg.h + scale);
   a.lam = lambda;
   a.lam_inv = 1.0 / lambda;
   const int64_t nwq = (p.nQ + kWave - 1) / kWave, nwe = (E + kWave - 1) / kWave;

@@ -44,6 +44,32 @@ SDOT_HD inline int bin_of(double v, double o, double inv_h, int n) {
   return (int)f;
 }
 
+// Row-major index of the bin holding p (the host and the device bin the same
+// way).
+SDOT_HD inline int bin_linear(const Grid &g, const double *p, int dim) {
+  int b[3] = {0, 0, 0};
+  for (int d = 0; d < dim; ++d) b[d] = bin_of(p[d], g.o[d], g.inv_h, g.n[d]);
+  return (b[2] * g.n[1] + b[1]) * g.n[0] + b[0];
+}
+
+// Morton key of p in the box [lo, hi]: 21 bits per axis in 3D, 31 in 2D.
+SDOT_HD inline uint64_t morton_key(const double *p, const double *lo, const double *hi, int dim) {
+  const int bits = dim == 3 ? 21 : 31;
+  const double scale = (double)((1u << (bits - 1)) - 1) * 2.0;
+  uint64_t key = 0;
+  uint64_t q[3] = {0, 0, 0};
+  for (int d = 0; d < dim; ++d) {
+    const double ext = hi[d] - lo[d];
+    double u = ext > 0 ? (p[d] - lo[d]) / ext : 0.0;
+    u = u > 0.0 ? u : 0.0;
+    u = u < 1.0 ? u : 1.0;
+    q[d] = (uint64_t)(u * scale);
+  }
+  for (int b = bits - 1; b >= 0; --b)
+    for (int d = dim - 1; d >= 0; --d) key = (key << 1) | ((q[d] >> b) & 1u);
+  return key;
+}
+
 // Device buffers and host metadata of one solver context.
 struct Dev {
   // ---- source (static after sdot_set_source) ----
@@ -213,15 +239,38 @@ struct Profile {
   sdot_profile acc{};
 };
 
-// Host-side layout of a target set (sdot_api.cpp).
-struct TargetLayout {
+// Shape of a target set's layout, from one host pass over the points
+// (sdot_api.cpp): bounding box, coordinate scale, bin grid (about 8 points per
+// bin) and the number of 64-target waves of the gradient pass.
+struct TargetShape {
   Grid tg{};
-  int64_t n = 0, n_twaves = 0;
+  int64_t n = 0, n_twaves = 0, n_bins = 0;
   double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}, scale = 0.0;
+};
+void target_shape(int dim, int64_t n, const double *y, TargetShape &S);
+
+// The same layout built on the host (sdot_api.cpp), for the softmax transfer's
+// coarse level, whose records are written on the host.
+struct TargetLayout {
+  TargetShape shape;
   std::vector<int32_t> m2o, pos_m, r2o, pos_r, tstart;
-  std::vector<double> twbox, tym;
 };
 void build_target_layout(int dim, int64_t n, const double *y, TargetLayout &L);
+
+// The target layout built on the device (sdot_layout.hip) from the points in
+// caller order already in HBM: stable radix sorts of the row-major bin index
+// (q-major record order) and of the Morton key (gradient waves), the bin
+// offsets and the wave boxes.  Outputs (N = S.n, dim-strided boxes):
+//   pos_r [N] caller -> row position,   r2o [N] row position -> caller (or null),
+//   pos_m [N] caller -> Morton position, m2o [N] Morton position -> caller,
+//   tstart [n_bins + 1],  twbox [n_twaves * 2 * dim] (min then max per wave),
+//   ty_m [N * 4] points in Morton order, padded to 4 (or null).
+// scratch: layout_scratch_bytes(S) bytes of device memory.
+size_t layout_scratch_bytes(const TargetShape &S);
+hipError_t device_target_layout(int dim, const TargetShape &S, const double *d_y, int32_t *pos_r,
+                                int32_t *r2o, int32_t *pos_m, int32_t *m2o, int32_t *tstart,
+                                double *twbox, double *ty_m, void *scratch, size_t scratch_bytes,
+                                hipStream_t s);
 
 // Largest double s with sqrt(s) <= R (candidate test s <= smax).
 double smax_for(double R);

@@ -117,6 +117,7 @@ struct Carve {
 
 struct SolveShape {
   int64_t N = 0, ntb = 0, n_twaves = 0, nPpart = 0;
+  size_t lay_bytes = 0;  // scratch of the device-built target layout
   int K1 = 1, K2 = 1;
   bool fp32 = false;
 };
@@ -161,7 +162,9 @@ SolveOffsets carve(const SourceView &src, const SolveShape &s) {
   for (int k = 0; k < kHist; ++k) o.S[k] = c.take<double>(s.N);
   for (int k = 0; k < kHist; ++k) o.Y[k] = c.take<double>(s.N);
   o.hist = c.take<SlotHist>(1);
-  o.total = c.off;
+  // the target layout is built at admission in the space of the evaluation
+  // buffers (trec_r onwards), which the first evaluation writes afterwards
+  o.total = std::max(c.off, align_up(o.trec_r + s.lay_bytes));
   return o;
 }
 
@@ -175,7 +178,7 @@ struct Request {
   int precond = SDOT_PRECOND_NONE;
   uint32_t max_it = 0;
   std::vector<double> y, nu, psi0;
-  TargetLayout L;
+  TargetShape L;
   SolveShape shape;
   int dense = 0;
   double smax = 0.0, Rq = 0.0, s_in = 0.0;
@@ -248,8 +251,9 @@ int fill_shape(sdot_server *s, Request &r) {
   const SourceView &src = s->src;
   SolveShape &sh = r.shape;
   sh.N = r.N;
-  sh.ntb = (int64_t)r.L.tg.n[0] * r.L.tg.n[1] * r.L.tg.n[2];
+  sh.ntb = r.L.n_bins;
   sh.n_twaves = r.L.n_twaves;
+  sh.lay_bytes = layout_scratch_bytes(r.L);
   sh.nPpart = (r.N + 255) / 256;
   sh.fp32 = s->mode == kF32Lse;
   // candidate rule and enumeration radius, as for a single solve
@@ -442,18 +446,19 @@ int admit(sdot_server *s, std::unique_ptr<Request> r, const SolveOffsets &o, siz
   init_driver(s, *v);
   v->t_admit = s->now();
   Request &q = *v->req;
-  const TargetLayout &L = q.L;
   const int dim = s->src.dim;
   auto up = [&](size_t rel, const void *h, size_t bytes) {
     return hipMemcpyAsync(s->arena.at(v->off + rel), h, bytes, hipMemcpyHostToDevice, s->stream);
   };
   SV_TRY(up(v->o.y, q.y.data(), sizeof(double) * q.N * dim));
   SV_TRY(up(v->o.nu, q.nu.data(), sizeof(double) * q.N));
-  SV_TRY(up(v->o.pos_r, L.pos_r.data(), sizeof(int32_t) * q.N));
-  SV_TRY(up(v->o.pos_m, L.pos_m.data(), sizeof(int32_t) * q.N));
-  SV_TRY(up(v->o.tm2o, L.m2o.data(), sizeof(int32_t) * q.N));
-  SV_TRY(up(v->o.tstart, L.tstart.data(), sizeof(int32_t) * L.tstart.size()));
-  SV_TRY(up(v->o.twbox, L.twbox.data(), sizeof(double) * L.twbox.size()));
+  {
+    auto at = [&](size_t rel) { return s->arena.at(v->off + rel); };
+    SV_TRY(device_target_layout(dim, q.L, (const double *)at(v->o.y), (int32_t *)at(v->o.pos_r),
+                                nullptr, (int32_t *)at(v->o.pos_m), (int32_t *)at(v->o.tm2o),
+                                (int32_t *)at(v->o.tstart), (double *)at(v->o.twbox), nullptr,
+                                at(v->o.trec_r), v->o.total - v->o.trec_r, s->stream));
+  }
   SV_TRY(up(v->o.x, q.psi0.data(), sizeof(double) * q.N));
   SV_TRY(up(v->o.hist, &kEmptyHist, sizeof(SlotHist)));
   build_desc(s, *v);
@@ -737,7 +742,7 @@ extern "C" int sdot_server_submit(sdot_server *s, const sdot_request *q, int64_t
   } else {
     r->psi0.assign(q->n, 0.0);
   }
-  build_target_layout(dim, r->N, r->y.data(), r->L);
+  target_shape(dim, r->N, r->y.data(), r->L);
   int rc = fill_shape(s, *r);
   if (rc) return rc;
   if (r->bytes > s->arena.size())
@@ -850,7 +855,7 @@ extern "C" int sdot_server_request_bytes(sdot_server *s, int64_t n, double radiu
   for (int64_t j = 0; j < n; ++j)
     for (int d = 0; d < s->src.dim; ++d)
       y[j * s->src.dim + d] = s->src.lo[d] + (s->src.hi[d] - s->src.lo[d]) * ((j * (d + 3) + d) % 997) / 997.0;
-  build_target_layout(s->src.dim, n, y.data(), r.L);
+  target_shape(s->src.dim, n, y.data(), r.L);
   fill_shape(s, r);
   *bytes = (int64_t)r.bytes;
   return SDOT_OK;
@@ -896,7 +901,7 @@ extern "C" int sdot_server_autotune(sdot_server *s, int32_t *pass_targets, int32
       Request r;
       r.N = sizes[i];
       r.R = INFINITY;
-      build_target_layout(dim, r.N, ys[i].data(), r.L);
+      target_shape(dim, r.N, ys[i].data(), r.L);
       rc = fill_shape(s, r);
       need += r.bytes;
     }
