@@ -874,8 +874,11 @@ def main():
     peak = pass_peak_pairs(3)
     eval_frac = (pairs / (r["ms"] * 1e-3)) / (peak / 2.0)
     rates = measured_rates()
+    measured_frac = None
     if rates:
         log(f"measured rates: {json.dumps(rates)}")
+        bound = 1.0 / max(1.0 / rates["exp_f32_per_s"], 6.0 / rates["fma_f64_per_s"])
+        measured_frac = achieved / bound
     serving = None
     if args.serve_requests > 0:
         log("serving leg ...")
@@ -919,9 +922,12 @@ def main():
                      "peak": peak / 1e9, "unit": "Gpair/s", "frac": achieved / peak,
                      "traffic": measured_traffic(dom),
                      "eval_frac": eval_frac,
+                     "frac_at_measured_rates": measured_frac,
                      "peak_basis": "per pair sweep 1/max(1/R_exp,(d+3)/R_fma) with R_fma = 3.93e13 FMA/s "
                                    "(published 78.6 TFLOP/s FP64/FP32 vector) and R_exp >= R_fma/4; "
-                                   "an evaluation is two sweeps (eval_frac)"},
+                                   "an evaluation is two sweeps (eval_frac); frac_at_measured_rates: "
+                                   "the same bound with this GPU's measured v_fma_f64 and v_exp_f32 "
+                                   "rates (measured_rates)"},
         "fp32_cold": cold,
         "fp32_vs_fp64": solve_table,
         "measured_rates": rates,
