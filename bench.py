@@ -239,8 +239,11 @@ def pmc_summary(fetch_csv, write_csv, copy_bytes):
         if k not in write:
             continue
         t = fetch[k] * kf + write[k] * kw
-        lines.append(f"| `{k.split('(')[0]}` | {nf[k]} | {fetch[k] * kf:.4g} | {write[k] * kw:.4g} | {t:.4g} |")
-        rows.append((k.split("(")[0].replace("void sdot::", "").replace("sdot::", ""), t))
+        short = k.replace("(anonymous namespace)::", "").split("(")[0]
+        if "rocprim" in short:
+            short = "rocprim " + ("radix" if "radix_sort" in short else "merge") + " sort kernel"
+        lines.append(f"| `{short}` | {nf[k]} | {fetch[k] * kf:.4g} | {write[k] * kw:.4g} | {t:.4g} |")
+        rows.append((short.replace("void sdot::", "").replace("sdot::", ""), t))
     lines.append("")
     lines += [f"traffic-bytes {name} = {t:.6g}" for name, t in rows]
     print("\n".join(lines))
