@@ -776,7 +776,12 @@ extern "C" int sdot_server_step(sdot_server *s, int32_t *n_active, int32_t *n_qu
       break;
     }
     int rc = admit(s, std::move(r), o, off);
-    if (rc) return rc;
+    if (rc) {
+      // a HIP failure while copying the request in or building its layout:
+      // the request is dropped and its block returned
+      s->arena.release_block(off, o.total);
+      return rc;
+    }
   }
   const int n = (int)s->active.size();
   if (n > 0) {
