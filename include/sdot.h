@@ -89,6 +89,17 @@ typedef struct {
   int32_t precision;              /* sdot_precision, FP64 */
   int32_t line_search;            /* sdot_line_search, REFERENCE */
   int32_t preconditioner;         /* sdot_preconditioner, NONE */
+  /* Value bound of the pair sums (this library's own; no counterpart in the
+   * reference).  Bins of candidate pairs whose terms all lie below
+   * 2^-prune_bits of their q-point's largest weighted term are skipped: a
+   * q-point's sum loses less than N 2^-prune_bits of itself and a target's
+   * mass less than 2^-prune_bits.  < 0: the precision's default: FP64 off
+   * (every candidate pair, as the reference sums them); FP32_LSE 50 bits,
+   * below the FP32 exponentials' own rounding, where it pays (at least 2^32
+   * pairs, and sqrt(100 ln2 epsilon) at most a fifth of the diagonal of the
+   * source and target boxes), else off.  0: off.  Plain (non-LSE) mode and
+   * targets with a zero weight turn it off. */
+  double prune_bits;              /* -1 */
 } sdot_params;
 
 typedef struct {

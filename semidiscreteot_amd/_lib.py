@@ -36,7 +36,8 @@ class Params(C.Structure):
                 ("number_of_threads", C.c_uint32), ("use_epsilon_scaling", C.c_int32),
                 ("epsilon_scaling_factor", C.c_double), ("epsilon_scaling_steps", C.c_uint32),
                 ("use_log_sum_exp_trick", C.c_int32), ("precision", C.c_int32),
-                ("line_search", C.c_int32), ("preconditioner", C.c_int32)]
+                ("line_search", C.c_int32), ("preconditioner", C.c_int32),
+                ("prune_bits", C.c_double)]
 
 
 class MeshParams(C.Structure):

@@ -90,6 +90,9 @@ Grid make_grid(const double *lo, const double *hi, int dim, int64_t n, double pe
 }
 
 constexpr int kMaxSeg = 128;
+// Default value bound of FP32_LSE evaluations: terms below 2^-50 of their
+// q-point's largest, far below the 2^-24 rounding of the FP32 exponentials.
+constexpr double kPruneBitsF32 = 50.0;
 constexpr int kProfSets = 128;
 
 }  // namespace
@@ -215,6 +218,7 @@ struct sdot_ctx {
   double tgt_lo[3] = {0, 0, 0}, tgt_hi[3] = {0, 0, 0};
   double coord_scale = 1.0;
   size_t cap_trec = 0, cap_qrec = 0, cap_M = 0, cap_G = 0, cap_F = 0, cap_P = 0, cap_N = 0;
+  size_t cap_vb_t = 0, cap_vb_c = 0;  // value-bound arrays (target side, cell side)
   // profiling: a pool of event sets resolved at synchronisation points
   bool prof_on = false;
   std::vector<std::array<hipEvent_t, 6>> ev_pool;
@@ -279,8 +283,28 @@ int plan_eval(sdot_ctx *c, double R) {
   if (std::isnan(R)) p.Rq = 0.0;
   // truncated sweeps run the small grouping (same R, same grouping: the FP32
   // history stays valid)
+  // value bound: off in plain mode and with a zero target weight; below 0 the
+  // precision's default, which bounds FP32_LSE evaluations where it pays: the
+  // radius sqrt(2 bits ln2 eps) beyond which a target's terms fall below
+  // 2^-bits (at equal potentials) is at most a fifth of the domain's diagonal,
+  // and there are at least 2^32 pairs.  Measured on the 32^3 source (one
+  // evaluation, FP32): 50,000 targets, eps 1e-3: 6.7 -> 5.0 ms truncated,
+  // 7.5 -> 5.7 ms dense; 20,000 targets, eps 3e-3 (radius 0.26 of the
+  // diagonal): 3.2 -> 3.4 ms; 4,096 targets, eps 1e-3: 0.87 -> 0.94 ms.
+  if (c->params.prune_bits < 0) {
+    p.prune_bits = 0.0;
+    const double r0 = std::sqrt(2.0 * kPruneBitsF32 * std::log(2.0) * p.eps);
+    if (p.mode == kF32Lse && r0 <= 0.2 * diag && (double)p.N * (double)p.nQ >= 4294967296.0)
+      p.prune_bits = kPruneBitsF32;
+  } else {
+    p.prune_bits = c->params.prune_bits;
+  }
+  if (p.mode == kF64Plain || !(p.nu_min > 0.0) || c->force_mode >= 0) p.prune_bits = 0.0;
   {
-    const Grouping &gr = (!p.dense && p.grp_small.G != p.grp_big.G) ? p.grp_small : p.grp_big;
+    // bounded dense sweeps take the small grouping too: its blocks span half
+    // the cells, which tightens the wave's value bound
+    const bool small = !p.dense || p.prune_bits > 0;
+    const Grouping &gr = (small && p.grp_small.G != p.grp_big.G) ? p.grp_small : p.grp_big;
     p.small = &gr == &p.grp_small ? 1 : 0;
     p.G = gr.G;
     p.gpc = gr.gpc;
@@ -373,6 +397,22 @@ int prepare(sdot_ctx *c, double R) {
   }
   if (!d.out) HIP_TRY(dalloc(d.out, 4));
   if (!d.ucount) HIP_TRY(dalloc(d.ucount, 1));
+  if (p.prune_bits > 0) {
+    const size_t nt = (size_t)p.N, ntb = (size_t)p.n_tbins, ntr = (size_t)grid_rows(p.tg);
+    const size_t nc = (size_t)p.n_cells, ncb = (size_t)p.n_cbins, ncr = (size_t)grid_rows(p.cg);
+    if (c->cap_vb_t < nt + ntb + ntr) {
+      HIP_TRY(dalloc(d.psit_r, nt));
+      HIP_TRY(dalloc(d.pbin, ntb));
+      HIP_TRY(dalloc(d.prow, ntr));
+      c->cap_vb_t = nt + ntb + ntr;
+    }
+    if (c->cap_vb_c < nc + ncb + ncr) {
+      HIP_TRY(dalloc(d.vcell, nc));
+      HIP_TRY(dalloc(d.vbin, ncb));
+      HIP_TRY(dalloc(d.vrow, ncr));
+      c->cap_vb_c = nc + ncb + ncr;
+    }
+  }
   return SDOT_OK;
 }
 
@@ -430,6 +470,14 @@ size_t carve_eval(const Plan &p, const Dev &ctx, double *base, Dev &d) {
   take(d.dpsi_cn, (N + 63) / 64);
   take(d.out, 4);
   take(d.ucount, 1);
+  if (p.prune_bits > 0) {
+    take(d.psit_r, N);
+    take(d.pbin, (size_t)p.n_tbins);
+    take(d.prow, (size_t)grid_rows(p.tg));
+    take(d.vcell, (size_t)p.n_cells);
+    take(d.vbin, (size_t)p.n_cbins);
+    take(d.vrow, (size_t)grid_rows(p.cg));
+  }
   return off * sizeof(double);
 }
 
@@ -695,7 +743,8 @@ extern "C" int sdot_ctx_destroy(sdot_ctx *c) {
                   d.cell_m2o, d.ty_o, d.tnu_o, d.tpos_r, d.tpos_m, d.tr2o, d.tm2o, d.tstart,
                   d.twbox, d.ty_m, d.trec_r, d.trec_m, d.qrec_r, d.Mpart, d.Spart, d.Gpart,
                   d.Fpart, d.Cpart, d.PNpart, d.psi, d.grad, d.out, d.scratch, d.i64scratch, d.Tpart,
-                  d.Upart, d.psi_prev, d.dpsi_r, d.dpsi_cx, d.dpsi_cn, d.ucount, d.Mglob};
+                  d.Upart, d.psi_prev, d.dpsi_r, d.dpsi_cx, d.dpsi_cn, d.ucount, d.Mglob,
+                  d.psit_r, d.pbin, d.prow, d.vcell, d.vbin, d.vrow};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto &set : c->ev_pool)
@@ -714,6 +763,8 @@ extern "C" int sdot_set_params(sdot_ctx *c, const sdot_params *p) {
     return fail(SDOT_ERR_INVALID, "unknown line_search");
   if (p->preconditioner != SDOT_PRECOND_NONE && p->preconditioner != SDOT_PRECOND_DIAGONAL)
     return fail(SDOT_ERR_INVALID, "unknown preconditioner");
+  if (std::isnan(p->prune_bits) || std::isinf(p->prune_bits))
+    return fail(SDOT_ERR_INVALID, "prune_bits must be finite");
   c->params = *p;
   c->hist_valid = false;
   return SDOT_OK;
@@ -828,9 +879,23 @@ extern "C" int sdot_set_source(sdot_ctx *c, int dim, int64_t n_cells, int nq, co
     return box;
   };
   const std::vector<double> box = boxes(p.grp_big), box_s = boxes(p.grp_small);
+  // largest distance of a cell's quadrature points from its center (value
+  // bound of the target-major pass), with a margin for its rounding
+  double rq2 = 0.0;
+  for (int64_t i = 0; i < n_cells; ++i)
+    for (int k = 0; k < nq; ++k) {
+      double s2 = 0.0;
+      for (int d = 0; d < dim; ++d) {
+        const double t = x[(i * nq + k) * dim + d] - centers[i * dim + d];
+        s2 += t * t;
+      }
+      rq2 = std::max(rq2, s2);
+    }
+  p.rq = std::sqrt(rq2) * (1.0 + 1e-9) + 1e-12 * scale;
   // row (bin) order of the cells (target-major pass)
   p.cg = make_grid(clo, chi, dim, n_cells, 8.0);
   const int64_t ncb = (int64_t)p.cg.n[0] * p.cg.n[1] * p.cg.n[2];
+  p.n_cbins = ncb;
   std::vector<int> cbin(n_cells);
   for (int64_t m = 0; m < n_cells; ++m) cbin[m] = bin_linear(p.cg, &ccm[m * dim], dim);
   std::vector<int64_t> r2m(n_cells);
@@ -857,7 +922,7 @@ extern "C" int sdot_set_source(sdot_ctx *c, int dim, int64_t n_cells, int nq, co
   HIP_TRY(upload(dv.cell_m2o, m2o));
   c->h_cell_r2o.resize(n_cells);
   for (int64_t r = 0; r < n_cells; ++r) c->h_cell_r2o[r] = m2o[r2m[r]];
-  c->cap_M = c->cap_qrec = c->cap_F = 0;
+  c->cap_M = c->cap_qrec = c->cap_F = c->cap_vb_c = 0;
   c->hist_valid = false;
   c->have_source = true;
   return SDOT_OK;
@@ -883,6 +948,9 @@ extern "C" int sdot_set_target(sdot_ctx *c, int64_t n, const double *y, const do
   }
   c->coord_scale = std::max(c->coord_scale, S.scale);
   c->n_tbins = S.n_bins;
+  p.n_tbins = S.n_bins;
+  p.nu_min = *std::min_element(nu, nu + n);
+  if (!(p.nu_min > 0.0)) p.nu_min = 0.0;
   c->have_target = false;  // until every target array is in place
   // the orders, bin offsets and wave boxes are built on the device
   Dev &dv = c->dev;
@@ -906,7 +974,7 @@ extern "C" int sdot_set_target(sdot_ctx *c, int64_t n, const double *y, const do
   (void)hipFree(scratch);
   if (e != hipSuccess) return fail(SDOT_ERR_HIP, std::string("target layout: ") + hipGetErrorString(e));
   c->h_nu = nv;
-  c->cap_G = c->cap_trec = c->cap_P = c->cap_N = 0;
+  c->cap_G = c->cap_trec = c->cap_P = c->cap_N = c->cap_vb_t = 0;
   c->hist_valid = false;
   c->have_target = true;
   return SDOT_OK;

@@ -175,6 +175,131 @@ __device__ __forceinline__ void for_rows(const Grid &g, const int32_t *__restric
   }
 }
 
+// Value bound of a traversal.  Every pair (q, j) a sweep sums satisfies
+//   psi~_j - |x_q - y_j|^2 / 2  >=  V_q
+// or its term is below 2^-theta of the terms the sum keeps (psi~_j = psi_j +
+// eps ln nu_j; V_q from the q-point's largest exponent, see pass1_body and
+// grad_body).  For a wave with q-point box B (q-major) or target box B
+// (target-major), a bin b of the traversed grid can hold such a pair only if
+//   A_b - d(B, bin b)^2 / 2  >=  B_b
+// with A the largest psi~ and B the smallest V on either side: per bin and per
+// row of the grid from arrays, or one value for the whole wave.  Bins that fail
+// are skipped whole.
+struct VBound {
+  const double *abin = nullptr, *arow = nullptr;  // per bin / per row A (null: auni)
+  const double *bbin = nullptr, *brow = nullptr;  // per bin / per row B (null: buni)
+  double auni = 0.0, buni = 0.0;
+  double lo[3] = {0.0, 0.0, 0.0}, hi[3] = {0.0, 0.0, 0.0};  // the wave's box, raw coordinates
+};
+
+// Gap between the intervals [lo, hi] and [a, b] (0 when they overlap).
+__device__ __forceinline__ double gap(double lo, double hi, double a, double b) {
+  return fmax(0.0, fmax(a - hi, lo - b));
+}
+
+// for_rows restricted by a value bound: the same rows and segments (row index
+// modulo nseg), the same candidate enumeration radius Rq and inner runs, and
+// only the bins vb admits; runs of consecutive admitted bins go to body.
+// dense: every target is a candidate (all bins, no candidate test).
+template <int DIM, class Body>
+__device__ __forceinline__ void for_rows_vb(const Grid &g, const int32_t *__restrict__ start,
+                                            const double *lo, const double *hi, double Rq,
+                                            double s_in, int seg, int nseg, bool dense,
+                                            const VBound &vb, Body &&body) {
+  cdouble *abin = cptr(vb.abin), *arow = cptr(vb.arow);
+  cdouble *bbin = cptr(vb.bbin), *brow = cptr(vb.brow);
+  int b0[3] = {0, 0, 0}, b1[3] = {0, 0, 0};
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    b0[d] = dense ? 0 : bin_of(lo[d] - Rq, g.o[d], g.inv_h, g.n[d]);
+    b1[d] = dense ? g.n[d] - 1 : bin_of(hi[d] + Rq, g.o[d], g.inv_h, g.n[d]);
+  }
+  const double R2 = Rq * Rq;
+  const double pad = 1e-9 * g.h;
+  for (int bz = b0[2]; bz <= b1[2]; ++bz) {
+    double dz = 0.0, dzf = 0.0, vdz = 0.0;
+    if (DIM == 3) {
+      const double zl = g.o[2] + bz * g.h, zh = zl + g.h;
+      dz = fmax(0.0, fmax(zl - hi[2], lo[2] - zh));
+      dzf = fmax(zh - lo[2], hi[2] - zl);
+      vdz = gap(vb.lo[2], vb.hi[2], zl, zh);
+    }
+    const int want = ((seg - (bz * g.n[1]) % nseg) % nseg + nseg) % nseg;
+    const int by0 = b0[1] + ((want - b0[1] % nseg) % nseg + nseg) % nseg;
+    for (int by = by0; by <= b1[1]; by += nseg) {
+      const double yl = g.o[1] + by * g.h, yh = yl + g.h;
+      const int row = bz * g.n[1] + by;
+      // value bound of the row: the x-interval of bins it can admit
+      const double vd2 = gap(vb.lo[1], vb.hi[1], yl, yh) * gap(vb.lo[1], vb.hi[1], yl, yh) +
+                         vdz * vdz;
+      const double ar = arow ? arow[row] : vb.auni;
+      const double br = brow ? brow[row] : vb.buni;
+      const double slack = ar - br - 0.5 * vd2;
+      if (!(slack >= 0.0)) continue;
+      const double wv = sqrt(2.0 * slack);
+      int x0 = bin_of(vb.lo[0] - wv, g.o[0], g.inv_h, g.n[0]);
+      int x1 = bin_of(vb.hi[0] + wv, g.o[0], g.inv_h, g.n[0]);
+      int i0 = x1 + 1, i1 = x1;
+      if (dense) {
+        i0 = x0;  // every bin is inside
+      } else {
+        const double dy = fmax(0.0, fmax(yl - hi[1], lo[1] - yh));
+        const double r2 = R2 - dy * dy - dz * dz;
+        if (r2 < 0.0) continue;
+        const double wx = sqrt(r2);
+        x0 = max(x0, bin_of(lo[0] - wx, g.o[0], g.inv_h, g.n[0]));
+        x1 = min(x1, bin_of(hi[0] + wx, g.o[0], g.inv_h, g.n[0]));
+        i0 = x1 + 1;
+        i1 = x1;
+        if (s_in > 0.0) {
+          const double dyf = fmax(yh - lo[1], hi[1] - yl);
+          const double r2x = s_in - dyf * dyf - dzf * dzf;
+          if (r2x > 0.0) {
+            const double w = sqrt(r2x);
+            const double xa = hi[0] - w + pad, xb = lo[0] + w - pad;
+            if (xa < xb) {
+              i0 = max(x0, (int)ceil((xa - g.o[0]) * g.inv_h));
+              i1 = min(x1, (int)floor((xb - g.o[0]) * g.inv_h) - 1);
+            }
+          }
+        }
+      }
+      if (x0 > x1) continue;
+      const int base = row * g.n[0];
+      // runs of consecutive admitted bins, split into tested and inside parts
+      auto emit = [&](int r0, int r1) {
+        const int s = __builtin_amdgcn_readfirstlane(start[base + r0]);
+        const int e = __builtin_amdgcn_readfirstlane(start[base + r1 + 1]);
+        if (s >= e) return;
+        const int j0 = max(r0, i0), j1 = min(r1, i1);
+        if (j0 <= j1) {
+          const int m0 = __builtin_amdgcn_readfirstlane(start[base + j0]);
+          const int m1 = __builtin_amdgcn_readfirstlane(start[base + j1 + 1]);
+          if (s < m0) body(s, m0, Tested{});
+          if (m0 < m1) body(m0, m1, Inside{});
+          if (m1 < e) body(m1, e, Tested{});
+        } else {
+          body(s, e, Tested{});
+        }
+      };
+      int rs = -1;
+      for (int b = x0; b <= x1; ++b) {
+        const double xl = g.o[0] + b * g.h;
+        const double dx = gap(vb.lo[0], vb.hi[0], xl, xl + g.h);
+        const double ab = abin ? abin[base + b] : vb.auni;
+        const double bb = bbin ? bbin[base + b] : vb.buni;
+        const bool keep = ab - bb - 0.5 * (dx * dx + vd2) >= 0.0;
+        if (keep && rs < 0) rs = b;
+        if (!keep && rs >= 0) {
+          emit(rs, b - 1);
+          rs = -1;
+        }
+      }
+      if (rs >= 0) emit(rs, x1);
+    }
+  }
+}
+
 // Fold f over the K segment partials p[k * stride], k = 0 .. K-1, in index
 // order.  The loads of 16 partials are issued together: a plain loop waits
 // out one memory round trip per partial, which dominates the finishing
@@ -309,6 +434,8 @@ __device__ __forceinline__ void prep_body(const PrepArgs &a, int64_t blk, double
       r2[k] = rec[k];
     }
     pn = psi * nu;
+    // value bound: psi + eps ln nu per target, row order
+    if (a.psit_r) a.psit_r[a.pos_r[j]] = psi + a.eps * log(nu);
     if (a.psi_prev) {
       // change of the exponent coefficient since the previous evaluation, in
       // the row order the q-major sweep streams
@@ -338,11 +465,59 @@ __device__ __forceinline__ void chunkext_body(const double *dpsi_r, int64_t N, d
   }
 }
 
+// Reductions of the value bound (see VBound).  Over the records / cells of bin
+// b (offsets start[b] .. start[b+1]): the largest psi~ of a target bin (empty:
+// -inf) or the smallest V of a cell bin (empty: +inf).
+template <bool MAX>
+__device__ __forceinline__ void bin_reduce_body(const double *v, const int32_t *start, int64_t nb,
+                                                double *out, int64_t b) {
+  if (b >= nb) return;
+  double r = MAX ? -INFINITY : INFINITY;
+  for (int32_t i = start[b]; i < start[b + 1]; ++i) r = MAX ? fmax(r, v[i]) : fmin(r, v[i]);
+  out[b] = r;
+}
+
+// ... and over the nx bins of each row of the grid.
+template <bool MAX>
+__device__ __forceinline__ void row_reduce_body(const double *bins, int64_t nrows, int nx,
+                                                double *out, int64_t row) {
+  if (row >= nrows) return;
+  double r = MAX ? -INFINITY : INFINITY;
+  for (int i = 0; i < nx; ++i) {
+    const double v = bins[row * nx + i];
+    r = MAX ? fmax(r, v) : fmin(r, v);
+  }
+  out[row] = r;
+}
+
+// Smallest V_q = (M_q + log S_q - theta)/c - |x'_q|^2/2 over the quadrature
+// points of each cell (row cell order), from the q records; q-points that
+// received nothing need nothing (+inf).  log2 S in the FP32 sweep's units.
+template <int DIM>
+__device__ __forceinline__ void cellv_body(const double *qrec_r, int64_t n_cells, int nq,
+                                           double theta, double inv_coef, bool log2_units,
+                                           double *vcell, int64_t c) {
+  if (c >= n_cells) return;
+  double v = INFINITY;
+  for (int k = 0; k < nq; ++k) {
+    const double *r = qrec_r + (c * nq + k) * kRec;
+    const double M = r[DIM], S = r[DIM + 2];
+    if (!(S > 0.0) || !(M > -INFINITY) || !(M < INFINITY)) continue;
+    double x2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) x2 = fma(r[d], r[d], x2);
+    const double ls = log2_units ? log2(S) : log(S);
+    v = fmin(v, (M + ls - theta) * inv_coef - 0.5 * x2);
+  }
+  // rounding of the bound: a relative margin far below 2^-theta
+  vcell[c] = v - 1e-12 * (1.0 + fabs(v));
+}
+
 // ---------------------------------------------------------------------------
 // k_pass1: q-major sweeps
 // ---------------------------------------------------------------------------
 
-template <int DIM, int G, int MODE, int PASS>
+template <int DIM, int G, int MODE, int PASS, bool VB = false>
 __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg) {
   const int lane = threadIdx.x;
   const int64_t grp = blk * kWave + lane;
@@ -411,17 +586,77 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
     }
   };
   auto sweep = [&](auto &&f) {
-    if (a.dense) {
+    if (VB && a.dense) {
+      // the segments of the bounded sums are sets of bin rows: the item's
+      // maxima and seed extrema must cover the same targets
+      VBound open;
+      open.auni = INFINITY;
+      open.buni = -INFINITY;
+      for_rows_vb<DIM>(a.tg, a.tstart, lo, hi, a.Rq, a.s_in, seg, a.K1, true, open, f);
+    } else if (a.dense) {
       const int64_t j0 = a.N * seg / a.K1, j1 = a.N * (seg + 1) / a.K1;
       f((int)j0, (int)j1, Inside{});
     } else {
       for_rows<DIM>(a.tg, a.tstart, lo, hi, a.Rq, a.s_in, seg, a.K1, f);
     }
   };
+  // Value bound of the sums (VB): with L_q a lower bound of the q-point's
+  // largest exponent (in the sweep's units, c = a.coef per unit of psi), a pair
+  // whose weighted term nu_j e^{t_qj} lies below 2^-theta nu_min e^{L_q} is
+  // skipped; this is psi~_j - |x_q - y_j|^2/2 < V_q with
+  //   V_q = (L_q - theta) / c + eps ln nu_min - |x'_q|^2 / 2.
+  // Every skipped term is then below 2^-theta of the q-point's sum (which holds
+  // its largest term, nu >= nu_min), so the sum loses less than N 2^-theta of
+  // itself and a target's mass less than 2^-theta.  vb takes the smallest V_q
+  // of the wave and the box of its points; the bins' largest psi~ are a.pbin /
+  // a.prow.
+  VBound vb;
+  auto set_bound = [&](const double (&L)[G]) {
+    double v = INFINITY, blo[DIM], bhi[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      blo[d] = INFINITY;
+      bhi[d] = -INFINITY;
+    }
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+      double x2 = 0.0;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        x2 = fma(x[k][d], x[k][d], x2);
+        blo[d] = fmin(blo[d], x[k][d]);
+        bhi[d] = fmax(bhi[d], x[k][d]);
+      }
+      // a q-point without candidates in this segment needs nothing from it
+      const double vk = (L[k] > -INFINITY)
+                            ? (L[k] - a.theta) * a.inv_coef + a.vshift - 0.5 * x2
+                            : INFINITY;
+      v = fmin(v, vk);
+    }
+    if (!active) v = INFINITY;
+    // rounding of the bound: a relative margin far below 2^-theta
+    v = wave_min(v);
+    vb.buni = v - 1e-12 * (1.0 + fabs(v));
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      vb.lo[d] = wave_min(active ? blo[d] : INFINITY) + a.o[d];
+      vb.hi[d] = wave_max(active ? bhi[d] : -INFINITY) + a.o[d];
+    }
+    vb.abin = a.pbin;
+    vb.arow = a.prow;
+  };
+  // the maxima of PASS 0 and the plain mode's sums are not bounded
+  constexpr bool vbound = VB && PASS != 0 && MODE != kF64Plain;
+  auto sweep_v = [&](auto &&f) {
+    if constexpr (vbound)
+      for_rows_vb<DIM>(a.tg, a.tstart, lo, hi, a.Rq, a.s_in, seg, a.K1, a.dense != 0, vb, f);
+    else
+      sweep(f);
+  };
   // the item's candidate records, with scalar loads
   cdouble *trec = cptr(a.trec);
-  auto records = [&](auto &&fn) {
-    sweep([&](int j0, int j1, auto tag) {
+  auto records = [&](auto &&fn, bool bounded) {
+    auto runs = [&](int j0, int j1, auto tag) {
       if (j0 >= j1) return;
       // each record is fetched one record ahead of its use
       TRec<DIM> cur;
@@ -432,16 +667,37 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
         fn(cur, decltype(tag)::value);
         cur = nxt;
       }
-    });
+    };
+    if (bounded)
+      sweep_v(runs);
+    else
+      sweep(runs);
+  };
+  // exact maximum over the item's candidates (every candidate: no value bound)
+  auto max_loop = [&]() {
+    records(
+        [&](const TRec<DIM> &r, bool tested) {
+          if (tested && !in_set<DIM>(r.y, c, a.smax)) return;
+#pragma unroll
+          for (int k = 0; k < G; ++k) {
+            double t = r.a;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) t = fma(x[k][d], r.yl[d], t);
+            T[k] = vmax64(T[k], t);
+          }
+        },
+        false);
   };
 
+  double L[G];  // lower bounds of the largest exponents (value bound)
+#pragma unroll
+  for (int k = 0; k < G; ++k) L[k] = M[k];
   bool seedq = false;  // the item's reference value came from the history seed
   if (PASS == 2) {
     // reference value of this item (see kSeedSpread)
     bool seed = false;
-    double dmx = -INFINITY;
+    double dmx = -INFINITY, dmn = INFINITY;
     if (a.hist) {
-      double dmn = INFINITY;
       if (a.dcx) {
         // extrema per chunk of kChunk records: a bound over a superset of
         // the item's candidates
@@ -467,46 +723,48 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
     }
     seedq = seed;
     if (seed) {
+      // the new maximum lies in [T_prev + dmn, T_prev + dmx]
 #pragma unroll
-      for (int k = 0; k < G; ++k)
-        M[k] = ceil(a.Tout[(int64_t)seg * a.nQ + q0 + k] + dmx) + a.seed_bias;
+      for (int k = 0; k < G; ++k) {
+        const double tp = a.Tout[(int64_t)seg * a.nQ + q0 + k];
+        M[k] = ceil(tp + dmx) + a.seed_bias;
+        L[k] = tp + dmn;
+      }
     } else {
-      // exact maximum over the item's candidates first
-      records([&](const TRec<DIM> &r, bool tested) {
-        if (tested && !in_set<DIM>(r.y, c, a.smax)) return;
-#pragma unroll
-        for (int k = 0; k < G; ++k) {
-          double t = r.a;
-#pragma unroll
-          for (int d = 0; d < DIM; ++d) t = fma(x[k][d], r.yl[d], t);
-          T[k] = vmax64(T[k], t);
-        }
-      });
+      max_loop();
 #pragma unroll
       for (int k = 0; k < G; ++k) {
         M[k] = ceil(T[k]);
+        L[k] = T[k];
         T[k] = -INFINITY;
       }
     }
   }
-  records(one);
+  if constexpr (vbound) set_bound(L);
+  records(one, true);
   if (PASS == 2 && seedq) {
     // The seeded bound holds by construction: the new maximum lies less than
     // kSeedSpread + 1 below it.  An item whose maximum fell further (history
     // that does not belong to these potentials) is swept again against its
-    // exact maximum, found by the sweep just done, so that its dominant terms
-    // keep their FP32 argument precision.
+    // exact maximum, so that its dominant terms keep their FP32 argument
+    // precision; the maximum comes from a sweep over every candidate, since
+    // the value bound of the first sweep rested on the history.
     bool low = false;
 #pragma unroll
-    for (int k = 0; k < G; ++k) low |= T[k] > -INFINITY && T[k] < M[k] - (kSeedSpread + 2.0);
+    for (int k = 0; k < G; ++k) low |= T[k] < M[k] - (kSeedSpread + 2.0);
     if (__ballot(low) != 0) {
+#pragma unroll
+      for (int k = 0; k < G; ++k) T[k] = -INFINITY;
+      max_loop();
 #pragma unroll
       for (int k = 0; k < G; ++k) {
         M[k] = ceil(T[k]);
+        L[k] = T[k];
         S[k] = 0.0;
         T[k] = -INFINITY;
       }
-      records(one);
+      if constexpr (vbound) set_bound(L);
+      records(one, true);
     }
   }
   if (active) {
@@ -610,7 +868,7 @@ __device__ __forceinline__ void qfin_body(const QFArgs &a, int64_t blk, double *
 // k_grad: target-major gradient partials
 // ---------------------------------------------------------------------------
 
-template <int DIM, int MODE, int G, bool BARY = false>
+template <int DIM, int MODE, int G, bool BARY = false, bool VB = false>
 __device__ __forceinline__ void grad_body(const GArgs &a, int64_t blk, int seg) {
   const int64_t j = blk * kWave + threadIdx.x;
   const bool active = j < a.N;
@@ -661,10 +919,33 @@ __device__ __forceinline__ void grad_body(const GArgs &a, int64_t blk, int seg) 
       }
     }
   };
+  // Value bound (VB): the pair (q, j) adds nu_j e^{t_qj - M_q} w_q / S_q
+  // to target j's mass; it is skipped when that is below 2^-theta w_q, i.e.
+  // when psi~_j - |x_q - y_j|^2/2 < V_q = (M_q + log S_q - theta)/c - |x'_q|^2/2
+  // (psi~_j = psi_j + eps ln nu_j), so a target loses less than 2^-theta of
+  // mass.  a.vbin / a.vrow hold the smallest V_q of each cell bin / row; the
+  // wave's largest psi~ and its box widened by a.rq (the largest distance of a
+  // cell's q-points from its center) bound the rest.  (A test per cell cost a
+  // quarter of a kept cell's work and was dropped.)
+  VBound vb;
+  if constexpr (VB) {
+    double yy = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) yy = fma(me.yl[d], me.yl[d], yy);
+    const double pt = (me.a + 0.5 * yy * a.inv_coef) * a.inv_coef + a.eps * log(me.nu);
+    vb.auni = wave_max(active ? pt : -INFINITY);
+    vb.bbin = a.vbin;
+    vb.brow = a.vrow;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      vb.lo[d] = lo[d] - a.rq;
+      vb.hi[d] = hi[d] + a.rq;
+    }
+  }
   auto body = [&](int c0, int c1, auto tag) {
     for (int c = c0; c < c1; ++c) {
+      cdouble *cp = ccr + (int64_t)c * 4;
       if constexpr (decltype(tag)::value) {
-        cdouble *cp = ccr + (int64_t)c * 4;
         double cv[DIM];
 #pragma unroll
         for (int d = 0; d < DIM; ++d) cv[d] = cp[d];
@@ -675,7 +956,9 @@ __device__ __forceinline__ void grad_body(const GArgs &a, int64_t blk, int seg) 
     }
   };
   auto cells = [&](auto &&f) {
-    if (a.dense) {
+    if constexpr (VB) {
+      for_rows_vb<DIM>(a.cg, a.cstart, lo, hi, a.Rq, a.s_in, seg, a.K2, a.dense != 0, vb, f);
+    } else if (a.dense) {
       const int64_t c0 = a.n_cells * seg / a.K2, c1 = a.n_cells * (seg + 1) / a.K2;
       f((int)c0, (int)c1, Inside{});
     } else {

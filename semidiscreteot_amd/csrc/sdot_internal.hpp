@@ -44,6 +44,9 @@ SDOT_HD inline int bin_of(double v, double o, double inv_h, int n) {
   return (int)f;
 }
 
+// Rows (x-runs of bins) of a grid.
+SDOT_HD inline int64_t grid_rows(const Grid &g) { return (int64_t)g.n[1] * g.n[2]; }
+
 // Row-major index of the bin holding p (the host and the device bin the same
 // way).
 SDOT_HD inline int bin_linear(const Grid &g, const double *p, int dim) {
@@ -115,6 +118,13 @@ struct Dev {
   double *dpsi_cx = nullptr;   // [ceil(N/64)] its maximum per chunk of 64 records
   double *dpsi_cn = nullptr;   // [ceil(N/64)] its minimum
   double *ucount = nullptr;    // [1] q-points whose FP32 sum lost its range
+  // value bound (per evaluation)
+  double *psit_r = nullptr;    // [N] psi + eps ln nu, row order
+  double *pbin = nullptr;      // [n_tbins] its largest per target bin
+  double *prow = nullptr;      // [n_trows] ... per bin row
+  double *vcell = nullptr;     // [n_cells] smallest V_q per cell, row order
+  double *vbin = nullptr;      // [n_cbins] ... per cell bin
+  double *vrow = nullptr;      // [n_crows] ... per cell-bin row
   double *scratch = nullptr;   // misc
   size_t scratch_bytes = 0;
   int64_t *i64scratch = nullptr;
@@ -149,6 +159,11 @@ struct Plan {
   int hist = 0;        // FP32 mode: seed the online sweep with the previous evaluation
   int dense = 0;       // every pair is a candidate (no enumeration, no test)
   double seed_bias = 0.0;  // test hook: raises the FP32 sweep's seeded reference values
+  // value bound of the pair sums (VBound): 2^-prune_bits, 0 = off
+  double prune_bits = 0.0;
+  double nu_min = 0.0;     // smallest target weight (0: some weight is 0, no bound)
+  double rq = 0.0;         // largest distance of a cell's q-points from its center
+  int64_t n_tbins = 0, n_cbins = 0;  // bins of the target / cell grids
   // the q-major sweeps stream the target records through LDS (record sets
   // larger than one XCD's L2)
 };
@@ -165,10 +180,12 @@ struct PrepArgs {
   double *PNpart;
   double *psi_prev;    // [N] psi of the previous FP32 evaluation (history mode)
   double *dpsi_r;      // [N] (psi - psi_prev) * coef, row order (history mode)
+  double *psit_r;      // [N] psi + eps ln nu, row order (value bound), or null
   int64_t N;
   int hist;
   double o[3];
   double coef;         // inv_eps * log-base scale
+  double eps;
 };
 
 struct P1Args {
@@ -193,6 +210,14 @@ struct P1Args {
   double s_in;            // squared radius of the untested inner runs (0: none)
   double seed_bias;       // added to every seeded reference value (0; a test hook that
                           // exercises the re-sweep of an item whose seed is too high)
+  // value bound of the sums (VBound, sdot_device.hpp): off when prune == 0
+  const double *pbin;     // [n_tbins] largest psi + eps ln nu per target bin
+  const double *prow;     // [n_trows] ... per bin row
+  double o[3];            // shift origin (raw = shifted + o)
+  double theta;           // 2^-bits in the sweep's units: bits (FP32) or bits ln 2 (FP64)
+  double inv_coef;        // 1 / coef
+  double vshift;          // eps ln nu_min
+  int prune;
 };
 
 struct QFArgs {
@@ -217,6 +242,13 @@ struct GArgs {
   int nq, K2, dense;
   double smax, Rq;
   double s_in;            // squared radius of the untested inner runs (0: none)
+  // value bound (VBound, sdot_device.hpp): off when prune == 0
+  const double *vcell;    // [n_cells] smallest V_q per cell, row order
+  const double *vbin;     // [n_cbins] ... per cell bin
+  const double *vrow;     // [n_crows] ... per cell-bin row
+  double rq;              // largest distance of a cell's q-points from its center
+  double inv_coef, eps;
+  int prune;
 };
 
 struct GFArgs {

@@ -78,14 +78,37 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
   prep_body<DIM>(a, blockIdx.x, sh);
 }
 
+// value bound: per-bin and per-row reductions, per-cell bounds
+template <bool MAX>
+__global__ __launch_bounds__(256) void k_binred(const double *v, const int32_t *start, int64_t nb,
+                                                double *out) {
+  bin_reduce_body<MAX>(v, start, nb, out, (int64_t)blockIdx.x * 256 + threadIdx.x);
+}
+
+template <bool MAX>
+__global__ __launch_bounds__(256) void k_rowred(const double *bins, int64_t nrows, int nx,
+                                                double *out) {
+  row_reduce_body<MAX>(bins, nrows, nx, out, (int64_t)blockIdx.x * 256 + threadIdx.x);
+}
+
+template <int DIM>
+__global__ __launch_bounds__(256) void k_cellv(const double *qrec_r, int64_t n_cells, int nq,
+                                               double theta, double inv_coef, int log2_units,
+                                               double *vcell) {
+  cellv_body<DIM>(qrec_r, n_cells, nq, theta, inv_coef, log2_units != 0, vcell,
+                  (int64_t)blockIdx.x * 256 + threadIdx.x);
+}
+
+static unsigned nblk256(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + 255) / 256); }
+
 __global__ __launch_bounds__(256) void k_chunkext(const double *dpsi_r, int64_t N, double *dcx,
                                                   double *dcn) {
   chunkext_body(dpsi_r, N, dcx, dcn, (int64_t)blockIdx.x * (256 / kWave) + threadIdx.x / kWave);
 }
 
-template <int DIM, int G, int MODE, int PASS>
+template <int DIM, int G, int MODE, int PASS, bool VB>
 __global__ __launch_bounds__(64) void k_pass1(P1Args a) {
-  pass1_body<DIM, G, MODE, PASS>(a, blockIdx.x, blockIdx.y);
+  pass1_body<DIM, G, MODE, PASS, VB>(a, blockIdx.x, blockIdx.y);
 }
 
 // Maximum of the K1 segment maxima per quadrature point (FP64 LSE), read once
@@ -107,17 +130,17 @@ __global__ __launch_bounds__(256) void k_qfin(QFArgs a) {
 // round-robin over the 8 XCDs, so with K2 a multiple of 8 XCD x runs exactly
 // the segments s = x (mod 8): its L2 holds one eighth of the q records instead
 // of streaming all of them.
-template <int DIM, int MODE, int G>
+template <int DIM, int MODE, int G, bool VB>
 __global__ __launch_bounds__(64) void k_grad(GArgs a) {
   const int64_t blk = blockIdx.x / a.K2;
-  grad_body<DIM, MODE, G>(a, blk, (int)(blockIdx.x - blk * a.K2));
+  grad_body<DIM, MODE, G, false, VB>(a, blk, (int)(blockIdx.x - blk * a.K2));
 }
 
 // The gradient sweep with the first moments of the plan (barycenters).
-template <int DIM, int MODE, int G>
+template <int DIM, int MODE, int G, bool VB>
 __global__ __launch_bounds__(64) void k_grad_bary(GArgs a) {
   const int64_t blk = blockIdx.x / a.K2;
-  grad_body<DIM, MODE, G, true>(a, blk, (int)(blockIdx.x - blk * a.K2));
+  grad_body<DIM, MODE, G, true, VB>(a, blk, (int)(blockIdx.x - blk * a.K2));
 }
 
 struct Origin {
@@ -416,12 +439,17 @@ hipError_t launch_hessian_final(const Plan &p, const double *d_A, int64_t Npad, 
 
 template <int DIM, int G, int MODE, int PASS>
 static void launch_pass1_t(const P1Args &a, int64_t nb, int K1, hipStream_t s) {
-  hipLaunchKernelGGL((k_pass1<DIM, G, MODE, PASS>), dim3((unsigned)nb, (unsigned)K1), dim3(64), 0,
-                     s, a);
+  const dim3 grid((unsigned)nb, (unsigned)K1);
+  // the value-bounded sweeps run the small grouping (at most 4 points per lane)
+  if (G <= 4 && a.prune)
+    hipLaunchKernelGGL((k_pass1<DIM, (G <= 4 ? G : 1), MODE, PASS, true>), grid, dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_pass1<DIM, G, MODE, PASS, false>), grid, dim3(64), 0, s, a);
 }
 
 template <int DIM, int MODE, int PASS>
 static hipError_t launch_pass1_g(int G, const P1Args &a, int64_t nb, int K1, hipStream_t s) {
+  if (a.prune && G > 4) return hipErrorInvalidValue;
   switch (G) {
     case 1: launch_pass1_t<DIM, 1, MODE, PASS>(a, nb, K1, s); break;
     case 2: launch_pass1_t<DIM, 2, MODE, PASS>(a, nb, K1, s); break;
@@ -438,14 +466,23 @@ static hipError_t launch_pass1_g(int G, const P1Args &a, int64_t nb, int K1, hip
 }
 
 template <int DIM, int MODE, bool BARY>
-static void launch_grad_g(int G, const GArgs &g, int64_t ntw, int K2, hipStream_t s) {
+static hipError_t launch_grad_g(int G, const GArgs &g, int64_t ntw, int K2, hipStream_t s) {
   const dim3 gg((unsigned)(ntw * K2));
-#define SDOT_GRAD(GG)                                                               \
-  do {                                                                              \
-    if constexpr (BARY)                                                             \
-      hipLaunchKernelGGL((k_grad_bary<DIM, MODE, GG>), gg, dim3(64), 0, s, g);      \
-    else                                                                            \
-      hipLaunchKernelGGL((k_grad<DIM, MODE, GG>), gg, dim3(64), 0, s, g);           \
+  if (g.prune && G > 4) return hipErrorInvalidValue;
+#define SDOT_GRAD(GG)                                                                          \
+  do {                                                                                         \
+    constexpr int GV = GG <= 4 ? GG : 1;                                                       \
+    if constexpr (BARY) {                                                                      \
+      if (g.prune)                                                                             \
+        hipLaunchKernelGGL((k_grad_bary<DIM, MODE, GV, true>), gg, dim3(64), 0, s, g);         \
+      else                                                                                     \
+        hipLaunchKernelGGL((k_grad_bary<DIM, MODE, GG, false>), gg, dim3(64), 0, s, g);        \
+    } else {                                                                                   \
+      if (g.prune)                                                                             \
+        hipLaunchKernelGGL((k_grad<DIM, MODE, GV, true>), gg, dim3(64), 0, s, g);              \
+      else                                                                                     \
+        hipLaunchKernelGGL((k_grad<DIM, MODE, GG, false>), gg, dim3(64), 0, s, g);             \
+    }                                                                                          \
   } while (0)
   switch (G) {
     case 1: SDOT_GRAD(1); break;
@@ -459,6 +496,7 @@ static void launch_grad_g(int G, const GArgs &g, int64_t ntw, int K2, hipStream_
     default: SDOT_GRAD(9); break;
   }
 #undef SDOT_GRAD
+  return hipGetLastError();
 }
 
 template <int DIM, int MODE>
@@ -485,7 +523,19 @@ static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_g
   pa.hist = p.hist;
   for (int k = 0; k < 3; ++k) pa.o[k] = p.origin[k];
   pa.coef = coef;
+  pa.eps = p.eps;
+  const bool prune = p.prune_bits > 0 && MODE != kF64Plain;
+  pa.psit_r = prune ? d.psit_r : nullptr;
   hipLaunchKernelGGL((k_prep<DIM>), dim3((unsigned)p.nPpart), dim3(256), 0, s, pa);
+  // value bound in the sweep's units: 2^-bits is bits log2 units (FP32) or bits ln 2 nats
+  const double theta = prune ? p.prune_bits * ((MODE == kF32Lse) ? 1.0 : 0.69314718055994530942) : 0.0;
+  if (prune) {
+    const int64_t ntr = grid_rows(p.tg);
+    hipLaunchKernelGGL(k_binred<true>, dim3(nblk256(p.n_tbins)), dim3(256), 0, s, d.psit_r,
+                       d.tstart, p.n_tbins, d.pbin);
+    hipLaunchKernelGGL(k_rowred<true>, dim3(nblk256(ntr)), dim3(256), 0, s, d.pbin, ntr,
+                       p.tg.n[0], d.prow);
+  }
   const bool chunks = MODE == kF32Lse && p.hist;
   if (chunks) {
     const int64_t nch = (p.N + kChunk - 1) / kChunk;
@@ -522,6 +572,13 @@ static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_g
   a.Mout = d.Mpart;
   a.Sout = d.Spart;
   a.Tout = d.Tpart;
+  a.prune = prune ? 1 : 0;
+  a.pbin = d.pbin;
+  a.prow = d.prow;
+  for (int k = 0; k < 3; ++k) a.o[k] = p.origin[k];
+  a.theta = theta;
+  a.inv_coef = 1.0 / coef;
+  a.vshift = prune ? p.eps * std::log(p.nu_min) : 0.0;
   hipError_t e;
   // exact maxima first (FP64 LSE; the FP32 sweep finds its own bounds)
   if (MODE == kF64Lse) {
@@ -558,6 +615,15 @@ static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_g
   q.plain_coef = 0.5 / p.eps;
   q.ln_scale = ln_scale;
   hipLaunchKernelGGL((k_qfin<DIM, MODE>), dim3((unsigned)p.nFpart), dim3(256), 0, s, q);
+  if (prune) {
+    const int64_t ncr = grid_rows(p.cg);
+    hipLaunchKernelGGL((k_cellv<DIM>), dim3(nblk256(p.n_cells)), dim3(256), 0, s, d.qrec_r,
+                       p.n_cells, p.nq, theta, 1.0 / coef, MODE == kF32Lse ? 1 : 0, d.vcell);
+    hipLaunchKernelGGL(k_binred<false>, dim3(nblk256(p.n_cbins)), dim3(256), 0, s, d.vcell,
+                       d.cstart, p.n_cbins, d.vbin);
+    hipLaunchKernelGGL(k_rowred<false>, dim3(nblk256(ncr)), dim3(256), 0, s, d.vbin, ncr,
+                       p.cg.n[0], d.vrow);
+  }
   if (timing) (void)hipEventRecord(prof->ev[3], s);
 
   GArgs g{};
@@ -577,10 +643,16 @@ static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_g
   g.Rq = p.Rq;
   g.s_in = p.s_in;
   g.Bpart = d_Bpart;
-  if (d_bary)
-    launch_grad_g<DIM, MODE, true>(p.G, g, p.n_twaves, p.K2, s);
-  else
-    launch_grad_g<DIM, MODE, false>(p.G, g, p.n_twaves, p.K2, s);
+  g.prune = prune ? 1 : 0;
+  g.vcell = d.vcell;
+  g.vbin = d.vbin;
+  g.vrow = d.vrow;
+  g.rq = p.rq;
+  g.inv_coef = 1.0 / coef;
+  g.eps = p.eps;
+  e = d_bary ? launch_grad_g<DIM, MODE, true>(p.G, g, p.n_twaves, p.K2, s)
+             : launch_grad_g<DIM, MODE, false>(p.G, g, p.n_twaves, p.K2, s);
+  if (e != hipSuccess) return e;
   if (timing) (void)hipEventRecord(prof->ev[4], s);
   GFArgs gf{d.Gpart, d.trec_m, d.tm2o, d_grad, p.N, p.K2};
   hipLaunchKernelGGL((k_gfin<DIM>), dim3((unsigned)((p.N + 255) / 256)), dim3(256), 0, s, gf);

@@ -10,9 +10,11 @@
 // "componentwise" (SotSolver.cc:156), distance_threshold_type falls back to
 // pointwise for anything but "integral"/"geometric" (SotSolver.cc:392-406).
 // `precision` (fp64 | fp32_lse), `line_search` (reference | noise_aware) and
-// `preconditioner` (none | diagonal) are this library's own keys.
+// `preconditioner` (none | diagonal) and `prune_bits` (a number) are this
+// library's own keys.
 #include <cctype>
 #include <climits>
+#include <cmath>
 #include <cstring>
 #include <cstdio>
 #include <cstdlib>
@@ -80,6 +82,7 @@ extern "C" int sdot_params_default(sdot_params *p) {
   p->precision = SDOT_PREC_FP64;
   p->line_search = SDOT_LS_REFERENCE;
   p->preconditioner = SDOT_PRECOND_NONE;
+  p->prune_bits = -1.0;
   return SDOT_OK;
 }
 
@@ -160,6 +163,11 @@ bool solver_key(const std::string &key, const std::string &val, sdot_params *p) 
     if (l == "none") p->preconditioner = SDOT_PRECOND_NONE;
     else if (l == "diagonal") p->preconditioner = SDOT_PRECOND_DIAGONAL;
     else ok = false;
+  } else if (key == "prune_bits") {
+    char *end = nullptr;
+    const double v = std::strtod(val.c_str(), &end);
+    ok = end && *end == '\0' && !val.empty() && std::isfinite(v);
+    if (ok) p->prune_bits = v;
   } else
     ok = false;  // deal.II rejects undeclared entries
   return ok;

@@ -34,8 +34,10 @@ def _p(a):
 class SolverParameters:
     """SotParameterManager::SolverParameters (ParameterManager.h:56-72) plus
     `precision` ("fp64" | "fp32_lse"), `line_search` ("reference" |
-    "noise_aware") and `preconditioner` ("none" | "diagonal": initial inverse
-    Hessian diag(epsilon / nu_j) of the L-BFGS recursion), include/sdot.h."""
+    "noise_aware"), `preconditioner` ("none" | "diagonal": initial inverse
+    Hessian diag(epsilon / nu_j) of the L-BFGS recursion) and `prune_bits`
+    (value bound of the pair sums: < 0 the precision's default, 0 off),
+    include/sdot.h."""
     max_iterations: int = 1000
     tolerance: float = 1.0
     solver_control_type: str = "l1norm"
@@ -54,6 +56,7 @@ class SolverParameters:
     precision: str = "fp64"
     line_search: str = "reference"
     preconditioner: str = "none"
+    prune_bits: float = -1.0
 
     def to_c(self) -> L.Params:
         p = L.Params()
@@ -75,6 +78,7 @@ class SolverParameters:
         p.precision = _PREC[self.precision]
         p.line_search = _LS[self.line_search]
         p.preconditioner = _PC[self.preconditioner]
+        p.prune_bits = float(self.prune_bits)
         return p
 
     @classmethod
@@ -94,7 +98,7 @@ class SolverParameters:
                    epsilon_scaling_steps=p.epsilon_scaling_steps,
                    use_log_sum_exp_trick=bool(p.use_log_sum_exp_trick),
                    precision=inv_p[p.precision], line_search=inv_l[p.line_search],
-                   preconditioner=inv_c[p.preconditioner])
+                   preconditioner=inv_c[p.preconditioner], prune_bits=p.prune_bits)
 
     @classmethod
     def from_prm(cls, text: str | None = None, path: str | None = None) -> "SolverParameters":
