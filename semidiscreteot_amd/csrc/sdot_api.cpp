@@ -90,9 +90,6 @@ Grid make_grid(const double *lo, const double *hi, int dim, int64_t n, double pe
 }
 
 constexpr int kMaxSeg = 128;
-// Default value bound of FP32_LSE evaluations: terms below 2^-50 of their
-// q-point's largest, far below the 2^-24 rounding of the FP32 exponentials.
-constexpr double kPruneBitsF32 = 50.0;
 constexpr int kProfSets = 128;
 
 }  // namespace
@@ -292,10 +289,8 @@ int plan_eval(sdot_ctx *c, double R) {
   // 7.5 -> 5.7 ms dense; 20,000 targets, eps 3e-3 (radius 0.26 of the
   // diagonal): 3.2 -> 3.4 ms; 4,096 targets, eps 1e-3: 0.87 -> 0.94 ms.
   if (c->params.prune_bits < 0) {
-    p.prune_bits = 0.0;
-    const double r0 = std::sqrt(2.0 * kPruneBitsF32 * std::log(2.0) * p.eps);
-    if (p.mode == kF32Lse && r0 <= 0.2 * diag && (double)p.N * (double)p.nQ >= 4294967296.0)
-      p.prune_bits = kPruneBitsF32;
+    p.prune_bits = (p.mode == kF32Lse && value_bound_default(p.eps, diag, R, p.N, p.nQ))
+                       ? kPruneBitsF32 : 0.0;
   } else {
     p.prune_bits = c->params.prune_bits;
   }
@@ -1758,6 +1753,13 @@ int sdot::ctx_source_view(sdot_ctx *c, SourceView &v) {
   v.cc_r = d.cc_r;
   v.cstart = d.cstart;
   v.cell_r2m = d.cell_r2m;
+  v.Gs = p.grp_small.G;
+  v.gpc_s = p.grp_small.gpc;
+  v.n_groups_s = p.grp_small.n_groups;
+  v.n_blocks_s = p.grp_small.n_blocks;
+  v.n_cbins = p.n_cbins;
+  v.blkbox_s = d.blkbox_s;
+  v.rq = p.rq;
   return SDOT_OK;
 }
 

@@ -321,7 +321,29 @@ struct SourceView {
   const double *blkbox = nullptr, *cc_r = nullptr;
   const int32_t *cstart = nullptr;
   const int64_t *cell_r2m = nullptr;
+  // the 4-point grouping of the value-bounded sweeps, and the bound's cell data
+  int Gs = 1, gpc_s = 1;
+  int64_t n_groups_s = 0, n_blocks_s = 0, n_cbins = 0;
+  const double *blkbox_s = nullptr;
+  double rq = 0.0;
 };
+// Value bound of the pair sums: FP32_LSE evaluations are bounded at this many
+// bits by default where the bound pays (value_bound_default below): terms below
+// 2^-50 of their q-point's largest, far below the 2^-24 rounding of the FP32
+// exponentials.
+constexpr double kPruneBitsF32 = 50.0;
+// Whether the FP32_LSE default bounds an evaluation of N targets against nQ
+// quadrature points at epsilon and truncation radius R, with domain diagonal
+// diag: the radius sqrt(2 bits ln2 eps) beyond which a target's terms fall
+// below 2^-bits (at equal potentials) is at most a fifth of the diagonal and
+// half of R, and there are at least 2^32 pairs.  Where it is close to R the
+// bound skips little and its sweeps (fewer waves per SIMD) cost more: the
+// 500,000-target ball workload (R = 0.205, radius 0.186) solved in 19.9 s
+// bounded against 17.9 s.
+inline bool value_bound_default(double eps, double diag, double R, int64_t N, int64_t nQ) {
+  const double r0 = std::sqrt(2.0 * kPruneBitsF32 * 0.69314718055994530942 * eps);
+  return r0 <= 0.2 * diag && !(r0 > 0.5 * R) && (double)N * (double)nQ >= 4294967296.0;
+}
 int ctx_source_view(sdot_ctx *c, SourceView &v);
 // Count of serving queues attached to the context (its source stays fixed meanwhile).
 void ctx_attach(sdot_ctx *c, int delta);

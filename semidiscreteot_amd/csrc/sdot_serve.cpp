@@ -116,7 +116,7 @@ struct Carve {
 };
 
 struct SolveShape {
-  int64_t N = 0, ntb = 0, n_twaves = 0, nPpart = 0;
+  int64_t N = 0, ntb = 0, ntr = 0, n_twaves = 0, nPpart = 0;
   size_t lay_bytes = 0;  // scratch of the device-built target layout
   int K1 = 1, K2 = 1;
   bool fp32 = false;
@@ -126,6 +126,7 @@ struct SolveOffsets {
   size_t y, nu, pos_r, pos_m, tm2o, tstart, twbox;
   size_t trec_r, trec_m, qrec_r, Mpart, Spart, Tpart, Gpart;
   size_t Fpart, Cpart, Upart, PNpart, psi_prev, dpsi;
+  size_t psit_r, pbin, prow, vcell, vbin, vrow;  // value bound (FP32 solves)
   size_t x, g, x0, gv, p, S[kHist], Y[kHist], hist;
   size_t total;
 };
@@ -154,6 +155,13 @@ SolveOffsets carve(const SourceView &src, const SolveShape &s) {
   o.PNpart = c.take<double>(s.nPpart);
   o.psi_prev = s.fp32 ? c.take<double>(s.N) : SIZE_MAX;
   o.dpsi = s.fp32 ? c.take<double>(s.N) : SIZE_MAX;
+  const int64_t ncr = grid_rows(src.cg);
+  o.psit_r = s.fp32 ? c.take<double>(s.N) : SIZE_MAX;
+  o.pbin = s.fp32 ? c.take<double>(s.ntb) : SIZE_MAX;
+  o.prow = s.fp32 ? c.take<double>(s.ntr) : SIZE_MAX;
+  o.vcell = s.fp32 ? c.take<double>(src.n_cells) : SIZE_MAX;
+  o.vbin = s.fp32 ? c.take<double>(src.n_cbins) : SIZE_MAX;
+  o.vrow = s.fp32 ? c.take<double>(ncr) : SIZE_MAX;
   o.x = c.take<double>(s.N);
   o.g = c.take<double>(s.N);
   o.x0 = c.take<double>(s.N);
@@ -182,6 +190,7 @@ struct Request {
   SolveShape shape;
   int dense = 0;
   double smax = 0.0, Rq = 0.0, s_in = 0.0;
+  double diag = 0.0, nu_min = 0.0;  // domain diagonal, smallest weight (value bound)
   size_t bytes = 0;
   double t_submit = 0.0;
 };
@@ -252,6 +261,7 @@ int fill_shape(sdot_server *s, Request &r) {
   SolveShape &sh = r.shape;
   sh.N = r.N;
   sh.ntb = r.L.n_bins;
+  sh.ntr = grid_rows(r.L.tg);
   sh.n_twaves = r.L.n_twaves;
   sh.lay_bytes = layout_scratch_bytes(r.L);
   sh.nPpart = (r.N + 255) / 256;
@@ -264,6 +274,8 @@ int fill_shape(sdot_server *s, Request &r) {
     diag2 += (hi - lo) * (hi - lo);
   }
   const double diag = std::sqrt(diag2);
+  r.diag = diag;
+  r.nu_min = r.nu.empty() ? 0.0 : *std::min_element(r.nu.begin(), r.nu.end());
   const double R = r.R;
   r.smax = smax_for(R);
   r.s_in = inner_s(r.smax);
@@ -308,6 +320,13 @@ void build_desc(sdot_server *s, Solve &v) {
   double *dpsi = dptr<double>(s, v, v.o.dpsi);
   double *x = dptr<double>(s, v, v.o.x), *g = dptr<double>(s, v, v.o.g);
   const int64_t nFpart = (src.nQ + 255) / 256;
+  // value bound: the single-solve default rule at this stage's epsilon
+  const bool bound = fp32 && r.nu_min > 0.0 && value_bound_default(eps, r.diag, r.R, r.N, src.nQ);
+  const double theta = kPruneBitsF32;  // FP32 sweeps: log2 units
+  double *psit_r = dptr<double>(s, v, v.o.psit_r);
+  double *pbin = dptr<double>(s, v, v.o.pbin), *prow = dptr<double>(s, v, v.o.prow);
+  double *vcell = dptr<double>(s, v, v.o.vcell), *vbin = dptr<double>(s, v, v.o.vbin);
+  double *vrow = dptr<double>(s, v, v.o.vrow);
 
   PrepArgs &pa = v.desc.prep;
   pa = PrepArgs{};
@@ -325,6 +344,8 @@ void build_desc(sdot_server *s, Solve &v) {
   pa.hist = v.seeded ? 1 : 0;
   for (int k = 0; k < 3; ++k) pa.o[k] = src.origin[k];
   pa.coef = coef;
+  pa.eps = eps;
+  pa.psit_r = bound ? psit_r : nullptr;
 
   P1Args &a = v.desc.p1;
   a = P1Args{};
@@ -353,6 +374,19 @@ void build_desc(sdot_server *s, Solve &v) {
   a.s_in = r.s_in;
   a.plain_coef = 0.5 / eps;
   a.coef = coef;
+  if (bound) {
+    // the 4-point grouping (blocks of half as many cells) and the bound
+    a.blkbox = src.blkbox_s;
+    a.n_groups = src.n_groups_s;
+    a.gpc = src.gpc_s;
+    a.prune = 1;
+    a.pbin = pbin;
+    a.prow = prow;
+    for (int k = 0; k < 3; ++k) a.o[k] = src.origin[k];
+    a.theta = theta;
+    a.inv_coef = 1.0 / coef;
+    a.vshift = eps * std::log(r.nu_min);
+  }
 
   QFArgs &q = v.desc.qf;
   q = QFArgs{};
@@ -391,6 +425,38 @@ void build_desc(sdot_server *s, Solve &v) {
   gr.smax = r.smax;
   gr.Rq = r.Rq;
   gr.s_in = r.s_in;
+  if (bound) {
+    gr.prune = 1;
+    gr.vcell = vcell;
+    gr.vbin = vbin;
+    gr.vrow = vrow;
+    gr.rq = src.rq;
+    gr.inv_coef = 1.0 / coef;
+    gr.eps = eps;
+  }
+
+  VBArgs &vb = v.desc.vb;
+  vb = VBArgs{};
+  vb.psit_r = psit_r;
+  vb.tstart = a.tstart;
+  vb.pbin = pbin;
+  vb.prow = prow;
+  vb.ntb = sh.ntb;
+  vb.ntr = sh.ntr;
+  vb.tnx = r.L.tg.n[0];
+  vb.qrec_r = qrec_r;
+  vb.cstart = src.cstart;
+  vb.vcell = vcell;
+  vb.vbin = vbin;
+  vb.vrow = vrow;
+  vb.n_cells = src.n_cells;
+  vb.ncb = src.n_cbins;
+  vb.ncr = grid_rows(src.cg);
+  vb.cnx = src.cg.n[0];
+  vb.nq = src.nq;
+  vb.log2_units = 1;
+  vb.theta = theta;
+  vb.inv_coef = 1.0 / coef;
 
   v.desc.gf = GFArgs{Gpart, trec_m, dptr<int32_t>(s, v, v.o.tm2o), g, r.N, sh.K2};
   // out / ucount are set per batch position in upload_batch
@@ -413,10 +479,18 @@ void build_desc(sdot_server *s, Solve &v) {
 
   for (int64_t &it : v.items) it = 0;
   v.items[kLPrep] = sh.nPpart;
-  v.items[fp32 ? kLPass32 : kLPass64] = src.n_blocks * sh.K1;
+  v.items[fp32 ? (bound ? kLPass32V : kLPass32) : kLPass64] =
+      (bound ? src.n_blocks_s : src.n_blocks) * sh.K1;
   v.items[fp32 ? kLQfin32 : kLQfin64] = nFpart;
-  v.items[fp32 ? kLGrad32 : kLGrad64] = sh.n_twaves * sh.K2;
+  v.items[fp32 ? (bound ? kLGrad32V : kLGrad32) : kLGrad64] = sh.n_twaves * sh.K2;
   v.items[kLGfin] = (r.N + 255) / 256;
+  if (bound) {
+    v.items[kLTbin] = (sh.ntb + 255) / 256;
+    v.items[kLTrow] = (sh.ntr + 255) / 256;
+    v.items[kLCell] = (src.n_cells + 255) / 256;
+    v.items[kLCbin] = (src.n_cbins + 255) / 256;
+    v.items[kLCrow] = (vb.ncr + 255) / 256;
+  }
 }
 
 const SlotHist kEmptyHist = [] {
@@ -561,7 +635,7 @@ int launch_step(sdot_server *s) {
   if (s->exec) {
     SV_TRY(hipGraphLaunch(s->exec, s->stream));
   } else {
-    SV_TRY(launch_serve_step(s->src.dim, s->src.G, s->sd, s->stream));
+    SV_TRY(launch_serve_step(s->src.dim, s->src.G, s->src.Gs, s->sd, s->stream));
   }
   return SDOT_OK;
 }
@@ -662,11 +736,11 @@ int server_init(sdot_server *s, const sdot_server_config &cfg) {
   sd.nact = reinterpret_cast<int *>(s->d_block + s->off_nact);
   sd.out = s->d_out;
   sd.ctr = s->d_ctr;
-  SV_TRY(serve_grid_sizes(s->src.dim, s->src.G, s->device, sd));
+  SV_TRY(serve_grid_sizes(s->src.dim, s->src.G, s->src.Gs, s->device, sd));
   sd.grid_slots = (unsigned)s->max_slots;
   if (s->use_graph) {
     SV_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-    hipError_t le = launch_serve_step(s->src.dim, s->src.G, sd, s->stream);
+    hipError_t le = launch_serve_step(s->src.dim, s->src.G, s->src.Gs, sd, s->stream);
     hipGraph_t g = nullptr;
     hipError_t ce = hipStreamEndCapture(s->stream, &g);
     s->graph = g;

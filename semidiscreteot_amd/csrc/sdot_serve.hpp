@@ -7,6 +7,23 @@
 
 namespace sdot {
 
+// Reductions of one solve's value bound (VBound, sdot_device.hpp): the largest
+// psi + eps ln nu per target bin and row, the smallest V_q per cell, cell bin
+// and row.
+struct VBArgs {
+  const double *psit_r;
+  const int32_t *tstart;
+  double *pbin, *prow;
+  int64_t ntb, ntr;
+  int tnx;
+  const double *qrec_r;
+  const int32_t *cstart;
+  double *vcell, *vbin, *vrow;
+  int64_t n_cells, ncb, ncr;
+  int cnx, nq, log2_units;
+  double theta, inv_coef;
+};
+
 // Argument blocks of one solve's evaluation.
 struct SlotDesc {
   PrepArgs prep;
@@ -15,6 +32,7 @@ struct SlotDesc {
   GArgs gr;
   GFArgs gf;
   FinArgs fin;
+  VBArgs vb;
   int64_t nPpart;
 };
 
@@ -36,17 +54,23 @@ struct SlotVecs {
 // Work lists of one step.  Kernels that do not depend on the arithmetic mode
 // run over every solve; the mode-templated passes run over the solves of their
 // mode (the other solves contribute no items).
+// FP32 solves whose evaluations are value-bounded run their own sweep lists
+// (V) and the bound's reductions (blocks of 256 bins / rows / cells).
 enum : int {
   kLPrep = 0,
-  kLPass32, kLPass64,
+  kLPass32, kLPass32V, kLPass64,
   kLQfin32, kLQfin64,
-  kLGrad32, kLGrad64,
+  kLGrad32, kLGrad32V, kLGrad64,
   kLGfin,
+  kLTbin, kLTrow, kLCell, kLCbin, kLCrow,
   kNumLists
 };
 
 // Item counters of the dynamically scheduled sweeps (one per kernel and XCD).
-enum : int { kQPass2F32 = 0, kQPass0F64, kQPass1F64, kQGradF32, kQGradF64, kNumQueues };
+enum : int {
+  kQPass2F32 = 0, kQPass2F32V, kQPass0F64, kQPass1F64, kQGradF32, kQGradF32V, kQGradF64,
+  kNumQueues
+};
 
 struct ServeDev {
   SlotDesc *desc;
@@ -64,8 +88,10 @@ struct ServeDev {
   unsigned grid_slots;  // one workgroup per solve (upper bound on the batch)
 };
 
-hipError_t launch_serve_step(int dim, int G, const ServeDev &sd, hipStream_t s);
+// G: points per lane of the unbounded q-major sweeps; Gs (at most 4): of the
+// value-bounded ones.
+hipError_t launch_serve_step(int dim, int G, int Gs, const ServeDev &sd, hipStream_t s);
 // Sets the persistent grid sizes from the occupancy of the step's kernels.
-hipError_t serve_grid_sizes(int dim, int G, int device, ServeDev &sd);
+hipError_t serve_grid_sizes(int dim, int G, int Gs, int device, ServeDev &sd);
 
 }  // namespace sdot

@@ -70,7 +70,7 @@ __global__ __launch_bounds__(256) void kl_prep(const SlotDesc *__restrict__ desc
   }
 }
 
-template <int DIM, int G, int MODE, int PASS>
+template <int DIM, int G, int MODE, int PASS, bool VB = false>
 __global__ __launch_bounds__(64) void kl_pass1(const SlotDesc *__restrict__ desc,
                                                const int64_t *__restrict__ base,
                                                const int *__restrict__ nact,
@@ -81,8 +81,41 @@ __global__ __launch_bounds__(64) void kl_pass1(const SlotDesc *__restrict__ desc
     const P1Args a = desc[s].p1;
     const int64_t loc = it - base[s];
     const int64_t blk = loc / a.K1;
-    pass1_body<DIM, G, MODE, PASS>(a, blk, (int)(loc - blk * a.K1));
+    pass1_body<DIM, G, MODE, PASS, VB>(a, blk, (int)(loc - blk * a.K1));
   });
+}
+
+// Value-bound reductions (one 256-entry block per item): target bins and rows
+// after kl_prep, cells, cell bins and rows after kl_qfin.
+template <int K>
+__global__ __launch_bounds__(256) void kl_vbred(const SlotDesc *__restrict__ desc,
+                                                const int64_t *__restrict__ base,
+                                                const int *__restrict__ nact) {
+  const int n = *nact;
+  const int64_t total = base[n];
+  for (int64_t it = blockIdx.x; it < total; it += gridDim.x) {
+    const int s = find_item(base, n, it);
+    const VBArgs &v = desc[s].vb;
+    const int64_t i = (it - base[s]) * 256 + threadIdx.x;
+    if constexpr (K == kLTbin) bin_reduce_body<true>(v.psit_r, v.tstart, v.ntb, v.pbin, i);
+    if constexpr (K == kLTrow) row_reduce_body<true>(v.pbin, v.ntr, v.tnx, v.prow, i);
+    if constexpr (K == kLCbin) bin_reduce_body<false>(v.vcell, v.cstart, v.ncb, v.vbin, i);
+    if constexpr (K == kLCrow) row_reduce_body<false>(v.vbin, v.ncr, v.cnx, v.vrow, i);
+  }
+}
+
+template <int DIM>
+__global__ __launch_bounds__(256) void kl_cellv(const SlotDesc *__restrict__ desc,
+                                                const int64_t *__restrict__ base,
+                                                const int *__restrict__ nact) {
+  const int n = *nact;
+  const int64_t total = base[n];
+  for (int64_t it = blockIdx.x; it < total; it += gridDim.x) {
+    const int s = find_item(base, n, it);
+    const VBArgs &v = desc[s].vb;
+    cellv_body<DIM>(v.qrec_r, v.n_cells, v.nq, v.theta, v.inv_coef, v.log2_units != 0, v.vcell,
+                    (it - base[s]) * 256 + threadIdx.x);
+  }
 }
 
 template <int DIM, int MODE>
@@ -99,7 +132,7 @@ __global__ __launch_bounds__(256) void kl_qfin(const SlotDesc *__restrict__ desc
   }
 }
 
-template <int DIM, int MODE, int G>
+template <int DIM, int MODE, int G, bool VB = false>
 __global__ __launch_bounds__(64) void kl_grad(const SlotDesc *__restrict__ desc,
                                               const int64_t *__restrict__ base,
                                               const int *__restrict__ nact,
@@ -110,7 +143,7 @@ __global__ __launch_bounds__(64) void kl_grad(const SlotDesc *__restrict__ desc,
     const GArgs a = desc[s].gr;
     const int64_t loc = it - base[s];
     const int64_t blk = loc / a.K2;
-    grad_body<DIM, MODE, G>(a, blk, (int)(loc - blk * a.K2));
+    grad_body<DIM, MODE, G, false, VB>(a, blk, (int)(loc - blk * a.K2));
   });
 }
 
@@ -293,15 +326,19 @@ __global__ __launch_bounds__(kBT) void kl_dots(const SlotVecs *__restrict__ vecs
 // ---------------------------------------------------------------------------
 // launcher: one step for the current batch
 // ---------------------------------------------------------------------------
-template <int DIM, int G>
+template <int DIM, int G, int GS>
 static void step_t(const ServeDev &sd, hipStream_t s) {
   const unsigned gp = sd.grid_pass, gg = sd.grid_grad, gb = sd.grid_b, gs = sd.grid_slots;
   const int64_t *b = sd.base;
   const int64_t L = sd.bstride;
   hipLaunchKernelGGL(kl_action, dim3(gs), dim3(kBT), 0, s, sd.vecs, sd.in, sd.out, sd.nact, sd.ctr);
   hipLaunchKernelGGL((kl_prep<DIM>), dim3(gb), dim3(256), 0, s, sd.desc, b + kLPrep * L, sd.nact);
+  hipLaunchKernelGGL(kl_vbred<kLTbin>, dim3(gb), dim3(256), 0, s, sd.desc, b + kLTbin * L, sd.nact);
+  hipLaunchKernelGGL(kl_vbred<kLTrow>, dim3(gb), dim3(256), 0, s, sd.desc, b + kLTrow * L, sd.nact);
   hipLaunchKernelGGL((kl_pass1<DIM, G, kF32Lse, 2>), dim3(gp), dim3(64), 0, s, sd.desc,
                      b + kLPass32 * L, sd.nact, sd.ctr + kQPass2F32 * 8);
+  hipLaunchKernelGGL((kl_pass1<DIM, GS, kF32Lse, 2, true>), dim3(gp), dim3(64), 0, s, sd.desc,
+                     b + kLPass32V * L, sd.nact, sd.ctr + kQPass2F32V * 8);
   hipLaunchKernelGGL((kl_pass1<DIM, G, kF64Lse, 0>), dim3(gp), dim3(64), 0, s, sd.desc,
                      b + kLPass64 * L, sd.nact, sd.ctr + kQPass0F64 * 8);
   hipLaunchKernelGGL((kl_pass1<DIM, G, kF64Lse, 1>), dim3(gp), dim3(64), 0, s, sd.desc,
@@ -310,8 +347,13 @@ static void step_t(const ServeDev &sd, hipStream_t s) {
                      b + kLQfin32 * L, sd.nact);
   hipLaunchKernelGGL((kl_qfin<DIM, kF64Lse>), dim3(gb), dim3(256), 0, s, sd.desc,
                      b + kLQfin64 * L, sd.nact);
+  hipLaunchKernelGGL((kl_cellv<DIM>), dim3(gb), dim3(256), 0, s, sd.desc, b + kLCell * L, sd.nact);
+  hipLaunchKernelGGL(kl_vbred<kLCbin>, dim3(gb), dim3(256), 0, s, sd.desc, b + kLCbin * L, sd.nact);
+  hipLaunchKernelGGL(kl_vbred<kLCrow>, dim3(gb), dim3(256), 0, s, sd.desc, b + kLCrow * L, sd.nact);
   hipLaunchKernelGGL((kl_grad<DIM, kF32Lse, G>), dim3(gg), dim3(64), 0, s, sd.desc,
                      b + kLGrad32 * L, sd.nact, sd.ctr + kQGradF32 * 8);
+  hipLaunchKernelGGL((kl_grad<DIM, kF32Lse, GS, true>), dim3(gg), dim3(64), 0, s, sd.desc,
+                     b + kLGrad32V * L, sd.nact, sd.ctr + kQGradF32V * 8);
   hipLaunchKernelGGL((kl_grad<DIM, kF64Lse, G>), dim3(gg), dim3(64), 0, s, sd.desc,
                      b + kLGrad64 * L, sd.nact, sd.ctr + kQGradF64 * 8);
   hipLaunchKernelGGL((kl_gfin<DIM>), dim3(gb), dim3(256), 0, s, sd.desc, b + kLGfin * L, sd.nact);
@@ -319,22 +361,12 @@ static void step_t(const ServeDev &sd, hipStream_t s) {
   hipLaunchKernelGGL(kl_dots, dim3(gs), dim3(kBT), 0, s, sd.vecs, sd.in, sd.out, sd.nact);
 }
 
-template <int DIM>
-static hipError_t step_g(int G, const ServeDev &sd, hipStream_t s) {
-  switch (G) {
-    case 1: step_t<DIM, 1>(sd, s); break;
-    case 2: step_t<DIM, 2>(sd, s); break;
-    case 3: step_t<DIM, 3>(sd, s); break;
-    case 4: step_t<DIM, 4>(sd, s); break;
-    case 5: step_t<DIM, 5>(sd, s); break;
-    case 6: step_t<DIM, 6>(sd, s); break;
-    case 7: step_t<DIM, 7>(sd, s); break;
-    case 8: step_t<DIM, 8>(sd, s); break;
-    case 9: step_t<DIM, 9>(sd, s); break;
-    default: return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
-}
+// (G, Gs) of a source: G is the largest divisor of nq up to 9 and Gs the
+// largest up to 4; these are all the pairs nq = 1 .. 4096 produce.
+#define SDOT_SERVE_GROUPINGS(X, D)                                                             \
+  X(D, 1, 1) X(D, 2, 2) X(D, 3, 3) X(D, 4, 4) X(D, 5, 1) X(D, 5, 2) X(D, 5, 3) X(D, 5, 4)      \
+  X(D, 6, 3) X(D, 6, 4) X(D, 7, 1) X(D, 7, 2) X(D, 7, 3) X(D, 7, 4) X(D, 8, 4) X(D, 9, 3)      \
+  X(D, 9, 4)
 
 // Workgroups of `kernel` resident per CU (occupancy), at least 1.
 template <class K>
@@ -344,40 +376,51 @@ static int resident(K kernel, int threads) {
   return n > 0 ? n : 1;
 }
 
-template <int DIM, int G>
+template <int DIM, int G, int GS>
 static void grids_t(unsigned cus, ServeDev &sd) {
   // the smallest occupancy of the kernels sharing a grid size, so that every
   // persistent workgroup is resident from the start
   int pass = resident(kl_pass1<DIM, G, kF32Lse, 2>, 64);
+  pass = std::min(pass, resident(kl_pass1<DIM, GS, kF32Lse, 2, true>, 64));
   pass = std::min(pass, resident(kl_pass1<DIM, G, kF64Lse, 0>, 64));
   pass = std::min(pass, resident(kl_pass1<DIM, G, kF64Lse, 1>, 64));
   int grad = std::min(resident(kl_grad<DIM, kF32Lse, G>, 64), resident(kl_grad<DIM, kF64Lse, G>, 64));
+  grad = std::min(grad, resident(kl_grad<DIM, kF32Lse, GS, true>, 64));
   int b = std::min(resident(kl_prep<DIM>, 256), resident(kl_qfin<DIM, kF32Lse>, 256));
   b = std::min(b, std::min(resident(kl_qfin<DIM, kF64Lse>, 256), resident(kl_gfin<DIM>, 256)));
+  b = std::min(b, resident(kl_cellv<DIM>, 256));
   // multiples of 8 keep a persistent workgroup's items on one XCD's segments
   sd.grid_pass = cus * (unsigned)pass;
   sd.grid_grad = cus * (unsigned)grad;
   sd.grid_b = cus * (unsigned)b;
 }
 
-hipError_t serve_grid_sizes(int dim, int G, int device, ServeDev &sd) {
+hipError_t serve_grid_sizes(int dim, int G, int Gs, int device, ServeDev &sd) {
   int cus = 0;
   hipError_t e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
   if (e != hipSuccess) return e;
   cus = std::max(8, cus - cus % 8);
-#define SDOT_GRIDS(D, GG) \
-  if (dim == D && G == GG) { grids_t<D, GG>((unsigned)cus, sd); return hipSuccess; }
-  SDOT_GRIDS(2, 1) SDOT_GRIDS(2, 2) SDOT_GRIDS(2, 3) SDOT_GRIDS(2, 4) SDOT_GRIDS(2, 5)
-  SDOT_GRIDS(2, 6) SDOT_GRIDS(2, 7) SDOT_GRIDS(2, 8) SDOT_GRIDS(2, 9)
-  SDOT_GRIDS(3, 1) SDOT_GRIDS(3, 2) SDOT_GRIDS(3, 3) SDOT_GRIDS(3, 4) SDOT_GRIDS(3, 5)
-  SDOT_GRIDS(3, 6) SDOT_GRIDS(3, 7) SDOT_GRIDS(3, 8) SDOT_GRIDS(3, 9)
+#define SDOT_GRIDS(D, GG, GGS)                                   \
+  if (dim == D && G == GG && Gs == GGS) {                        \
+    grids_t<D, GG, GGS>((unsigned)cus, sd);                      \
+    return hipSuccess;                                           \
+  }
+  SDOT_SERVE_GROUPINGS(SDOT_GRIDS, 2)
+  SDOT_SERVE_GROUPINGS(SDOT_GRIDS, 3)
 #undef SDOT_GRIDS
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_serve_step(int dim, int G, const ServeDev &sd, hipStream_t s) {
-  if (dim == 2) return step_g<2>(G, sd, s);
-  return step_g<3>(G, sd, s);
+hipError_t launch_serve_step(int dim, int G, int Gs, const ServeDev &sd, hipStream_t s) {
+#define SDOT_STEP(D, GG, GGS)                                    \
+  if (dim == D && G == GG && Gs == GGS) {                        \
+    step_t<D, GG, GGS>(sd, s);                                   \
+    return hipGetLastError();                                    \
+  }
+  SDOT_SERVE_GROUPINGS(SDOT_STEP, 2)
+  SDOT_SERVE_GROUPINGS(SDOT_STEP, 3)
+#undef SDOT_STEP
+  return hipErrorInvalidValue;
 }
 
 }  // namespace sdot

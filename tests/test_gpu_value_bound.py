@@ -179,3 +179,42 @@ def test_default_bound_where_it_pays():
     assert not np.array_equal(g1, g0)  # the default did bound this evaluation
     assert out[("fp64", -1)][0] == out[("fp64", 0)][0]
     assert np.array_equal(out[("fp64", -1)][1], out[("fp64", 0)][1])
+
+
+def test_served_bounded_request_mix_vs_alone_and_oracle():
+    """A served FP32 request that the default bounds (20,000 targets at eps 1e-3
+    against the 262,144 q-points of the 32^3 source: 5.2e9 pairs) beside one
+    it does not bound (3,000 targets): each result is bitwise the same alone
+    and in the mix, and within 1e-6 relative marginal error of the FP64
+    oracle at its potential."""
+    from semidiscreteot_amd import workloads
+    wl = workloads.gmm_3d()
+    s = sd.SotSolver(3)
+    s.setup_source(wl.source.x, wl.source.w, wl.source.centers, wl.source.nq)
+    rng = np.random.default_rng(36)
+    reqs = []
+    for n in (20000, 3000):
+        y = workloads.gmm_points(rng, n, 3, 5)
+        nu = rng.dirichlet(np.full(n, 2.0))
+        reqs.append((y.ravel(), nu / nu.sum()))
+    kw = dict(control="componentwise", max_iterations=1000, preconditioner="diagonal")
+
+    def serve(rs):
+        with sd.SotServer(s, precision="fp32_lse", max_slots=8, hbm_budget_bytes=8 << 30) as srv:
+            ids = [srv.submit(y, nu, EPS, 1e-6, **kw) for y, nu in rs]
+            out = srv.run(ids)
+            return [out[i] for i in ids]
+
+    mix = serve(reqs)
+    alone = [serve([r])[0] for r in reqs]
+    src = O.Source(3, wl.source.nq, wl.source.x, wl.source.w, np.ones_like(wl.source.w),
+                   wl.source.centers)
+    for (y, nu), (rm, pm), (ra, pa) in zip(reqs, mix, alone):
+        _, go, _ = O.evaluate_R(src, O.Target(y, nu), O.params(epsilon=EPS, use_lse=1), DMAX, pm)
+        err = float(np.max(np.abs(go) / nu))
+        say(f"served N {nu.size}: {rm.status} after {rm.iterations} iterations, oracle relerr "
+            f"{err:.3e}, bitwise alone/mix {np.array_equal(pm, pa)}")
+        assert rm.status == ra.status == "converged"
+        assert np.array_equal(pm, pa) and rm.iterations == ra.iterations
+        assert err <= 1e-6
+    s.close()
