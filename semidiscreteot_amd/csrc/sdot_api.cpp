@@ -734,7 +734,7 @@ extern "C" int sdot_ctx_destroy(sdot_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   Dev &d = c->dev;
-  void *bufs[] = {d.qx_m, d.qxr_m, d.qw_m, d.qx2_m, d.cc_m, d.blkbox, d.blkbox_s, d.cc_r, d.cstart, d.cell_r2m,
+  void *bufs[] = {d.qx_m, d.qxr_m, d.qw_m, d.qx2_m, d.cc_m, d.blkbox, d.blkbox_s, d.qbox_s, d.cc_r, d.cstart, d.cell_r2m,
                   d.cell_m2o, d.ty_o, d.tnu_o, d.tpos_r, d.tpos_m, d.tr2o, d.tm2o, d.tstart,
                   d.twbox, d.ty_m, d.trec_r, d.trec_m, d.qrec_r, d.Mpart, d.Spart, d.Gpart,
                   d.Fpart, d.Cpart, d.PNpart, d.psi, d.grad, d.out, d.scratch, d.i64scratch, d.Tpart,
@@ -874,6 +874,29 @@ extern "C" int sdot_set_source(sdot_ctx *c, int dim, int64_t n_cells, int nq, co
     return box;
   };
   const std::vector<double> box = boxes(p.grp_big), box_s = boxes(p.grp_small);
+  // AABB of the quadrature points of each block of the small grouping (value
+  // bound of the bounded sweeps, which run that grouping)
+  std::vector<double> qbox_s((size_t)p.grp_small.n_blocks * 2 * dim);
+  {
+    const Grouping &gr = p.grp_small;
+    for (int64_t b = 0; b < gr.n_blocks; ++b) {
+      double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      const int64_t g0 = b * kWave, g1 = std::min<int64_t>(gr.n_groups, g0 + kWave);
+      for (int64_t g = g0; g < g1; ++g) {
+        const int64_t cm = g / gr.gpc;
+        const int64_t q0 = cm * nq + (g - cm * gr.gpc) * gr.G;
+        for (int k = 0; k < gr.G; ++k)
+          for (int d = 0; d < dim; ++d) {
+            lo[d] = std::min(lo[d], qxr[(q0 + k) * dim + d]);
+            hi[d] = std::max(hi[d], qxr[(q0 + k) * dim + d]);
+          }
+      }
+      for (int d = 0; d < dim; ++d) {
+        qbox_s[b * 2 * dim + d] = lo[d];
+        qbox_s[b * 2 * dim + dim + d] = hi[d];
+      }
+    }
+  }
   // largest distance of a cell's quadrature points from its center (value
   // bound of the target-major pass), with a margin for its rounding
   double rq2 = 0.0;
@@ -911,6 +934,7 @@ extern "C" int sdot_set_source(sdot_ctx *c, int dim, int64_t n_cells, int nq, co
   HIP_TRY(upload(dv.cc_m, ccm));
   HIP_TRY(upload(dv.blkbox, box));
   HIP_TRY(upload(dv.blkbox_s, box_s));
+  HIP_TRY(upload(dv.qbox_s, qbox_s));
   HIP_TRY(upload(dv.cc_r, ccr));
   HIP_TRY(upload(dv.cstart, cstart));
   HIP_TRY(upload(dv.cell_r2m, r2m));
@@ -1759,6 +1783,7 @@ int sdot::ctx_source_view(sdot_ctx *c, SourceView &v) {
   v.n_blocks_s = p.grp_small.n_blocks;
   v.n_cbins = p.n_cbins;
   v.blkbox_s = d.blkbox_s;
+  v.qbox_s = d.qbox_s;
   v.rq = p.rq;
   return SDOT_OK;
 }

@@ -230,8 +230,8 @@ __device__ __forceinline__ void for_rows_vb(const Grid &g, const int32_t *__rest
       const double yl = g.o[1] + by * g.h, yh = yl + g.h;
       const int row = bz * g.n[1] + by;
       // value bound of the row: the x-interval of bins it can admit
-      const double vd2 = gap(vb.lo[1], vb.hi[1], yl, yh) * gap(vb.lo[1], vb.hi[1], yl, yh) +
-                         vdz * vdz;
+      const double vd2 = wave_uniform(gap(vb.lo[1], vb.hi[1], yl, yh) *
+                                      gap(vb.lo[1], vb.hi[1], yl, yh) + vdz * vdz);
       const double ar = arow ? arow[row] : vb.auni;
       const double br = brow ? brow[row] : vb.buni;
       const double slack = ar - br - 0.5 * vd2;
@@ -612,21 +612,12 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
   // a.prow.
   VBound vb;
   auto set_bound = [&](const double (&L)[G]) {
-    double v = INFINITY, blo[DIM], bhi[DIM];
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-      blo[d] = INFINITY;
-      bhi[d] = -INFINITY;
-    }
+    double v = INFINITY;
 #pragma unroll
     for (int k = 0; k < G; ++k) {
       double x2 = 0.0;
 #pragma unroll
-      for (int d = 0; d < DIM; ++d) {
-        x2 = fma(x[k][d], x[k][d], x2);
-        blo[d] = fmin(blo[d], x[k][d]);
-        bhi[d] = fmax(bhi[d], x[k][d]);
-      }
+      for (int d = 0; d < DIM; ++d) x2 = fma(x[k][d], x[k][d], x2);
       // a q-point without candidates in this segment needs nothing from it
       const double vk = (L[k] > -INFINITY)
                             ? (L[k] - a.theta) * a.inv_coef + a.vshift - 0.5 * x2
@@ -634,13 +625,16 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
       v = fmin(v, vk);
     }
     if (!active) v = INFINITY;
-    // rounding of the bound: a relative margin far below 2^-theta
+    // rounding of the bound: a relative margin far below 2^-theta; the wave's
+    // values are kept in scalar registers (the sweep's VGPRs bound its occupancy)
     v = wave_min(v);
-    vb.buni = v - 1e-12 * (1.0 + fabs(v));
+    vb.buni = wave_uniform(v - 1e-12 * (1.0 + fabs(v)));
+    // the box of the block's q-points (host-computed, raw coordinates)
+    cdouble *qb = cptr(a.qbox) + blk * 2 * DIM;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-      vb.lo[d] = wave_min(active ? blo[d] : INFINITY) + a.o[d];
-      vb.hi[d] = wave_max(active ? bhi[d] : -INFINITY) + a.o[d];
+      vb.lo[d] = qb[d];
+      vb.hi[d] = qb[DIM + d];
     }
     vb.abin = a.pbin;
     vb.arow = a.prow;

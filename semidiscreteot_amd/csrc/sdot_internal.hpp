@@ -83,6 +83,7 @@ struct Dev {
   double *cc_m = nullptr;      // [n_cells*dim] raw centers, Morton order
   double *blkbox = nullptr;    // [n_blocks*2*dim] AABB of the block's cell centers
   double *blkbox_s = nullptr;  // the same for the small grouping
+  double *qbox_s = nullptr;    // [n_blocks_s*2*dim] AABB of the block's q-points, small grouping
   double *cc_r = nullptr;      // [n_cells*4] raw centers, row (bin) order
   int32_t *cstart = nullptr;   // [n_cbins+1] row-order cell offsets per bin
   int64_t *cell_r2m = nullptr; // [n_cells] row-order cell -> Morton cell
@@ -211,6 +212,7 @@ struct P1Args {
   double seed_bias;       // added to every seeded reference value (0; a test hook that
                           // exercises the re-sweep of an item whose seed is too high)
   // value bound of the sums (VBound, sdot_device.hpp): off when prune == 0
+  const double *qbox;     // [n_blocks*2*DIM] the q-points' box per block (raw coordinates)
   const double *pbin;     // [n_tbins] largest psi + eps ln nu per target bin
   const double *prow;     // [n_trows] ... per bin row
   double o[3];            // shift origin (raw = shifted + o)
@@ -324,7 +326,7 @@ struct SourceView {
   // the 4-point grouping of the value-bounded sweeps, and the bound's cell data
   int Gs = 1, gpc_s = 1;
   int64_t n_groups_s = 0, n_blocks_s = 0, n_cbins = 0;
-  const double *blkbox_s = nullptr;
+  const double *blkbox_s = nullptr, *qbox_s = nullptr;
   double rq = 0.0;
 };
 // Value bound of the pair sums: FP32_LSE evaluations are bounded at this many

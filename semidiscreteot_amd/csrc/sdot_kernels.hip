@@ -573,6 +573,7 @@ static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_g
   a.Sout = d.Spart;
   a.Tout = d.Tpart;
   a.prune = prune ? 1 : 0;
+  a.qbox = d.qbox_s;
   a.pbin = d.pbin;
   a.prow = d.prow;
   for (int k = 0; k < 3; ++k) a.o[k] = p.origin[k];

@@ -380,6 +380,7 @@ void build_desc(sdot_server *s, Solve &v) {
     a.n_groups = src.n_groups_s;
     a.gpc = src.gpc_s;
     a.prune = 1;
+    a.qbox = src.qbox_s;
     a.pbin = pbin;
     a.prow = prow;
     for (int k = 0; k < 3; ++k) a.o[k] = src.origin[k];
