@@ -97,7 +97,7 @@ typedef struct {
    * (every candidate pair, as the reference sums them); FP32_LSE 50 bits,
    * below the FP32 exponentials' own rounding, where it pays (at least 2^32
    * pairs, and sqrt(100 ln2 epsilon) at most a fifth of the diagonal of the
-   * source and target boxes), else off.  0: off.  Plain (non-LSE) mode and
+   * source box and half the truncation radius), else off.  0: off.  Plain (non-LSE) mode and
    * targets with a zero weight turn it off. */
   double prune_bits;              /* -1 */
 } sdot_params;

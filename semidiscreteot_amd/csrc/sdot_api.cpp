@@ -289,7 +289,11 @@ int plan_eval(sdot_ctx *c, double R) {
   // 7.5 -> 5.7 ms dense; 20,000 targets, eps 3e-3 (radius 0.26 of the
   // diagonal): 3.2 -> 3.4 ms; 4,096 targets, eps 1e-3: 0.87 -> 0.94 ms.
   if (c->params.prune_bits < 0) {
-    p.prune_bits = (p.mode == kF32Lse && value_bound_default(p.eps, diag, R, p.N, p.nQ))
+    double sdiag2 = 0.0;
+    for (int d = 0; d < p.dim; ++d)
+      sdiag2 += (c->src_hi[d] - c->src_lo[d]) * (c->src_hi[d] - c->src_lo[d]);
+    p.prune_bits = (p.mode == kF32Lse &&
+                    value_bound_default(p.eps, std::sqrt(sdiag2), R, p.N, p.nQ))
                        ? kPruneBitsF32 : 0.0;
   } else {
     p.prune_bits = c->params.prune_bits;

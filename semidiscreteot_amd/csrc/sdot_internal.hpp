@@ -335,13 +335,14 @@ struct SourceView {
 // exponentials.
 constexpr double kPruneBitsF32 = 50.0;
 // Whether the FP32_LSE default bounds an evaluation of N targets against nQ
-// quadrature points at epsilon and truncation radius R, with domain diagonal
-// diag: the radius sqrt(2 bits ln2 eps) beyond which a target's terms fall
-// below 2^-bits (at equal potentials) is at most a fifth of the diagonal and
-// half of R, and there are at least 2^32 pairs.  Where it is close to R the
-// bound skips little and its sweeps (fewer waves per SIMD) cost more: the
-// 500,000-target ball workload (R = 0.205, radius 0.186) solved in 19.9 s
-// bounded against 17.9 s.
+// quadrature points at epsilon and truncation radius R, with source-box
+// diagonal diag: the radius sqrt(2 bits ln2 eps) beyond which a target's terms
+// fall below 2^-bits (at equal potentials) is at most a fifth of the diagonal
+// and half of R, and there are at least 2^32 pairs.  Where the radius is close
+// to R or to the source's extent the bound skips little and its sweeps (4-point
+// lanes, fewer waves per SIMD) cost more: the 500,000-target ball workload
+// (R = 0.205, radius 0.186) solved in 19.9 s bounded against 17.9 s, and served
+// trace requests at eps 3e-3 (radius 0.46 in the unit cube) took longer.
 inline bool value_bound_default(double eps, double diag, double R, int64_t N, int64_t nQ) {
   const double r0 = std::sqrt(2.0 * kPruneBitsF32 * 0.69314718055994530942 * eps);
   return r0 <= 0.2 * diag && !(r0 > 0.5 * R) && (double)N * (double)nQ >= 4294967296.0;

@@ -190,7 +190,7 @@ struct Request {
   SolveShape shape;
   int dense = 0;
   double smax = 0.0, Rq = 0.0, s_in = 0.0;
-  double diag = 0.0, nu_min = 0.0;  // domain diagonal, smallest weight (value bound)
+  double diag = 0.0, nu_min = 0.0;  // source-box diagonal, smallest weight (value bound)
   size_t bytes = 0;
   double t_submit = 0.0;
 };
@@ -274,7 +274,9 @@ int fill_shape(sdot_server *s, Request &r) {
     diag2 += (hi - lo) * (hi - lo);
   }
   const double diag = std::sqrt(diag2);
-  r.diag = diag;
+  double sdiag2 = 0.0;
+  for (int d = 0; d < src.dim; ++d) sdiag2 += (src.hi[d] - src.lo[d]) * (src.hi[d] - src.lo[d]);
+  r.diag = std::sqrt(sdiag2);  // the value bound's default rule takes the source's extent
   r.nu_min = r.nu.empty() ? 0.0 : *std::min_element(r.nu.begin(), r.nu.end());
   const double R = r.R;
   r.smax = smax_for(R);
