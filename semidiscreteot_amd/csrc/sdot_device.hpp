@@ -468,13 +468,18 @@ __device__ __forceinline__ void chunkext_body(const double *dpsi_r, int64_t N, d
 // Reductions of the value bound (see VBound).  Over the records / cells of bin
 // b (offsets start[b] .. start[b+1]): the largest psi~ of a target bin (empty:
 // -inf) or the smallest V of a cell bin (empty: +inf).
+// One wave per bin (clustered targets put hundreds of records in some bins):
+// the lanes stride over the bin's entries, then a wave reduction.
 template <bool MAX>
 __device__ __forceinline__ void bin_reduce_body(const double *v, const int32_t *start, int64_t nb,
                                                 double *out, int64_t b) {
-  if (b >= nb) return;
+  const int lane = threadIdx.x & (kWave - 1);
   double r = MAX ? -INFINITY : INFINITY;
-  for (int32_t i = start[b]; i < start[b + 1]; ++i) r = MAX ? fmax(r, v[i]) : fmin(r, v[i]);
-  out[b] = r;
+  if (b < nb)
+    for (int32_t i = start[b] + lane; i < start[b + 1]; i += kWave)
+      r = MAX ? fmax(r, v[i]) : fmin(r, v[i]);
+  r = MAX ? wave_max(r) : wave_min(r);
+  if (b < nb && lane == 0) out[b] = r;
 }
 
 // ... and over the nx bins of each row of the grid.

@@ -79,10 +79,11 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
 }
 
 // value bound: per-bin and per-row reductions, per-cell bounds
+// one wave per bin, four bins per workgroup
 template <bool MAX>
 __global__ __launch_bounds__(256) void k_binred(const double *v, const int32_t *start, int64_t nb,
                                                 double *out) {
-  bin_reduce_body<MAX>(v, start, nb, out, (int64_t)blockIdx.x * 256 + threadIdx.x);
+  bin_reduce_body<MAX>(v, start, nb, out, (int64_t)blockIdx.x * 4 + threadIdx.x / kWave);
 }
 
 template <bool MAX>
@@ -531,8 +532,8 @@ static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_g
   const double theta = prune ? p.prune_bits * ((MODE == kF32Lse) ? 1.0 : 0.69314718055994530942) : 0.0;
   if (prune) {
     const int64_t ntr = grid_rows(p.tg);
-    hipLaunchKernelGGL(k_binred<true>, dim3(nblk256(p.n_tbins)), dim3(256), 0, s, d.psit_r,
-                       d.tstart, p.n_tbins, d.pbin);
+    hipLaunchKernelGGL(k_binred<true>, dim3((unsigned)((p.n_tbins + 3) / 4)), dim3(256), 0, s,
+                       d.psit_r, d.tstart, p.n_tbins, d.pbin);
     hipLaunchKernelGGL(k_rowred<true>, dim3(nblk256(ntr)), dim3(256), 0, s, d.pbin, ntr,
                        p.tg.n[0], d.prow);
   }
@@ -620,8 +621,8 @@ static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_g
     const int64_t ncr = grid_rows(p.cg);
     hipLaunchKernelGGL((k_cellv<DIM>), dim3(nblk256(p.n_cells)), dim3(256), 0, s, d.qrec_r,
                        p.n_cells, p.nq, theta, 1.0 / coef, MODE == kF32Lse ? 1 : 0, d.vcell);
-    hipLaunchKernelGGL(k_binred<false>, dim3(nblk256(p.n_cbins)), dim3(256), 0, s, d.vcell,
-                       d.cstart, p.n_cbins, d.vbin);
+    hipLaunchKernelGGL(k_binred<false>, dim3((unsigned)((p.n_cbins + 3) / 4)), dim3(256), 0, s,
+                       d.vcell, d.cstart, p.n_cbins, d.vbin);
     hipLaunchKernelGGL(k_rowred<false>, dim3(nblk256(ncr)), dim3(256), 0, s, d.vbin, ncr,
                        p.cg.n[0], d.vrow);
   }

@@ -488,10 +488,10 @@ void build_desc(sdot_server *s, Solve &v) {
   v.items[fp32 ? (bound ? kLGrad32V : kLGrad32) : kLGrad64] = sh.n_twaves * sh.K2;
   v.items[kLGfin] = (r.N + 255) / 256;
   if (bound) {
-    v.items[kLTbin] = (sh.ntb + 255) / 256;
+    v.items[kLTbin] = (sh.ntb + 3) / 4;
     v.items[kLTrow] = (sh.ntr + 255) / 256;
     v.items[kLCell] = (src.n_cells + 255) / 256;
-    v.items[kLCbin] = (src.n_cbins + 255) / 256;
+    v.items[kLCbin] = (src.n_cbins + 3) / 4;
     v.items[kLCrow] = (vb.ncr + 255) / 256;
   }
 }

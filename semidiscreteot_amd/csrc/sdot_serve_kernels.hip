@@ -85,8 +85,8 @@ __global__ __launch_bounds__(64) void kl_pass1(const SlotDesc *__restrict__ desc
   });
 }
 
-// Value-bound reductions (one 256-entry block per item): target bins and rows
-// after kl_prep, cells, cell bins and rows after kl_qfin.
+// Value-bound reductions: target bins and rows after kl_prep, cells, cell
+// bins and rows after kl_qfin.  An item is 4 bins (one wave each) or 256 rows.
 template <int K>
 __global__ __launch_bounds__(256) void kl_vbred(const SlotDesc *__restrict__ desc,
                                                 const int64_t *__restrict__ base,
@@ -97,9 +97,10 @@ __global__ __launch_bounds__(256) void kl_vbred(const SlotDesc *__restrict__ des
     const int s = find_item(base, n, it);
     const VBArgs &v = desc[s].vb;
     const int64_t i = (it - base[s]) * 256 + threadIdx.x;
-    if constexpr (K == kLTbin) bin_reduce_body<true>(v.psit_r, v.tstart, v.ntb, v.pbin, i);
+    const int64_t b = (it - base[s]) * 4 + threadIdx.x / kWave;
+    if constexpr (K == kLTbin) bin_reduce_body<true>(v.psit_r, v.tstart, v.ntb, v.pbin, b);
     if constexpr (K == kLTrow) row_reduce_body<true>(v.pbin, v.ntr, v.tnx, v.prow, i);
-    if constexpr (K == kLCbin) bin_reduce_body<false>(v.vcell, v.cstart, v.ncb, v.vbin, i);
+    if constexpr (K == kLCbin) bin_reduce_body<false>(v.vcell, v.cstart, v.ncb, v.vbin, b);
     if constexpr (K == kLCrow) row_reduce_body<false>(v.vbin, v.ncr, v.cnx, v.vrow, i);
   }
 }
