@@ -85,6 +85,11 @@ def test_prm_reader():
     assert p.epsilon_scaling_steps == 3 and p.precision == "fp32_lse"
     assert p.line_search == "noise_aware" and p.preconditioner == "diagonal"
     assert sd.SolverParameters.from_prm("").preconditioner == "none"
+    # the value bound: -1 (the precision's default) unless set
+    assert p.prune_bits == -1.0 and sd.SolverParameters().prune_bits == -1.0
+    q = sd.SolverParameters.from_prm("subsection rsot_solver\n set prune_bits = 64\nend\n")
+    assert q.prune_bits == 64.0
+    assert sd.SolverParameters.from_c(q.to_c()).prune_bits == 64.0
 
 
 def test_prm_reader_errors():
@@ -96,6 +101,9 @@ def test_prm_reader_errors():
         sd.SolverParameters.from_prm("subsection rsot_solver\n set epsilon = 1\n")
     with pytest.raises(sd.SdotError):
         sd.SolverParameters.from_prm("subsection rsot_solver\n set preconditioner = jacobi\nend\n")
+    for bad in ("many", "", "inf", "nan", "1e400", "12 bits"):
+        with pytest.raises(sd.SdotError):
+            sd.SolverParameters.from_prm(f"subsection rsot_solver\n set prune_bits = {bad}\nend\n")
     # unknown threshold names fall back to pointwise as in SotSolver.cc:406
     p = sd.SolverParameters.from_prm("subsection rsot_solver\n set distance_threshold_type = foo\nend\n")
     assert p.distance_threshold_type == "pointwise"
