@@ -218,3 +218,24 @@ def test_served_bounded_request_mix_vs_alone_and_oracle():
         assert np.array_equal(pm, pa) and rm.iterations == ra.iterations
         assert err <= 1e-6
     s.close()
+
+
+@pytest.mark.parametrize("precision, bits", [("fp64", 100.0), ("fp32_lse", 50.0)])
+@pytest.mark.parametrize("R", [0.45, DMAX], ids=["truncated", "dense"])
+def test_bounded_barycenters_vs_oracle(precision, bits, R):
+    """The Lloyd step's weighted barycenters run the bounded gradient sweep with
+    the first moments: FP64 at 100 bits within 1e-10 of the oracle, FP32 with
+    the 50-bit bound within the FP32 barycenter bar (2e-6)."""
+    src, tgt = case(N=1500, seed=37)
+    rng = np.random.default_rng(7)
+    psi = 0.01 * rng.standard_normal(tgt.n)
+    s = solver(src, tgt, precision, bits)
+    b = s.compute_weighted_barycenters_R(R, psi)
+    s.close()
+    bo = O.barycenters_R(src, tgt, O.params(epsilon=EPS, use_lse=1), R, psi)
+    err = float(np.max(np.abs(b - bo)))
+    say(f"barycenters {precision} {bits:g} bits R={R:g}: max |b - b_oracle| {err:.3e}")
+    if precision == "fp64":
+        assert err <= 1e-10 * max(1.0, float(np.max(np.abs(bo))))
+    else:
+        assert err <= 2e-6
