@@ -616,7 +616,7 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
   // of the wave and the box of its points; the bins' largest psi~ are a.pbin /
   // a.prow.
   VBound vb;
-  auto set_bound = [&](const double (&L)[G]) {
+  auto set_bound = [&](const double (&L)[G], double theta) {
     double v = INFINITY;
 #pragma unroll
     for (int k = 0; k < G; ++k) {
@@ -625,7 +625,7 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
       for (int d = 0; d < DIM; ++d) x2 = fma(x[k][d], x[k][d], x2);
       // a q-point without candidates in this segment needs nothing from it
       const double vk = (L[k] > -INFINITY)
-                            ? (L[k] - a.theta) * a.inv_coef + a.vshift - 0.5 * x2
+                            ? (L[k] - theta) * a.inv_coef + a.vshift - 0.5 * x2
                             : INFINITY;
       v = fmin(v, vk);
     }
@@ -673,7 +673,8 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
       sweep(runs);
   };
   // exact maximum over the item's candidates (every candidate: no value bound)
-  auto max_loop = [&]() {
+  // (bounded: only the bins the current bound admits)
+  auto max_loop = [&](bool bounded) {
     records(
         [&](const TRec<DIM> &r, bool tested) {
           if (tested && !in_set<DIM>(r.y, c, a.smax)) return;
@@ -685,7 +686,7 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
             T[k] = vmax64(T[k], t);
           }
         },
-        false);
+        bounded);
   };
 
   double L[G];  // lower bounds of the largest exponents (value bound)
@@ -730,7 +731,19 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
         L[k] = tp + dmn;
       }
     } else {
-      max_loop();
+      bool bounded = false;
+      if constexpr (vbound) {
+        if (a.hist) {
+          // T_prev + dmn is a lower bound of the new maximum: a bin whose
+          // exponents all lie below it cannot hold the maximum (the bound at
+          // 0 bits, the weights' spread included)
+#pragma unroll
+          for (int k = 0; k < G; ++k) L[k] = a.Tout[(int64_t)seg * a.nQ + q0 + k] + dmn;
+          set_bound(L, 0.0);
+          bounded = true;
+        }
+      }
+      max_loop(bounded);
 #pragma unroll
       for (int k = 0; k < G; ++k) {
         M[k] = ceil(T[k]);
@@ -739,7 +752,7 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
       }
     }
   }
-  if constexpr (vbound) set_bound(L);
+  if constexpr (vbound) set_bound(L, a.theta);
   records(one, true);
   if (PASS == 2 && seedq) {
     // The seeded bound holds by construction: the new maximum lies less than
@@ -754,7 +767,7 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
     if (__ballot(low) != 0) {
 #pragma unroll
       for (int k = 0; k < G; ++k) T[k] = -INFINITY;
-      max_loop();
+      max_loop(false);
 #pragma unroll
       for (int k = 0; k < G; ++k) {
         M[k] = ceil(T[k]);
@@ -762,7 +775,7 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
         S[k] = 0.0;
         T[k] = -INFINITY;
       }
-      if constexpr (vbound) set_bound(L);
+      if constexpr (vbound) set_bound(L, a.theta);
       records(one, true);
     }
   }
