@@ -396,7 +396,70 @@ struct TRec {
     }
     nu = v[1 + 2 * DIM];
   }
+  // the fields of the exponent and the weight only (a, yl, nu: y is left
+  // unset), each load of them wholly used: a record load with unused parts
+  // lets the next record's address arithmetic reuse those registers, which
+  // makes it wait for the load
+  __device__ __forceinline__ void load_core(cdouble *r) {
+    if constexpr (DIM == 3) {
+      typedef double d4 __attribute__((ext_vector_type(4)));
+      const d4 v = *reinterpret_cast<const __attribute__((address_space(4))) d4 *>(r);
+      a = v[0];
+      yl[0] = v[1];
+      yl[1] = v[2];
+      yl[2] = v[3];
+    } else {
+      typedef double d2 __attribute__((ext_vector_type(2)));
+      const d2 v = *reinterpret_cast<const __attribute__((address_space(4))) d2 *>(r);
+      a = v[0];
+      yl[0] = v[1];
+      yl[1] = r[2];
+    }
+    nu = r[1 + 2 * DIM];
+  }
 };
+
+// Records streamed two at a time: both records' scalar loads are issued
+// together and waited for once, which halves the load round trips a wave
+// waits out per record (a load issued one record ahead is sunk by the
+// compiler to its use, and scalar loads complete out of order, so any wait
+// covers every load in flight).  The lanes of a record: every lane
+// (TESTED false) or the lanes whose point set c holds it (the candidate test,
+// for both records before either's arithmetic, so that no load is sunk below
+// it).
+template <int DIM, bool TESTED, class Body>
+__device__ __forceinline__ void stream_records(cdouble *trec, int j0, int j1, const double *c,
+                                               double smax, Body &&body) {
+  auto load = [&](TRec<DIM> &r, int j) {
+    if constexpr (TESTED)
+      r.load_whole(trec + (int64_t)j * kRec);
+    else
+      r.load_core(trec + (int64_t)j * kRec);
+  };
+  int j = j0;
+  for (; j + 1 < j1; j += 2) {
+    TRec<DIM> A, B;
+    load(A, j);
+    load(B, j + 1);
+    // both loads before the first use (the scheduler would delay the second
+    // past the wait for the first)
+    __builtin_amdgcn_sched_barrier(0);
+    if constexpr (TESTED) {
+      const uint64_t ma = __ballot(in_set<DIM>(A.y, c, smax));
+      const uint64_t mb = __ballot(in_set<DIM>(B.y, c, smax));
+      if (__builtin_amdgcn_inverse_ballot_w64(ma)) body(A);
+      if (__builtin_amdgcn_inverse_ballot_w64(mb)) body(B);
+    } else {
+      body(A);
+      body(B);
+    }
+  }
+  if (j < j1) {
+    TRec<DIM> A;
+    load(A, j);
+    if (!TESTED || in_set<DIM>(A.y, c, smax)) body(A);
+  }
+}
 
 // ---------------------------------------------------------------------------
 // k_prep: target records
@@ -524,7 +587,7 @@ __device__ __forceinline__ void cellv_body(const double *qrec_r, int64_t n_cells
 
 template <int DIM, int G, int MODE, int PASS, bool VB = false>
 __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg) {
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x % kWave;
   const int64_t grp = blk * kWave + lane;
   const bool active = grp < a.n_groups;
   const int64_t gg = active ? grp : 0;
@@ -559,8 +622,8 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
     lo[d] = a.blkbox[blk * 2 * DIM + d];
     hi[d] = a.blkbox[blk * 2 * DIM + DIM + d];
   }
-  auto one = [&](const TRec<DIM> &r, bool tested) {
-    if (tested && !in_set<DIM>(r.y, c, a.smax)) return;
+  // one candidate record on the lanes that hold it (records() applies the test)
+  auto one = [&](const TRec<DIM> &r) {
     if (PASS == 0) {
 #pragma unroll
       for (int k = 0; k < G; ++k) {
@@ -656,16 +719,7 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
   cdouble *trec = cptr(a.trec);
   auto records = [&](auto &&fn, bool bounded) {
     auto runs = [&](int j0, int j1, auto tag) {
-      if (j0 >= j1) return;
-      // each record is fetched one record ahead of its use
-      TRec<DIM> cur;
-      cur.load_whole(trec + (int64_t)j0 * kRec);
-      for (int j = j0; j < j1; ++j) {
-        TRec<DIM> nxt;
-        nxt.load_whole(trec + (int64_t)min(j + 1, j1 - 1) * kRec);
-        fn(cur, decltype(tag)::value);
-        cur = nxt;
-      }
+      stream_records<DIM, decltype(tag)::value>(trec, j0, j1, c, a.smax, fn);
     };
     if (bounded)
       sweep_v(runs);
@@ -676,8 +730,7 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
   // (bounded: only the bins the current bound admits)
   auto max_loop = [&](bool bounded) {
     records(
-        [&](const TRec<DIM> &r, bool tested) {
-          if (tested && !in_set<DIM>(r.y, c, a.smax)) return;
+        [&](const TRec<DIM> &r) {
 #pragma unroll
           for (int k = 0; k < G; ++k) {
             double t = r.a;

@@ -107,8 +107,11 @@ __global__ __launch_bounds__(256) void k_chunkext(const double *dpsi_r, int64_t 
   chunkext_body(dpsi_r, N, dcx, dcn, (int64_t)blockIdx.x * (256 / kWave) + threadIdx.x / kWave);
 }
 
+// At most 4 points per lane (truncated sweeps): 5 waves per SIMD, which the
+// unbounded sweeps reach without spills (their records stream two at a time).
 template <int DIM, int G, int MODE, int PASS, bool VB>
-__global__ __launch_bounds__(64) void k_pass1(P1Args a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((G <= 4 && !VB) ? 5 : 1)))
+void k_pass1(P1Args a) {
   pass1_body<DIM, G, MODE, PASS, VB>(a, blockIdx.x, blockIdx.y);
 }
 
