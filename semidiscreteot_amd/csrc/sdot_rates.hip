@@ -5,6 +5,8 @@
 //   mfma64  v_mfma_f64_16x16x4_f64 (the matrix-core exponent tiles)
 //   mix     two MFMA tiles, the VALU work of an FP32 sum sweep on 8 pairs
 //           per lane, and both together: whether the two units overlap
+//   op      v_add_f64, v_max_f64, the two FP64/FP32 conversions, v_fma_f32:
+//           SIMD cycles per wave instruction of every operation of the sweeps
 // Every kernel runs independent dependency chains per lane (enough to cover
 // the units' latencies), over a grid of 8 waves per SIMD, and stores its
 // chains' values so the work is not removed.  Prints one JSON object:
@@ -51,6 +53,37 @@ __global__ __launch_bounds__(256) void k_fma64(double *out, double seed) {
 #pragma unroll
   for (int c = 0; c < kChains; ++c) s += x[c];
   out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+// Issue cost of the other instructions of the sweeps' pair arithmetic, one
+// operation kind per kernel (8 chains per lane, as above): K = 0 v_add_f64,
+// 1 v_max_f64 + v_add_f64, 2 v_cvt_f32_f64 + v_cvt_f64_f32 + v_add_f64,
+// 3 v_fma_f32.
+template <int K>
+__global__ __launch_bounds__(256) void k_op(double *out, double s, double m, float sf) {
+  double x[kChains];
+  float f[kChains];
+#pragma unroll
+  for (int c = 0; c < kChains; ++c) {
+    x[c] = s * (c + 1) + 1e-9 * threadIdx.x;
+    f[c] = sf * (c + 1) + 1e-6f * threadIdx.x;
+  }
+  for (int it = 0; it < kIters; ++it)
+#pragma unroll
+    for (int c = 0; c < kChains; ++c) {
+      if (K == 0) x[c] = x[c] - s;
+      if (K == 1) {
+        double r;
+        asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(x[c]), "v"(m));
+        x[c] = r - s;
+      }
+      if (K == 2) x[c] = (double)(float)x[c] - s;
+      if (K == 3) f[c] = fmaf(f[c], sf, sf);
+    }
+  double r = 0.0;
+#pragma unroll
+  for (int c = 0; c < kChains; ++c) r += x[c] + f[c];
+  out[blockIdx.x * blockDim.x + threadIdx.x] = r;
 }
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -141,6 +174,11 @@ int main() {
   double t_mix[4] = {0, 0, 0, 0};
   for (int mode = 1; mode <= 3; ++mode)
     t_mix[mode] = time_ms([&] { hipLaunchKernelGGL(k_mix, dim3(blocks), dim3(threads), 0, 0, dd, 0.5, mode); });
+  double t_op[4];
+  t_op[0] = time_ms([&] { hipLaunchKernelGGL(k_op<0>, dim3(blocks), dim3(threads), 0, 0, dd, 0.5, 0.999999, 0.5f); });
+  t_op[1] = time_ms([&] { hipLaunchKernelGGL(k_op<1>, dim3(blocks), dim3(threads), 0, 0, dd, 0.5, 0.999999, 0.5f); });
+  t_op[2] = time_ms([&] { hipLaunchKernelGGL(k_op<2>, dim3(blocks), dim3(threads), 0, 0, dd, 0.5, 0.999999, 0.5f); });
+  t_op[3] = time_ms([&] { hipLaunchKernelGGL(k_op<3>, dim3(blocks), dim3(threads), 0, 0, dd, 0.5, 0.999999, 0.5f); });
   CK(hipGetLastError());
   const double exp_ops = lanes * kIters * kChains;          // lane-operations
   const double fma_ops = lanes * kIters * kChains;
@@ -150,16 +188,26 @@ int main() {
   const double cyc = clk_hz * cus;                           // CU cycles per second
   // SIMD cycles per k_mix step (4 SIMDs per CU)
   auto per_step = [&](double ms) { return cyc * 4.0 * (ms * 1e-3) / steps; };
+  // SIMD cycles per wave instruction (chain step) of the k_op kernels and of
+  // k_exp / k_fma64: cycles of all SIMDs over wave-level chain steps
+  const double wsteps = waves * kIters * kChains;
+  auto per_ins = [&](double ms) { return cyc * 4.0 * (ms * 1e-3) / wsteps; };
+  const double c_add = per_ins(t_op[0]);
+  const double c_max = per_ins(t_op[1]) - c_add;
+  const double c_cvt = 0.5 * (per_ins(t_op[2]) - c_add);
   std::printf(
       "{\"device\": \"%s\", \"cus\": %d, \"clock_mhz\": %.0f, "
       "\"exp_f32_per_s\": %.4e, \"exp_f32_per_cu_cycle\": %.2f, "
       "\"fma_f64_per_s\": %.4e, \"fma_f64_per_cu_cycle\": %.2f, "
       "\"mfma_f64_16x16x4_per_s\": %.4e, \"mfma_f64_cycles_per_simd\": %.1f, "
-      "\"coexec_simd_cycles_per_step\": {\"mfma_only\": %.1f, \"valu_only\": %.1f, \"both\": %.1f}}\n",
+      "\"coexec_simd_cycles_per_step\": {\"mfma_only\": %.1f, \"valu_only\": %.1f, \"both\": %.1f}, "
+      "\"simd_cycles_per_wave_instruction\": {\"v_fma_f64\": %.2f, \"v_add_f64\": %.2f, "
+      "\"v_max_f64\": %.2f, \"v_cvt_f32_f64_or_f64_f32\": %.2f, \"v_fma_f32\": %.2f, "
+      "\"v_exp_f32\": %.2f}}\n",
       prop.name, cus, clk_hz / 1e6, exp_ops / (t_exp * 1e-3), exp_ops / (t_exp * 1e-3) / cyc,
       fma_ops / (t_fma * 1e-3), fma_ops / (t_fma * 1e-3) / cyc, mfma_ins / (t_mfma * 1e-3),
       cyc * 4.0 / (mfma_ins / (t_mfma * 1e-3)), per_step(t_mix[1]), per_step(t_mix[2]),
-      per_step(t_mix[3]));
+      per_step(t_mix[3]), per_ins(t_fma), c_add, c_max, c_cvt, per_ins(t_op[3]), per_ins(t_exp));
   CK(hipFree(dd));
   CK(hipFree(df));
   return 0;
