@@ -370,6 +370,7 @@ int prepare(sdot_ctx *c, double R) {
   }
   if (c->cap_qrec < (size_t)p.nQ * kRec) {
     HIP_TRY(dalloc(d.qrec_r, (size_t)p.nQ * kRec));
+    HIP_TRY(dalloc(d.qchunk, (size_t)p.nQ * (p.dim + 2)));
     HIP_TRY(dalloc(d.Mglob, (size_t)p.nQ));
     c->cap_qrec = (size_t)p.nQ * kRec;
   }
@@ -454,6 +455,7 @@ size_t carve_eval(const Plan &p, const Dev &ctx, double *base, Dev &d) {
   take(d.trec_r, N * kRec);
   take(d.trec_m, N * kRec);
   take(d.qrec_r, nQ * kRec);
+  take(d.qchunk, nQ * (size_t)(p.dim + 2));
   take(d.Mpart, M);
   take(d.Spart, M);
   take(d.Tpart, M);
@@ -740,7 +742,7 @@ extern "C" int sdot_ctx_destroy(sdot_ctx *c) {
   Dev &d = c->dev;
   void *bufs[] = {d.qx_m, d.qxr_m, d.qw_m, d.qx2_m, d.cc_m, d.blkbox, d.blkbox_s, d.qbox_s, d.cc_r, d.cstart, d.cell_r2m,
                   d.cell_m2o, d.ty_o, d.tnu_o, d.tpos_r, d.tpos_m, d.tr2o, d.tm2o, d.tstart,
-                  d.twbox, d.ty_m, d.trec_r, d.trec_m, d.qrec_r, d.Mpart, d.Spart, d.Gpart,
+                  d.twbox, d.ty_m, d.trec_r, d.trec_m, d.qrec_r, d.qchunk, d.Mpart, d.Spart, d.Gpart,
                   d.Fpart, d.Cpart, d.PNpart, d.psi, d.grad, d.out, d.scratch, d.i64scratch, d.Tpart,
                   d.Upart, d.psi_prev, d.dpsi_r, d.dpsi_cx, d.dpsi_cn, d.ucount, d.Mglob,
                   d.psit_r, d.pbin, d.prow, d.vcell, d.vbin, d.vrow};

@@ -918,6 +918,14 @@ __device__ __forceinline__ void qfin_body(const QFArgs &a, int64_t blk, double *
     r[DIM + 2] = (scale > 0.0) ? S : 0.0;
 #pragma unroll
     for (int t = DIM + 3; t < kRec; ++t) r[t] = 0.0;
+    if (a.qchunk) {
+      const int64_t ch = qr / a.G, k = qr - ch * a.G;
+      double *cq = a.qchunk + ch * (DIM + 2) * a.G + k;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) cq[d * a.G] = x[d];
+      cq[DIM * a.G] = Mrec;
+      cq[(DIM + 1) * a.G] = scale;
+    }
   }
   const double fs = block_sum_256(Fq, sh);
   const double cs = block_sum_256(Cq, sh);
@@ -950,16 +958,18 @@ __device__ __forceinline__ void grad_body(const GArgs &a, int64_t blk, int seg) 
 #pragma unroll
   for (int d = 0; d < DIM; ++d) accx[d] = 0.0;
   const int nq = a.nq;
-  cdouble *qrec = cptr(a.qrec_r);
+  cdouble *qchk = cptr(a.qchunk);
   cdouble *ccr = cptr(a.cc_r);
+  // one chunk of G q-points (GArgs::qchunk layout: x'_d of the G points, then
+  // their M, then their w/S)
   auto chunk = [&](cdouble *qp) {
     double qx[G][DIM], qm[G], qs[G];
 #pragma unroll
     for (int k = 0; k < G; ++k) {
 #pragma unroll
-      for (int d = 0; d < DIM; ++d) qx[k][d] = qp[k * kRec + d];
-      qm[k] = qp[k * kRec + DIM];
-      qs[k] = qp[k * kRec + DIM + 1];
+      for (int d = 0; d < DIM; ++d) qx[k][d] = qp[d * G + k];
+      qm[k] = qp[DIM * G + k];
+      qs[k] = qp[(DIM + 1) * G + k];
     }
 #pragma unroll
     for (int k = 0; k < G; ++k) {
@@ -1016,8 +1026,8 @@ __device__ __forceinline__ void grad_body(const GArgs &a, int64_t blk, int seg) 
         for (int d = 0; d < DIM; ++d) cv[d] = cp[d];
         if (!in_set<DIM>(me.y, cv, a.smax)) continue;
       }
-      cdouble *qp = qrec + (int64_t)c * nq * kRec;
-      for (int k0 = 0; k0 < nq; k0 += G) chunk(qp + (int64_t)k0 * kRec);
+      cdouble *qp = qchk + (int64_t)c * nq * (DIM + 2);
+      for (int k0 = 0; k0 < nq; k0 += G) chunk(qp + (int64_t)k0 * (DIM + 2));
     }
   };
   auto cells = [&](auto &&f) {

@@ -102,6 +102,7 @@ struct Dev {
   double *trec_r = nullptr;    // [N*kRec] row order
   double *trec_m = nullptr;    // [N*kRec] Morton order
   double *qrec_r = nullptr;    // [nQ*kRec] row cell order
+  double *qchunk = nullptr;    // [nQ*(dim+2)] GArgs::qchunk
   double *Mpart = nullptr;     // [K1*nQ]
   double *Spart = nullptr;     // [K1*nQ]
   double *Mglob = nullptr;     // [nQ] FP64 LSE: the maximum over the segments
@@ -226,8 +227,10 @@ struct QFArgs {
   const double *qx, *qw, *qx2, *Mpart, *Spart, *Tpart;
   const int64_t *cell_r2m;
   double *qrec_r, *Fpart, *Cpart, *Upart;
+  double *qchunk;  // [nQ*(DIM+2)] the gradient sweep's copy of x', M, w/S (GArgs::qchunk)
   int64_t nQ;
   int nq, K1;
+  int G;           // its chunk size (q-points per chunk; divides nq)
   double eps, plain_coef, ln_scale;
 };
 
@@ -237,6 +240,11 @@ struct GArgs {
   const double *cc_r;    // [n_cells*4] raw centers, row order
   const int32_t *cstart;
   const double *qrec_r;  // [nQ*kRec]
+  // x', M and w/S of the q-points in chunks of G (row cell order), each chunk
+  // [x'_0 (G values) .. x'_DIM-1, M, w/S]: a chunk's (DIM+2) G doubles are
+  // contiguous, so a few wide scalar loads fetch it (the q records take two
+  // loads per q-point)
+  const double *qchunk;
   double *Gpart;         // [K2*N]
   double *Bpart;         // [K2*DIM*N] first moments sum_q (w/S) e x' (barycenters), or null
   Grid cg;

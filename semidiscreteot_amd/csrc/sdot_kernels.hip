@@ -610,6 +610,8 @@ static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_g
   q.Tpart = d.Tpart;
   q.cell_r2m = d.cell_r2m;
   q.qrec_r = d.qrec_r;
+  q.qchunk = d.qchunk;
+  q.G = p.G;
   q.Fpart = d.Fpart;
   q.Cpart = d.Cpart;
   q.Upart = d.Upart;
@@ -637,6 +639,7 @@ static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_g
   g.cc_r = d.cc_r;
   g.cstart = d.cstart;
   g.qrec_r = d.qrec_r;
+  g.qchunk = d.qchunk;
   g.Gpart = d.Gpart;
   g.cg = p.cg;
   g.N = p.N;

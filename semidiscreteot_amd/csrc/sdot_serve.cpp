@@ -124,7 +124,7 @@ struct SolveShape {
 
 struct SolveOffsets {
   size_t y, nu, pos_r, pos_m, tm2o, tstart, twbox;
-  size_t trec_r, trec_m, qrec_r, Mpart, Spart, Tpart, Gpart;
+  size_t trec_r, trec_m, qrec_r, qchunk, Mpart, Spart, Tpart, Gpart;
   size_t Fpart, Cpart, Upart, PNpart, psi_prev, dpsi;
   size_t psit_r, pbin, prow, vcell, vbin, vrow;  // value bound (FP32 solves)
   size_t x, g, x0, gv, p, S[kHist], Y[kHist], hist;
@@ -145,6 +145,7 @@ SolveOffsets carve(const SourceView &src, const SolveShape &s) {
   o.trec_r = c.take<double>((size_t)s.N * kRec);
   o.trec_m = c.take<double>((size_t)s.N * kRec);
   o.qrec_r = c.take<double>((size_t)src.nQ * kRec);
+  o.qchunk = c.take<double>((size_t)src.nQ * (src.dim + 2));
   o.Mpart = c.take<double>((size_t)s.K1 * src.nQ);
   o.Spart = c.take<double>((size_t)s.K1 * src.nQ);
   o.Tpart = s.fp32 ? c.take<double>((size_t)s.K1 * src.nQ) : SIZE_MAX;
@@ -401,6 +402,8 @@ void build_desc(sdot_server *s, Solve &v) {
   q.Tpart = dptr<double>(s, v, v.o.Tpart);
   q.cell_r2m = src.cell_r2m;
   q.qrec_r = qrec_r;
+  q.qchunk = dptr<double>(s, v, v.o.qchunk);
+  q.G = (fp32 && bound) ? src.Gs : src.G;  // the chunk size of this solve's kl_grad
   q.Fpart = Fpart;
   q.Cpart = Cpart;
   q.Upart = Upart;
@@ -418,6 +421,7 @@ void build_desc(sdot_server *s, Solve &v) {
   gr.cc_r = src.cc_r;
   gr.cstart = src.cstart;
   gr.qrec_r = qrec_r;
+  gr.qchunk = q.qchunk;
   gr.Gpart = Gpart;
   gr.cg = src.cg;
   gr.N = r.N;
