@@ -202,6 +202,7 @@ struct sdot_ctx {
   int force_mode = -1;  // overrides the arithmetic mode (the Hessian runs FP64 LSE)
   bool hist_valid = false;  // device holds psi and M_q of a previous FP32 evaluation
   double hist_R = 0.0;      // its truncation radius
+  int hist_G = 0, hist_small = -1, hist_K1 = 0, hist_dense = -1;  // and its work items
   int n_servers = 0;        // serving queues reading this context's source arrays
   double seed_bias = 0.0;   // test hooks (sdot_ctx_create)
   int force_K1 = 0, force_K2 = 0;
@@ -273,8 +274,6 @@ int plan_eval(sdot_ctx *c, double R) {
   p.s_in = inner_s(p.smax);
   p.dense = (R >= diag * (1.0 + 1e-6) + 1e-300) ? 1 : 0;
   p.Rq = R * (1.0 + 1e-9) + 1e-9 * (p.tg.h + p.cg.h + c->coord_scale);
-  // the seed needs the previous evaluation's candidate sets: same R
-  p.hist = (p.mode == kF32Lse && c->hist_valid && c->hist_R == R) ? 1 : 0;
   p.seed_bias = c->seed_bias;
   if (!std::isfinite(p.Rq)) p.dense = std::isnan(R) ? 0 : 1;
   if (std::isnan(R)) p.Rq = 0.0;
@@ -340,7 +339,32 @@ int plan_eval(sdot_ctx *c, double R) {
   if (c->force_K2 > 0) p.K2 = std::min(c->force_K2, kMaxSeg);
   p.nFpart = (p.nQ + 255) / 256;
   p.nPpart = (p.N + 255) / 256;
+  // The FP32 seed reads the previous evaluation's segment maxima: the same
+  // work items (grouping, q-major segments, dense or truncated).  Its
+  // candidates may differ with the radius: a larger one adds targets the seed
+  // does not bound (the sweep re-sweeps an item whose maximum rose above its
+  // seed), a smaller one removes some (re-swept when its maximum fell too far
+  // below; a point that had no candidates in the item has the seed -inf,
+  // below any new maximum).  The value bound takes the previous maxima as bounds of the
+  // new ones over the same candidates: bounded sweeps are seeded at the same
+  // radius only.
+  const bool same_items = c->hist_G == p.G && c->hist_small == p.small && c->hist_K1 == p.K1 &&
+                          c->hist_dense == p.dense;
+  const bool radius_ok = R == c->hist_R || !(p.prune_bits > 0);
+  p.hist = (p.mode == kF32Lse && c->hist_valid && same_items && radius_ok) ? 1 : 0;
   return SDOT_OK;
+}
+
+// Note that an FP32 evaluation of plan p at radius R left its potentials and
+// segment maxima behind (the next evaluation's seed).
+void note_history(sdot_ctx *c, double R) {
+  const Plan &p = c->plan;
+  c->hist_valid = true;
+  c->hist_R = R;
+  c->hist_G = p.G;
+  c->hist_small = p.small;
+  c->hist_K1 = p.K1;
+  c->hist_dense = p.dense;
 }
 
 // The plan at radius R with every per-evaluation buffer of the context large
@@ -433,10 +457,7 @@ int run_eval(sdot_ctx *c, double R, const double *d_psi, double *d_grad, double 
   }
   HIP_TRY(launch_eval(c->plan, c->dev, d_psi, d_grad, d_FC, c->stream, pp));
   // an FP32 evaluation leaves psi and M_q behind to seed the next one
-  if (c->plan.mode == kF32Lse) {
-    c->hist_valid = true;
-    c->hist_R = R;
-  }
+  if (c->plan.mode == kF32Lse) note_history(c, R);
   return SDOT_OK;
 }
 
@@ -574,10 +595,7 @@ int host_barycenters(sdot_ctx *c, double R, const double *psi, double *bary) {
     e = launch_eval_bary(c->plan, c->dev, c->dev.psi, c->dev.grad, c->dev.out, d_bary, d_B,
                          c->stream);
     if (e != hipSuccess) break;
-    if (c->plan.mode == kF32Lse) {
-      c->hist_valid = true;
-      c->hist_R = R;
-    }
+    if (c->plan.mode == kF32Lse) note_history(c, R);
     if (!seeded) break;
     // a seed far above the new maxima would have flushed the FP32 terms: redo unseeded
     double lost = 0.0;

@@ -364,8 +364,13 @@ __device__ __forceinline__ double block_max_256(double v, double *sh) {
 // raised by the largest exponent change among its targets: an upper bound of
 // the new maximum, at most (spread of the changes) + 1 above it.  The seed is
 // used while that spread, over the item's own targets, stays below
-// kSeedSpread log2 units.
+// kSeedSpread log2 units.  When the radius changed since that evaluation, the
+// item's candidates may have changed too: a target that joined can lie above
+// the seed, and an item whose maximum exceeds it by more than kSeedAbove (or
+// fell further than the spread allows below it) is swept again against its
+// exact maximum.
 constexpr double kSeedSpread = 2.0;
+constexpr double kSeedAbove = 0.5;
 
 // Target record (64 bytes):
 //   [0]=a  [1..DIM]=y'/eps  [1+DIM..2DIM]=y raw  [1+2DIM]=nu
@@ -808,15 +813,17 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
   if constexpr (vbound) set_bound(L, a.theta);
   records(one, true);
   if (PASS == 2 && seedq) {
-    // The seeded bound holds by construction: the new maximum lies less than
-    // kSeedSpread + 1 below it.  An item whose maximum fell further (history
-    // that does not belong to these potentials) is swept again against its
-    // exact maximum, so that its dominant terms keep their FP32 argument
-    // precision; the maximum comes from a sweep over every candidate, since
-    // the value bound of the first sweep rested on the history.
+    // With the same candidates the seeded bound holds by construction: the
+    // new maximum lies at most M and less than kSeedSpread + 1 below it.  An
+    // item whose maximum fell further (history that does not belong to these
+    // potentials) or rose above M (a target that joined at a larger radius) is
+    // swept again against its exact maximum, so that its dominant terms keep
+    // their FP32 argument precision; the maximum comes from a sweep over every
+    // candidate, since the value bound of the first sweep rested on the history.
     bool low = false;
 #pragma unroll
-    for (int k = 0; k < G; ++k) low |= T[k] < M[k] - (kSeedSpread + 2.0);
+    for (int k = 0; k < G; ++k)
+      low |= T[k] < M[k] - (kSeedSpread + 2.0) || T[k] > M[k] + kSeedAbove;
     if (__ballot(low) != 0) {
 #pragma unroll
       for (int k = 0; k < G; ++k) T[k] = -INFINITY;
