@@ -345,13 +345,15 @@ int plan_eval(sdot_ctx *c, double R) {
   // does not bound (the sweep re-sweeps an item whose maximum rose above its
   // seed), a smaller one removes some (re-swept when its maximum fell too far
   // below; a point that had no candidates in the item has the seed -inf,
-  // below any new maximum).  The value bound takes the previous maxima as bounds of the
-  // new ones over the same candidates: bounded sweeps are seeded at the same
-  // radius only.
+  // below any new maximum).  The value bound takes the previous maxima as
+  // lower bounds of the new ones, which holds while every previous candidate
+  // remains one: bounded sweeps are seeded at the same or a larger radius
+  // (and not in items with a point that had no candidate before).
   const bool same_items = c->hist_G == p.G && c->hist_small == p.small && c->hist_K1 == p.K1 &&
                           c->hist_dense == p.dense;
-  const bool radius_ok = R == c->hist_R || !(p.prune_bits > 0);
+  const bool radius_ok = R == c->hist_R || !(p.prune_bits > 0) || R > c->hist_R;
   p.hist = (p.mode == kF32Lse && c->hist_valid && same_items && radius_ok) ? 1 : 0;
+  p.hist_same = (p.hist && R == c->hist_R) ? 1 : 0;
   return SDOT_OK;
 }
 

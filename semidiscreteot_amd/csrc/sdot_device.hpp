@@ -779,6 +779,20 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
       }
       seed = dmx - dmn < kSeedSpread;
     }
+    // the previous maxima bound the new ones from below (value bound) when the
+    // item's candidates are the same or more: at a larger radius, unless a
+    // point had no candidate in the item before (no bound at all)
+    bool hist_lb = a.hist != 0;
+    if constexpr (vbound) {
+      if (a.hist && !a.hist_same) {
+        bool empty = false;
+#pragma unroll
+        for (int k = 0; k < G; ++k)
+          empty |= active && !(a.Tout[(int64_t)seg * a.nQ + q0 + k] > -INFINITY);
+        hist_lb = __ballot(empty) == 0;
+      }
+      seed = seed && hist_lb;
+    }
     seedq = seed;
     if (seed) {
       // the new maximum lies in [T_prev + dmn, T_prev + dmx]
@@ -791,7 +805,7 @@ __device__ __forceinline__ void pass1_body(const P1Args &a, int64_t blk, int seg
     } else {
       bool bounded = false;
       if constexpr (vbound) {
-        if (a.hist) {
+        if (hist_lb) {
           // T_prev + dmn is a lower bound of the new maximum: a bin whose
           // exponents all lie below it cannot hold the maximum (the bound at
           // 0 bits, the weights' spread included)

@@ -159,6 +159,7 @@ struct Plan {
   double Rq = 0.0;     // conservative enumeration radius
   double s_in = 0.0;    // inner runs: every pair closer than sqrt(s_in) passes the rule
   int hist = 0;        // FP32 mode: seed the online sweep with the previous evaluation
+  int hist_same = 0;   // ... which ran at this radius
   int dense = 0;       // every pair is a candidate (no enumeration, no test)
   double seed_bias = 0.0;  // test hook: raises the FP32 sweep's seeded reference values
   // value bound of the pair sums (VBound): 2^-prune_bits, 0 = off
@@ -208,6 +209,7 @@ struct P1Args {
   Grid tg;
   int64_t n_groups, nQ, N;
   int nq, gpc, K1, dense, hist;
+  int hist_same;          // the history's radius is this one (else a smaller one)
   double smax, Rq, plain_coef, coef;
   double s_in;            // squared radius of the untested inner runs (0: none)
   double seed_bias;       // added to every seeded reference value (0; a test hook that

@@ -567,6 +567,7 @@ static hipError_t eval_t(const Plan &p, Dev &d, const double *d_psi, double *d_g
   a.K1 = p.K1;
   a.dense = p.dense;
   a.hist = p.hist;
+  a.hist_same = p.hist_same;
   a.smax = p.smax;
   a.Rq = p.Rq;
   a.s_in = p.s_in;

@@ -372,6 +372,7 @@ void build_desc(sdot_server *s, Solve &v) {
   a.K1 = sh.K1;
   a.dense = r.dense;
   a.hist = v.seeded ? 1 : 0;
+  a.hist_same = 1;  // a solve's radius does not change
   a.smax = r.smax;
   a.Rq = r.Rq;
   a.s_in = r.s_in;

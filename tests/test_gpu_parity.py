@@ -148,21 +148,24 @@ def test_fp32_small_eps_error():
         psi = psi2
 
 
-def test_fp32_seed_across_radius_changes():
-    """Unbounded FP32 sweeps are seeded with the previous evaluation's maxima
-    when the radius changed: a larger radius adds candidates the seed does
-    not bound (their items are swept again, among them points that had no
-    candidate at all), a smaller one removes some.  Every evaluation of the
+@pytest.mark.parametrize("prune", [0.0, 50.0])
+def test_fp32_seed_across_radius_changes(prune):
+    """FP32 sweeps are seeded with the previous evaluation's maxima when the
+    radius changed: a larger radius adds candidates the seed does not bound
+    (their items are swept again, among them points that had no candidate at
+    all), a smaller one removes some (value-bounded sweeps: not seeded, since
+    their bound rests on the previous candidates).  Every evaluation of the
     sequence matches the oracle at the FP32 bar."""
-    say("fp32 seed across radii")
+    say(f"fp32 seed across radii, prune_bits {prune}")
     src, tgt = structured_case(3, 12, 2, 3000, seed=5, gmm_source=True, gmm_targets=True,
                                dirichlet=5)
     eps = 1e-3
     rng = np.random.default_rng(21)
     psi = 0.02 * rng.standard_normal(tgt.n)
     s = make(src, tgt, epsilon=eps, use_log_sum_exp_trick=True, precision="fp32_lse",
-             prune_bits=0.0)
-    for R, step in ((0.08, 0.0), (0.3, 2e-4), (0.2, 2e-4), (0.35, 3e-2), (0.1, 2e-4), (0.3, 0.0)):
+             prune_bits=prune)
+    for R, step in ((0.08, 0.0), (0.3, 2e-4), (0.2, 2e-4), (0.35, 3e-2), (0.1, 2e-4), (0.3, 0.0),
+                    (0.32, 2e-4), (0.34, 2e-4)):
         psi = psi + step * rng.standard_normal(tgt.n)
         F, g, C = s.evaluate_functional_R(R, psi)
         Fo, go, Co = O.evaluate_R(src, tgt, O.params(epsilon=eps, use_lse=1), R, psi)
