@@ -20,7 +20,7 @@ RATES = os.path.join(LIBDIR, "sdot_rates")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 HIP_SOURCES = ["sdot_kernels.hip", "sdot_serve_kernels.hip", "sdot_refine.hip", "sdot_points.hip",
-               "sdot_layout.hip"]
+               "sdot_layout.hip", "sdot_lbfgs.hip"]
 CXX_SOURCES = ["sdot_api.cpp", "sdot_mesh.cpp", "sdot_prm.cpp", "sdot_serve.cpp"]
 
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",

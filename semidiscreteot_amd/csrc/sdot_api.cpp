@@ -10,6 +10,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <string>
@@ -1512,6 +1513,83 @@ extern "C" int sdot_transport_map(sdot_ctx *c, int strategy, const double *psi, 
                      points, nullptr, 0, nullptr, mapped);
 }
 
+// Potentials at least this long keep the (s, y) pairs of sdot_solve in HBM
+// and run its two-loop recursion there: on one host thread the recursion
+// over 5 pairs of 500,000-vectors takes ~10 ms, on the device it is a few
+// dozen launches plus the transfers of g and p.
+// SDOT_LBFGS_DEVICE_MIN (a non-negative integer) overrides the length.
+constexpr int64_t kDevicePairsMin = 65536;
+static int64_t device_pairs_min() {
+  const char *e = std::getenv("SDOT_LBFGS_DEVICE_MIN");
+  if (e && *e) {
+    char *end = nullptr;
+    const long long v = std::strtoll(e, &end, 10);
+    if (end && *end == 0 && v >= 0) return (int64_t)v;
+  }
+  return kDevicePairsMin;
+}
+
+// The pairs store and direction of lbfgs_solve on the device (sdot_lbfgs.hip);
+// the first HIP error ends the solve and is kept in err.
+struct DevicePairs {
+  hipStream_t stream = nullptr;
+  LbVecs L{};
+  double *mem = nullptr;
+  hipError_t err = hipSuccess;
+  const char *what = "";
+
+  bool check(hipError_t e, const char *where) {
+    if (e != hipSuccess && err == hipSuccess) {
+      err = e;
+      what = where;
+    }
+    return e == hipSuccess;
+  }
+  bool init(int64_t n, int history, const double *h0, hipStream_t s) {
+    stream = s;
+    const size_t words = (size_t)(2 * history + 1 + (h0 ? 1 : 0)) * (size_t)n + kLbParts + history;
+    if (!check(hipMalloc((void **)&mem, words * sizeof(double)), "hipMalloc")) return false;
+    double *w = mem;
+    L.S = w;
+    w += (size_t)history * n;
+    L.Y = w;
+    w += (size_t)history * n;
+    L.p = w;
+    w += n;
+    L.h0 = nullptr;
+    if (h0) {
+      if (!check(hipMemcpyAsync(w, h0, sizeof(double) * n, hipMemcpyHostToDevice, s), "h0 upload"))
+        return false;
+      L.h0 = w;
+      w += n;
+    }
+    L.part = w;
+    L.coef = w + kLbParts;
+    L.n = n;
+    return true;
+  }
+  bool store(int slot, const double *sk, const double *yk) {
+    const size_t off = (size_t)slot * (size_t)L.n, bytes = sizeof(double) * (size_t)L.n;
+    return check(hipMemcpyAsync(const_cast<double *>(L.S) + off, sk, bytes, hipMemcpyHostToDevice,
+                                stream), "pair upload") &&
+           check(hipMemcpyAsync(const_cast<double *>(L.Y) + off, yk, bytes, hipMemcpyHostToDevice,
+                                stream), "pair upload");
+  }
+  bool direction(const double *g, double *p, const int *slots, const double *rho, int count) {
+    const size_t bytes = sizeof(double) * (size_t)L.n;
+    return check(hipMemcpyAsync(L.p, g, bytes, hipMemcpyHostToDevice, stream), "gradient upload") &&
+           check(launch_lb_direction(L, slots, rho, count, stream), "two-loop recursion") &&
+           check(hipMemcpyAsync(p, L.p, bytes, hipMemcpyDeviceToHost, stream), "direction download") &&
+           check(hipStreamSynchronize(stream), "two-loop recursion");
+  }
+  ~DevicePairs() {
+    if (mem) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipFree(mem);
+    }
+  }
+};
+
 extern "C" int sdot_solve(sdot_ctx *c, double *psi, sdot_solve_info *info) {
   if (ctx_check(c)) return SDOT_ERR_INVALID;
   if (!c->have_source || !c->have_target)
@@ -1592,6 +1670,22 @@ extern "C" int sdot_solve(sdot_ctx *c, double *psi, sdot_solve_info *info) {
       h0v[j] = c->params.epsilon / (c->h_nu[j] > 0.0 ? c->h_nu[j] : mean);
   }
   const double *h0 = h0v.empty() ? nullptr : h0v.data();
+  // lbfgs_solve with the pairs on the host, or in HBM for long potentials
+  DevicePairs dev_pairs;
+  const bool on_device = N >= device_pairs_min();
+  if (on_device && !dev_pairs.init(N, 5, h0, c->stream))
+    return fail(SDOT_ERR_HIP, std::string("L-BFGS pairs: ") + dev_pairs.what + ": " +
+                                  hipGetErrorString(dev_pairs.err));
+  auto bfgs = [&](auto &&chk, double noise) {
+    return on_device ? lbfgs_solve(N, psi, eval, chk, 5, noise, h0, &dev_pairs)
+                     : lbfgs_solve(N, psi, eval, chk, 5, noise, h0);
+  };
+  auto pairs_rc = [&]() {
+    return dev_pairs.err == hipSuccess
+               ? SDOT_OK
+               : fail(SDOT_ERR_HIP, std::string("L-BFGS pairs: ") + dev_pairs.what + ": " +
+                                        hipGetErrorString(dev_pairs.err));
+  };
   BfgsResult r;
   int its = 0, evals = 0;
   if (low) {
@@ -1606,8 +1700,9 @@ extern "C" int sdot_solve(sdot_ctx *c, double *psi, sdot_solve_info *info) {
       if ((unsigned)step >= max_it || std::isnan(cv)) return kFailure;
       return kIterate;
     };
-    r = lbfgs_solve(N, psi, eval, check_low, 5, noise_aware ? kNoiseRelF32 : 0.0, h0);
+    r = bfgs(check_low, noise_aware ? kNoiseRelF32 : 0.0);
     if (eval_rc) return eval_rc;
+    if (pairs_rc()) return SDOT_ERR_HIP;
     its = r.iterations;
     evals = r.evaluations;
     if (r.status != kNonFinite) {
@@ -1622,15 +1717,17 @@ extern "C" int sdot_solve(sdot_ctx *c, double *psi, sdot_solve_info *info) {
       };
       c->force_mode = kF64Lse;
       c->hist_valid = false;
-      r = lbfgs_solve(N, psi, eval, check_hi, 5, noise_aware ? kNoiseRelF64 : 0.0, h0);
+      r = bfgs(check_hi, noise_aware ? kNoiseRelF64 : 0.0);
       c->force_mode = -1;
       if (eval_rc) return eval_rc;
+      if (pairs_rc()) return SDOT_ERR_HIP;
       its += r.iterations;
       evals += r.evaluations;
     }
   } else {
-    r = lbfgs_solve(N, psi, eval, check, 5, noise_aware ? kNoiseRelF64 : 0.0, h0);
+    r = bfgs(check, noise_aware ? kNoiseRelF64 : 0.0);
     if (eval_rc) return eval_rc;
+    if (pairs_rc()) return SDOT_ERR_HIP;
     its = r.iterations;
     evals = r.evaluations;
   }

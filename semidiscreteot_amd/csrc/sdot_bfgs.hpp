@@ -28,6 +28,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 namespace sdot {
@@ -152,14 +153,23 @@ inline double first_step(bool first, bool surrogate, double f_prev, double f, do
 // noise_rel |f|; 0 is the reference algorithm.
 // h0: diagonal of the initial inverse Hessian of the two-loop recursion
 // (nullptr: the identity, as deal.II's SolverBFGS).
-template <class Eval, class Check>
+// hist: where the (s, y) pairs live and the two-loop recursion runs (nullptr:
+// here, on host vectors); an object with
+//   store(slot, s, y)                  keeps pair `slot` (host vectors given),
+//   direction(g, p, slots, rho, count) p = -H g over the pairs slots[0..count)
+//                                      (newest first) with their rho.
+struct HostPairs {};
+template <class Eval, class Check, class Pairs = HostPairs>
 BfgsResult lbfgs_solve(int64_t n, double *x, Eval &&eval, Check &&check, int history = 5,
-                       double noise_rel = 0.0, const double *h0 = nullptr) {
+                       double noise_rel = 0.0, const double *h0 = nullptr,
+                       Pairs *hist = nullptr) {
+  constexpr bool host_pairs = std::is_same<Pairs, HostPairs>::value;
   BfgsResult res;
   std::vector<double> g(n), p(n), sk(n), yk(n), x0(n);
-  std::vector<std::vector<double>> S(history, std::vector<double>(n)),
-      Y(history, std::vector<double>(n));
+  std::vector<std::vector<double>> S(host_pairs ? history : 0, std::vector<double>(n)),
+      Y(host_pairs ? history : 0, std::vector<double>(n));
   std::vector<double> rho(history), coef(history);
+  std::vector<int> slots(history);
   int count = 0, newest = -1;  // ring buffer; slot of pair i (0 = newest) is (newest - i) mod h
   auto slot = [&](int i) { return ((newest - i) % history + history) % history; };
 
@@ -172,24 +182,38 @@ BfgsResult lbfgs_solve(int64_t n, double *x, Eval &&eval, Check &&check, int his
   double f_prev = 0.0;
   while (st == kIterate) {
     // two-loop recursion, H0 = identity or diag(h0)
-    std::memcpy(p.data(), g.data(), sizeof(double) * n);
-    for (int i = 0; i < count; ++i) {
-      const int s = slot(i);
-      coef[i] = rho[s] * bfgs_dot(S[s].data(), p.data(), n);
-      const double c = -coef[i];
-      const double *ys = Y[s].data();
-      for (int64_t t = 0; t < n; ++t) p[t] += c * ys[t];
+    if constexpr (!host_pairs) {
+      for (int i = 0; i < count; ++i) {
+        slots[i] = slot(i);
+        coef[i] = rho[slot(i)];
+      }
+      if (!hist->direction(g.data(), p.data(), slots.data(), coef.data(), count)) {
+        res.status = kNonFinite;
+        res.iterations = k;
+        res.f = f;
+        res.check = cv;
+        return res;
+      }
+    } else {
+      std::memcpy(p.data(), g.data(), sizeof(double) * n);
+      for (int i = 0; i < count; ++i) {
+        const int s = slot(i);
+        coef[i] = rho[s] * bfgs_dot(S[s].data(), p.data(), n);
+        const double c = -coef[i];
+        const double *ys = Y[s].data();
+        for (int64_t t = 0; t < n; ++t) p[t] += c * ys[t];
+      }
+      if (h0)
+        for (int64_t t = 0; t < n; ++t) p[t] *= h0[t];
+      for (int i = count - 1; i >= 0; --i) {
+        const int s = slot(i);
+        const double c2 = rho[s] * bfgs_dot(Y[s].data(), p.data(), n);
+        const double c = coef[i] - c2;
+        const double *ss = S[s].data();
+        for (int64_t t = 0; t < n; ++t) p[t] += c * ss[t];
+      }
+      for (int64_t t = 0; t < n; ++t) p[t] = -p[t];
     }
-    if (h0)
-      for (int64_t t = 0; t < n; ++t) p[t] *= h0[t];
-    for (int i = count - 1; i >= 0; --i) {
-      const int s = slot(i);
-      const double c2 = rho[s] * bfgs_dot(Y[s].data(), p.data(), n);
-      const double c = coef[i] - c2;
-      const double *ss = S[s].data();
-      for (int64_t t = 0; t < n; ++t) p[t] += c * ss[t];
-    }
-    for (int64_t t = 0; t < n; ++t) p[t] = -p[t];
     std::memcpy(sk.data(), x, sizeof(double) * n);
     std::memcpy(yk.data(), g.data(), sizeof(double) * n);
 
@@ -225,12 +249,20 @@ BfgsResult lbfgs_solve(int64_t n, double *x, Eval &&eval, Check &&check, int his
     const double curv = bfgs_dot(sk.data(), yk.data(), n);
     if (curv > 0.0 && history > 0) {
       newest = (newest + 1) % history;
-      S[newest].swap(sk);
-      Y[newest].swap(yk);
+      if constexpr (host_pairs) {
+        S[newest].swap(sk);
+        Y[newest].swap(yk);
+        sk.resize(n);
+        yk.resize(n);
+      } else if (!hist->store(newest, sk.data(), yk.data())) {
+        res.status = kNonFinite;
+        res.iterations = k;
+        res.f = f;
+        res.check = cv;
+        return res;
+      }
       rho[newest] = 1.0 / curv;
       if (count < history) ++count;
-      sk.resize(n);
-      yk.resize(n);
     }
   }
   res.status = st == kSuccess ? kConverged : (std::isnan(cv) ? kNonFinite : kMaxIter);

@@ -462,4 +462,20 @@ hipError_t launch_hessian_chunk(const Plan &p, Dev &d, int64_t q_begin, int64_t 
 hipError_t launch_hessian_final(const Plan &p, const double *d_A, int64_t Npad, const double *d_g,
                                 const double *d_nu, double *d_H, hipStream_t s);
 
+// The two-loop recursion of L-BFGS on device vectors (sdot_lbfgs.hip).
+// S, Y: the stored pairs, slot k at S + k n; p: g on entry, -H g on return,
+// H the inverse-Hessian approximation of the pairs slots[0..count) (newest
+// first, with their rho) over the initial diag(h0) (h0 == nullptr: identity).
+// part: kLbParts partial sums; coef: one value per pair.
+constexpr int kLbParts = 512;
+struct LbVecs {
+  const double *S, *Y;
+  int64_t n;
+  double *p;
+  const double *h0;
+  double *part, *coef;
+};
+hipError_t launch_lb_direction(const LbVecs &L, const int *slots, const double *rho, int count,
+                               hipStream_t s);
+
 }  // namespace sdot

@@ -399,6 +399,45 @@ def test_plain_2d_config_solve():
     s.close()
 
 
+@pytest.mark.parametrize("precond,precision", [("none", "fp64"), ("diagonal", "fp64"),
+                                               ("diagonal", "fp32_lse")])
+def test_device_lbfgs_pairs(precond, precision, monkeypatch):
+    """Solves of 65,536 targets and more keep the L-BFGS pairs in HBM and run the
+    two-loop recursion there (dot products over a fixed grid of partial sums).
+    Forced on (SDOT_LBFGS_DEVICE_MIN=0) for the plain 2D configuration
+    (1,000 targets, l1-norm tolerance 1e-8, 60 iterations from psi = 0), the
+    solve takes the decisions of the host recursion: same status and
+    evaluation count, potentials and functional to rounding (FP64); with
+    FP32 evaluations and the FP64 finish, the same status and potentials
+    within 1e-9 of their range."""
+    say(f"device L-BFGS pairs, {precond} {precision}")
+    src = O.structured_source(2, [64, 64], [0.0, 0.0], [1.0, 1.0], 3)
+    y = np.random.default_rng(0).random((1000, 2))
+    tgt = O.Target(y.ravel(), np.full(1000, 1e-3))
+    s = make(src, tgt, epsilon=1e-2, tau=0.0, tolerance=1e-8, solver_control_type="l1norm",
+             use_log_sum_exp_trick=True, verbose_output=False, max_iterations=60,
+             preconditioner=precond, precision=precision)
+    monkeypatch.delenv("SDOT_LBFGS_DEVICE_MIN", raising=False)
+    psi_h = s.solve()
+    info_h = s.get_last_solve_info()
+    monkeypatch.setenv("SDOT_LBFGS_DEVICE_MIN", "0")
+    psi_d = s.solve()
+    info_d = s.get_last_solve_info()
+    monkeypatch.delenv("SDOT_LBFGS_DEVICE_MIN")
+    d = float(np.max(np.abs(psi_d - psi_h)))
+    say(f"host {info_h['status']} ev={info_h['evaluations']} F={info_h['functional']:.15e}; "
+        f"device {info_d['status']} ev={info_d['evaluations']} F={info_d['functional']:.15e}; "
+        f"max |dpsi| {d:.2e}")
+    assert info_d["status"] == info_h["status"]
+    if precision == "fp64":
+        assert info_d["evaluations"] == info_h["evaluations"]
+        assert d < 1e-9
+        assert abs(info_d["functional"] - info_h["functional"]) <= 1e-13 * abs(info_h["functional"])
+    else:
+        assert d <= 1e-9 * float(np.ptp(psi_h))
+    s.close()
+
+
 @pytest.mark.parametrize("which", ["targets", "large_source"])
 def test_large_record_sets_match_oracle(which):
     """Record sets larger than an XCD's L2: 70,000 targets (4.5 MB of target
