@@ -651,10 +651,11 @@ double vmin(const double *v, int64_t n) {
 // compute_distance_threshold (SotSolver.cc:379-429) with its three bounds:
 // integral (SotSolver.cc:475-507), geometric (SotSolver.cc:509-551) and
 // pointwise (SotSolver.cc:406-419).
-int threshold(sdot_ctx *c, const double *psi, double *R) {
+// The radius of the stored threshold rule at a potential with maximum mx and
+// minimum mn (mn is read by the geometric rule only).
+int threshold_from(sdot_ctx *c, double mx, double mn, double *R) {
   const int64_t N = c->plan.N;
   const double eps = c->params.epsilon, tau = c->params.tau;
-  const double mx = vmax(psi, N);
   const int type = c->params.distance_threshold_type;
   if (type == SDOT_THRESH_INTEGRAL) {
     double absF = std::fabs(c->global_functional);
@@ -668,7 +669,7 @@ int threshold(sdot_ctx *c, const double *psi, double *R) {
       int rc = raw_covering_radius(c, &r0);
       if (rc) return rc;
     }
-    const double range = mx - vmin(psi, N);
+    const double range = mx - mn;
     const double mind = c->min_target_density > 0.0 ? c->min_target_density : vmin(c->h_nu.data(), N);
     double fv = std::fabs(c->global_functional);
     if (fv < 1e-10) fv = 1e-10;
@@ -688,6 +689,12 @@ int threshold(sdot_ctx *c, const double *psi, double *R) {
   return SDOT_OK;
 }
 
+int threshold(sdot_ctx *c, const double *psi, double *R) {
+  const int64_t N = c->plan.N;
+  const bool geometric = c->params.distance_threshold_type == SDOT_THRESH_GEOMETRIC;
+  return threshold_from(c, vmax(psi, N), geometric ? vmin(psi, N) : 0.0, R);
+}
+
 // evaluate_functional: threshold from the stored state, assembly, state update.
 int stateful_eval(sdot_ctx *c, const double *psi, double *grad, double *F, double *C) {
   double R;
@@ -701,6 +708,75 @@ int stateful_eval(sdot_ctx *c, const double *psi, double *grad, double *F, doubl
   c->C_global = cg;
   if (F) *F = f;
   if (C) *C = cg;
+  return SDOT_OK;
+}
+
+// The statistics of launch_lb_stats for device vector a, reduced on the host
+// in block order; d_part: kLbStats * kLbParts doubles on the device, h_part
+// as many on the host.
+hipError_t lb_stats(hipStream_t s, const double *a, const double *nu, double t, int64_t n,
+                    double *d_part, double *h_part, double out[kLbStats]) {
+  hipError_t e = launch_lb_stats(a, nu, t, n, d_part, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(h_part, d_part, sizeof(double) * kLbStats * kLbParts, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return e;
+  for (int q = 0; q < kLbStats; ++q) {
+    const double *v = h_part + q * kLbParts;
+    double r = v[0];
+    for (int b = 1; b < kLbParts; ++b) {
+      if (q == kLbL1 || q == kLbL2)
+        r += v[b];
+      else if (q == kLbMin)
+        r = std::min(r, v[b]);
+      else
+        r = std::max(r, v[b]);
+    }
+    out[q] = r;
+  }
+  return hipSuccess;
+}
+
+// stateful_eval on a potential and gradient in HBM (caller order); the
+// threshold rule reads the potential's maximum and minimum from a device
+// reduction (d_part, h_part as in lb_stats).
+int stateful_eval_device(sdot_ctx *c, const double *d_psi, double *d_grad, double *d_part,
+                         double *h_part, double *F) {
+  const int64_t N = c->plan.N;
+  double st[kLbStats];
+  HIP_TRY(lb_stats(c->stream, d_psi, nullptr, 0.0, N, d_part, h_part, st));
+  double R;
+  int rc = threshold_from(c, st[kLbMax], st[kLbMin], &R);
+  if (rc) return rc;
+  c->current_threshold = R;
+  rc = prepare(c, R);
+  if (rc) return rc;
+  const bool seeded = c->plan.hist != 0;
+  rc = run_eval(c, R, d_psi, d_grad, c->dev.out);
+  if (rc) return rc;
+  if (seeded) {
+    // a seed far above the new maxima would have flushed the FP32 terms: redo unseeded
+    double lost = 0.0;
+    HIP_TRY(hipMemcpyAsync(&lost, c->dev.ucount, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (lost > 0.0) {
+      c->hist_valid = false;
+      rc = prepare(c, R);
+      if (rc) return rc;
+      rc = run_eval(c, R, d_psi, d_grad, c->dev.out);
+      if (rc) return rc;
+    }
+  }
+  double fc[2];
+  HIP_TRY(hipMemcpyAsync(fc, c->dev.out, sizeof(double) * 2, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (c->prof_on) {
+    rc = resolve_profile(c);
+    if (rc) return rc;
+  }
+  c->global_functional = fc[0];
+  c->C_global = fc[1];
+  *F = fc[0];
   return SDOT_OK;
 }
 
@@ -1513,10 +1589,10 @@ extern "C" int sdot_transport_map(sdot_ctx *c, int strategy, const double *psi, 
                      points, nullptr, 0, nullptr, mapped);
 }
 
-// Potentials at least this long keep the (s, y) pairs of sdot_solve in HBM
-// and run its two-loop recursion there: on one host thread the recursion
-// over 5 pairs of 500,000-vectors takes ~10 ms, on the device it is a few
-// dozen launches plus the transfers of g and p.
+// Potentials at least this long are solved with every L-BFGS vector in HBM
+// (the iterate, gradient, direction and the 5 (s, y) pairs): on one host
+// thread the two-loop recursion over 500,000-vectors alone takes ~10 ms, and
+// x and g would cross PCIe at every evaluation.
 // SDOT_LBFGS_DEVICE_MIN (a non-negative integer) overrides the length.
 constexpr int64_t kDevicePairsMin = 65536;
 static int64_t device_pairs_min() {
@@ -1529,15 +1605,59 @@ static int64_t device_pairs_min() {
   return kDevicePairsMin;
 }
 
-// The pairs store and direction of lbfgs_solve on the device (sdot_lbfgs.hip);
-// the first HIP error ends the solve and is kept in err.
-struct DevicePairs {
-  hipStream_t stream = nullptr;
-  LbVecs L{};
-  double *mem = nullptr;
+// The space of lbfgs_core on device vectors (sdot_lbfgs.hip): the iterate
+// x(), the work vectors, the pairs, diag(h0) and the target weights in one
+// allocation.  Dot products are reduced on the host from kLbParts partial
+// sums in block order.  The first HIP error ends the solve and is kept.
+class DeviceSpace {
+ public:
   hipError_t err = hipSuccess;
   const char *what = "";
 
+  bool init(int64_t n, int history, const double *h0, const double *nu, hipStream_t s) {
+    n_ = n;
+    stream_ = s;
+    history_ = history;
+    if (history > kLbMaxHist) return check(hipErrorInvalidValue, "history length");
+    const size_t vn = (size_t)n;
+    const size_t words = (size_t)(kNumVecs + 1 + 2 * history + 1 + (h0 ? 1 : 0)) * vn +
+                         (size_t)kLbStats * kLbParts + (size_t)history;
+    if (!check(hipMalloc((void **)&mem_, words * sizeof(double)), "hipMalloc")) return false;
+    if (!check(hipHostMalloc((void **)&hpart_, sizeof(double) * kLbStats * kLbParts,
+                             hipHostMallocDefault), "hipHostMalloc"))
+      return false;
+    double *w = mem_;
+    for (int i = 0; i < kNumVecs; ++i, w += vn) work_[i] = w;
+    x_ = w;
+    w += vn;
+    for (int i = 0; i < history; ++i, w += vn) S_[i] = w;
+    for (int i = 0; i < history; ++i, w += vn) Y_[i] = w;
+    nu_ = w;
+    w += vn;
+    if (!upload(nu_, nu, "weights upload")) return false;
+    if (h0) {
+      h0_ = w;
+      w += vn;
+      if (!upload(h0_, h0, "h0 upload")) return false;
+    }
+    part_ = w;
+    coef_ = w + (size_t)kLbStats * kLbParts;
+    return true;
+  }
+  ~DeviceSpace() {
+    if (mem_) {
+      (void)hipStreamSynchronize(stream_);
+      (void)hipFree(mem_);
+    }
+    if (hpart_) (void)hipHostFree(hpart_);
+  }
+
+  double *x() { return x_; }
+  double *vec(int i) { return work_[i]; }
+  double *part() { return part_; }
+  double *hpart() { return hpart_; }
+  const double *nu() const { return nu_; }
+  bool ok() const { return err == hipSuccess; }
   bool check(hipError_t e, const char *where) {
     if (e != hipSuccess && err == hipSuccess) {
       err = e;
@@ -1545,49 +1665,67 @@ struct DevicePairs {
     }
     return e == hipSuccess;
   }
-  bool init(int64_t n, int history, const double *h0, hipStream_t s) {
-    stream = s;
-    const size_t words = (size_t)(2 * history + 1 + (h0 ? 1 : 0)) * (size_t)n + kLbParts + history;
-    if (!check(hipMalloc((void **)&mem, words * sizeof(double)), "hipMalloc")) return false;
-    double *w = mem;
-    L.S = w;
-    w += (size_t)history * n;
-    L.Y = w;
-    w += (size_t)history * n;
-    L.p = w;
-    w += n;
-    L.h0 = nullptr;
-    if (h0) {
-      if (!check(hipMemcpyAsync(w, h0, sizeof(double) * n, hipMemcpyHostToDevice, s), "h0 upload"))
-        return false;
-      L.h0 = w;
-      w += n;
+  bool upload(double *d, const double *h, const char *where) {
+    return check(hipMemcpyAsync(d, h, sizeof(double) * n_, hipMemcpyHostToDevice, stream_), where);
+  }
+  bool download(double *h, const double *d, const char *where) {
+    return check(hipMemcpyAsync(h, d, sizeof(double) * n_, hipMemcpyDeviceToHost, stream_), where) &&
+           check(hipStreamSynchronize(stream_), where);
+  }
+
+  void copy(double *d, const double *s) {
+    if (ok()) check(hipMemcpyAsync(d, s, sizeof(double) * n_, hipMemcpyDeviceToDevice, stream_), "copy");
+  }
+  double dot(const double *a, const double *b) {
+    if (!ok() || !check(launch_lb_dot(a, b, n_, part_, stream_), "dot") ||
+        !check(hipMemcpyAsync(hpart_, part_, sizeof(double) * kLbParts, hipMemcpyDeviceToHost,
+                              stream_), "dot") ||
+        !check(hipStreamSynchronize(stream_), "dot"))
+      return NAN;
+    double r = hpart_[0];
+    for (int b = 1; b < kLbParts; ++b) r += hpart_[b];
+    return r;
+  }
+  void lin(double *x, const double *x0, double a, const double *p) {
+    if (ok()) check(launch_lb_lin(x, x0, a, p, n_, stream_), "x0 + a p");
+  }
+  void diff(double *sv, const double *x, double *y, const double *g) {
+    if (ok()) check(launch_lb_diff(sv, x, y, g, n_, stream_), "pair differences");
+  }
+  void store(int slot) {
+    std::swap(S_[slot], work_[kVecS]);
+    std::swap(Y_[slot], work_[kVecY]);
+  }
+  void direction(const double *g, double *p, const int *slots, const double *rho, int count) {
+    if (!ok()) return;
+    LbVecs L{};
+    for (int i = 0; i < history_; ++i) {
+      L.S[i] = S_[i];
+      L.Y[i] = Y_[i];
     }
-    L.part = w;
-    L.coef = w + kLbParts;
-    L.n = n;
-    return true;
+    L.n = n_;
+    L.p = p;
+    L.h0 = h0_;
+    L.part = part_;
+    L.coef = coef_;
+    copy(p, g);
+    if (ok()) check(launch_lb_direction(L, slots, rho, count, stream_), "two-loop recursion");
   }
-  bool store(int slot, const double *sk, const double *yk) {
-    const size_t off = (size_t)slot * (size_t)L.n, bytes = sizeof(double) * (size_t)L.n;
-    return check(hipMemcpyAsync(const_cast<double *>(L.S) + off, sk, bytes, hipMemcpyHostToDevice,
-                                stream), "pair upload") &&
-           check(hipMemcpyAsync(const_cast<double *>(L.Y) + off, yk, bytes, hipMemcpyHostToDevice,
-                                stream), "pair upload");
+  // sum |a|, sum a^2, max (|a| - t nu) of a device vector (NaN after a failure)
+  bool stats(const double *a, double t, double out[kLbStats]) {
+    if (ok() && check(lb_stats(stream_, a, nu_, t, n_, part_, hpart_, out), "gradient statistics"))
+      return true;
+    for (int q = 0; q < kLbStats; ++q) out[q] = NAN;
+    return false;
   }
-  bool direction(const double *g, double *p, const int *slots, const double *rho, int count) {
-    const size_t bytes = sizeof(double) * (size_t)L.n;
-    return check(hipMemcpyAsync(L.p, g, bytes, hipMemcpyHostToDevice, stream), "gradient upload") &&
-           check(launch_lb_direction(L, slots, rho, count, stream), "two-loop recursion") &&
-           check(hipMemcpyAsync(p, L.p, bytes, hipMemcpyDeviceToHost, stream), "direction download") &&
-           check(hipStreamSynchronize(stream), "two-loop recursion");
-  }
-  ~DevicePairs() {
-    if (mem) {
-      (void)hipStreamSynchronize(stream);
-      (void)hipFree(mem);
-    }
-  }
+
+ private:
+  int64_t n_ = 0;
+  int history_ = 0;
+  hipStream_t stream_ = nullptr;
+  double *mem_ = nullptr, *hpart_ = nullptr;
+  double *work_[kNumVecs] = {}, *x_ = nullptr, *S_[kLbMaxHist] = {}, *Y_[kLbMaxHist] = {};
+  double *nu_ = nullptr, *h0_ = nullptr, *part_ = nullptr, *coef_ = nullptr;
 };
 
 extern "C" int sdot_solve(sdot_ctx *c, double *psi, sdot_solve_info *info) {
@@ -1604,40 +1742,13 @@ extern "C" int sdot_solve(sdot_ctx *c, double *psi, sdot_solve_info *info) {
   c->covering_radius = r0 * 1.1;
   c->min_target_density = vmin(c->h_nu.data(), N);
   int eval_rc = SDOT_OK;
-  auto eval = [&](const double *x, double *g) -> double {
-    double F = NAN;
-    if (eval_rc == SDOT_OK) {
-      eval_rc = stateful_eval(c, x, g, &F, nullptr);
-      if (eval_rc) {
-        F = NAN;
-        for (int64_t j = 0; j < N; ++j) g[j] = NAN;
-      }
-    }
-    return F;
-  };
   const bool comp = c->params.solver_control_type == SDOT_CTRL_COMPONENTWISE;
   const double tol = comp ? DBL_MIN : c->params.tolerance;
   const double utol = c->params.tolerance;
   const unsigned max_it = c->params.max_iterations;
-  // stopping value at tolerance scale `mult` (SotSolver.h:382-446)
-  auto check_value = [&](const double *g, double mult) {
-    double cv;
-    if (comp) {
-      cv = -INFINITY;
-      for (int64_t j = 0; j < N; ++j) cv = std::max(cv, std::fabs(g[j]) - c->h_nu[j] * utol * mult);
-    } else {
-      cv = 0.0;
-      for (int64_t j = 0; j < N; ++j) cv += std::fabs(g[j]);
-    }
-    return cv;
-  };
   // the iteration line VerboseSolverControl::check prints (SotSolver.h:408-440)
   double l1_first = 0.0;
-  auto report = [&](int step, const double *g, double cv) {
-    if (!c->params.verbose_output) return;
-    double l2 = 0.0;
-    for (int64_t j = 0; j < N; ++j) l2 += g[j] * g[j];
-    l2 = std::sqrt(l2);
+  auto report = [&](int step, double l2, double cv) {
     if (comp) {
       std::printf("Iteration %d - L-2 gradient norm: %g - Max Scaled Residual (max |g_i| - T_i*tol): %g\n",
                   step, l2, cv);
@@ -1648,13 +1759,6 @@ extern "C" int sdot_solve(sdot_ctx *c, double *psi, sdot_solve_info *info) {
                   step, l2, cv, rel);
     }
     std::fflush(stdout);
-  };
-  auto check = [&](int step, const double *g, double &cv) -> int {
-    cv = check_value(g, 1.0);
-    report(step, g, cv);
-    if (cv <= tol) return kSuccess;
-    if ((unsigned)step >= max_it || std::isnan(cv)) return kFailure;
-    return kIterate;
   };
   const bool noise_aware = c->params.line_search == SDOT_LS_NOISE_AWARE;
   const bool low = c->params.use_log_sum_exp_trick && c->params.precision == SDOT_PREC_FP32_LSE;
@@ -1670,66 +1774,130 @@ extern "C" int sdot_solve(sdot_ctx *c, double *psi, sdot_solve_info *info) {
       h0v[j] = c->params.epsilon / (c->h_nu[j] > 0.0 ? c->h_nu[j] : mean);
   }
   const double *h0 = h0v.empty() ? nullptr : h0v.data();
-  // lbfgs_solve with the pairs on the host, or in HBM for long potentials
-  DevicePairs dev_pairs;
-  const bool on_device = N >= device_pairs_min();
-  if (on_device && !dev_pairs.init(N, 5, h0, c->stream))
-    return fail(SDOT_ERR_HIP, std::string("L-BFGS pairs: ") + dev_pairs.what + ": " +
-                                  hipGetErrorString(dev_pairs.err));
-  auto bfgs = [&](auto &&chk, double noise) {
-    return on_device ? lbfgs_solve(N, psi, eval, chk, 5, noise, h0, &dev_pairs)
-                     : lbfgs_solve(N, psi, eval, chk, 5, noise, h0);
-  };
-  auto pairs_rc = [&]() {
-    return dev_pairs.err == hipSuccess
-               ? SDOT_OK
-               : fail(SDOT_ERR_HIP, std::string("L-BFGS pairs: ") + dev_pairs.what + ": " +
-                                        hipGetErrorString(dev_pairs.err));
-  };
   BfgsResult r;
   int its = 0, evals = 0;
-  if (low) {
-    // FP32_LSE: iterate on FP32 evaluations until the stopping value is within
-    // kPolishFactor of the tolerance (or the budget ends, or a line search
-    // fails), then finish on FP64 evaluations, so that the stopping decision
-    // is taken on an FP64 gradient
-    auto check_low = [&](int step, const double *g, double &cv) -> int {
-      report(step, g, check_value(g, 1.0));
-      cv = check_value(g, kPolishFactor);
-      if (cv <= (comp ? DBL_MIN : utol * kPolishFactor)) return kSuccess;
+  // The solve over a space's vectors: stat(g, mult, &l2) is the stopping value
+  // at tolerance scale `mult` (SotSolver.h:382-446), with the L-2 norm of g
+  // when l2 != nullptr.
+  auto run = [&](auto &sp, double *x, auto &&eval, auto &&stat) {
+    auto check = [&](int step, const double *g, double &cv) -> int {
+      double l2 = 0.0;
+      cv = stat(g, 1.0, c->params.verbose_output ? &l2 : nullptr);
+      if (c->params.verbose_output) report(step, l2, cv);
+      if (cv <= tol) return kSuccess;
       if ((unsigned)step >= max_it || std::isnan(cv)) return kFailure;
       return kIterate;
     };
-    r = bfgs(check_low, noise_aware ? kNoiseRelF32 : 0.0);
-    if (eval_rc) return eval_rc;
-    if (pairs_rc()) return SDOT_ERR_HIP;
-    its = r.iterations;
-    evals = r.evaluations;
-    if (r.status != kNonFinite) {
-      const unsigned left = max_it > (unsigned)its ? max_it - (unsigned)its : 0u;
-      const int offset = its;
-      auto check_hi = [&](int step, const double *g, double &cv) -> int {
-        cv = check_value(g, 1.0);
-        report(offset + step, g, cv);
-        if (cv <= tol) return kSuccess;
-        if ((unsigned)step >= left || std::isnan(cv)) return kFailure;
+    if (low) {
+      // FP32_LSE: iterate on FP32 evaluations until the stopping value is within
+      // kPolishFactor of the tolerance (or the budget ends, or a line search
+      // fails), then finish on FP64 evaluations, so that the stopping decision
+      // is taken on an FP64 gradient
+      auto check_low = [&](int step, const double *g, double &cv) -> int {
+        if (c->params.verbose_output) {
+          double l2 = 0.0;
+          const double v = stat(g, 1.0, &l2);
+          report(step, l2, v);
+        }
+        cv = stat(g, kPolishFactor, nullptr);
+        if (cv <= (comp ? DBL_MIN : utol * kPolishFactor)) return kSuccess;
+        if ((unsigned)step >= max_it || std::isnan(cv)) return kFailure;
         return kIterate;
       };
-      c->force_mode = kF64Lse;
-      c->hist_valid = false;
-      r = bfgs(check_hi, noise_aware ? kNoiseRelF64 : 0.0);
-      c->force_mode = -1;
-      if (eval_rc) return eval_rc;
-      if (pairs_rc()) return SDOT_ERR_HIP;
-      its += r.iterations;
-      evals += r.evaluations;
+      r = lbfgs_core(sp, x, eval, check_low, 5, noise_aware ? kNoiseRelF32 : 0.0);
+      if (eval_rc) return;
+      its = r.iterations;
+      evals = r.evaluations;
+      if (r.status != kNonFinite) {
+        const unsigned left = max_it > (unsigned)its ? max_it - (unsigned)its : 0u;
+        const int offset = its;
+        auto check_hi = [&](int step, const double *g, double &cv) -> int {
+          double l2 = 0.0;
+          cv = stat(g, 1.0, c->params.verbose_output ? &l2 : nullptr);
+          if (c->params.verbose_output) report(offset + step, l2, cv);
+          if (cv <= tol) return kSuccess;
+          if ((unsigned)step >= left || std::isnan(cv)) return kFailure;
+          return kIterate;
+        };
+        c->force_mode = kF64Lse;
+        c->hist_valid = false;
+        r = lbfgs_core(sp, x, eval, check_hi, 5, noise_aware ? kNoiseRelF64 : 0.0);
+        c->force_mode = -1;
+        if (eval_rc) return;
+        its += r.iterations;
+        evals += r.evaluations;
+      }
+    } else {
+      r = lbfgs_core(sp, x, eval, check, 5, noise_aware ? kNoiseRelF64 : 0.0);
+      if (eval_rc) return;
+      its = r.iterations;
+      evals = r.evaluations;
     }
+  };
+  if (N >= device_pairs_min()) {
+    // every vector in HBM; psi goes up once and comes back once
+    DeviceSpace sp;
+    auto space_rc = [&]() {
+      return fail(SDOT_ERR_HIP, std::string("L-BFGS on the device: ") + sp.what + ": " +
+                                    hipGetErrorString(sp.err));
+    };
+    if (!sp.init(N, 5, h0, c->h_nu.data(), c->stream) || !sp.upload(sp.x(), psi, "psi upload"))
+      return space_rc();
+    auto eval = [&](const double *x, double *g) -> double {
+      double F = NAN;
+      if (eval_rc == SDOT_OK) {
+        eval_rc = stateful_eval_device(c, x, g, sp.part(), sp.hpart(), &F);
+        if (eval_rc) F = NAN;
+      }
+      return F;
+    };
+    auto stat = [&](const double *g, double mult, double *l2) -> double {
+      double st[kLbStats];
+      if (eval_rc || !sp.stats(g, utol * mult, st)) return NAN;
+      if (l2) *l2 = std::sqrt(st[kLbL2]);
+      return comp ? st[kLbCw] : st[kLbL1];
+    };
+    run(sp, sp.x(), eval, stat);
+    const bool back = sp.download(psi, sp.x(), "psi download");
+    if (eval_rc) {
+      c->force_mode = -1;
+      return eval_rc;
+    }
+    if (!back || !sp.ok()) return space_rc();
   } else {
-    r = bfgs(check, noise_aware ? kNoiseRelF64 : 0.0);
-    if (eval_rc) return eval_rc;
-    if (pairs_rc()) return SDOT_ERR_HIP;
-    its = r.iterations;
-    evals = r.evaluations;
+    HostSpace sp(N, 5, h0);
+    auto eval = [&](const double *x, double *g) -> double {
+      double F = NAN;
+      if (eval_rc == SDOT_OK) {
+        eval_rc = stateful_eval(c, x, g, &F, nullptr);
+        if (eval_rc) {
+          F = NAN;
+          for (int64_t j = 0; j < N; ++j) g[j] = NAN;
+        }
+      }
+      return F;
+    };
+    auto stat = [&](const double *g, double mult, double *l2) -> double {
+      if (l2) {
+        double q = 0.0;
+        for (int64_t j = 0; j < N; ++j) q += g[j] * g[j];
+        *l2 = std::sqrt(q);
+      }
+      double cv;
+      if (comp) {
+        cv = -INFINITY;
+        for (int64_t j = 0; j < N; ++j) cv = std::max(cv, std::fabs(g[j]) - c->h_nu[j] * utol * mult);
+      } else {
+        cv = 0.0;
+        for (int64_t j = 0; j < N; ++j) cv += std::fabs(g[j]);
+      }
+      return cv;
+    };
+    run(sp, psi, eval, stat);
+    if (eval_rc) {
+      c->force_mode = -1;
+      return eval_rc;
+    }
   }
   c->last_iterations = its;
   c->converged = r.status == kConverged;

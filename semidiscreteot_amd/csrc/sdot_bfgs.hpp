@@ -149,118 +149,85 @@ inline double first_step(bool first, bool surrogate, double f_prev, double f, do
   return std::fmin(1.0, -1.01 * 2.0 * df / g0);
 }
 
-// noise_rel > 0 selects the noise-aware line search with noise floor
-// noise_rel |f|; 0 is the reference algorithm.
-// h0: diagonal of the initial inverse Hessian of the two-loop recursion
-// (nullptr: the identity, as deal.II's SolverBFGS).
-// hist: where the (s, y) pairs live and the two-loop recursion runs (nullptr:
-// here, on host vectors); an object with
-//   store(slot, s, y)                  keeps pair `slot` (host vectors given),
-//   direction(g, p, slots, rho, count) p = -H g over the pairs slots[0..count)
-//                                      (newest first) with their rho.
-struct HostPairs {};
-template <class Eval, class Check, class Pairs = HostPairs>
-BfgsResult lbfgs_solve(int64_t n, double *x, Eval &&eval, Check &&check, int history = 5,
-                       double noise_rel = 0.0, const double *h0 = nullptr,
-                       Pairs *hist = nullptr) {
-  constexpr bool host_pairs = std::is_same<Pairs, HostPairs>::value;
+// The vectors of lbfgs_core live in a Space, which holds the work vectors
+// vec(kG .. kX0) and the (s, y) pairs and does the vector arithmetic:
+//   copy(d, s)               d = s
+//   dot(a, b)                a . b
+//   lin(x, x0, a, p)         x = x0 + a p
+//   diff(s, x, y, g)         s = x - s, y = g - y
+//   direction(g, p, slots, rho, count)
+//                            p = -H g, H over the pairs slots[0..count) (newest
+//                            first) with their rho, on the initial inverse
+//                            Hessian of the space (identity or a diagonal)
+//   store(slot)              pair `slot` takes vec(kS), vec(kY) (whose contents
+//                            are free afterwards)
+//   ok()                     false after a failure of the space's own (which
+//                            ends the solve with kNonFinite)
+// x is a vector of the space; eval(x, g) and check(k, g, cv) take the space's
+// vectors.
+enum { kVecG = 0, kVecP, kVecS, kVecY, kVecX0, kNumVecs };
+
+template <class Space, class Eval, class Check>
+BfgsResult lbfgs_core(Space &sp, double *x, Eval &&eval, Check &&check, int history,
+                      double noise_rel) {
   BfgsResult res;
-  std::vector<double> g(n), p(n), sk(n), yk(n), x0(n);
-  std::vector<std::vector<double>> S(host_pairs ? history : 0, std::vector<double>(n)),
-      Y(host_pairs ? history : 0, std::vector<double>(n));
   std::vector<double> rho(history), coef(history);
   std::vector<int> slots(history);
   int count = 0, newest = -1;  // ring buffer; slot of pair i (0 = newest) is (newest - i) mod h
   auto slot = [&](int i) { return ((newest - i) % history + history) % history; };
+  auto stop = [&](int status, int k, double f, double cv) {
+    res.status = status;
+    res.iterations = k;
+    res.f = f;
+    res.check = cv;
+    return res;
+  };
+  double *g = sp.vec(kVecG), *p = sp.vec(kVecP), *x0 = sp.vec(kVecX0);
 
-  double f = eval(x, g.data());
+  double f = eval(x, g);
   res.evaluations = 1;
   int k = 0;
   double cv = 0.0;
-  int st = check(k, g.data(), cv);
+  int st = check(k, g, cv);
   bool first = true, sur = false;
   double f_prev = 0.0;
   while (st == kIterate) {
     // two-loop recursion, H0 = identity or diag(h0)
-    if constexpr (!host_pairs) {
-      for (int i = 0; i < count; ++i) {
-        slots[i] = slot(i);
-        coef[i] = rho[slot(i)];
-      }
-      if (!hist->direction(g.data(), p.data(), slots.data(), coef.data(), count)) {
-        res.status = kNonFinite;
-        res.iterations = k;
-        res.f = f;
-        res.check = cv;
-        return res;
-      }
-    } else {
-      std::memcpy(p.data(), g.data(), sizeof(double) * n);
-      for (int i = 0; i < count; ++i) {
-        const int s = slot(i);
-        coef[i] = rho[s] * bfgs_dot(S[s].data(), p.data(), n);
-        const double c = -coef[i];
-        const double *ys = Y[s].data();
-        for (int64_t t = 0; t < n; ++t) p[t] += c * ys[t];
-      }
-      if (h0)
-        for (int64_t t = 0; t < n; ++t) p[t] *= h0[t];
-      for (int i = count - 1; i >= 0; --i) {
-        const int s = slot(i);
-        const double c2 = rho[s] * bfgs_dot(Y[s].data(), p.data(), n);
-        const double c = coef[i] - c2;
-        const double *ss = S[s].data();
-        for (int64_t t = 0; t < n; ++t) p[t] += c * ss[t];
-      }
-      for (int64_t t = 0; t < n; ++t) p[t] = -p[t];
+    for (int i = 0; i < count; ++i) {
+      slots[i] = slot(i);
+      coef[i] = rho[slot(i)];
     }
-    std::memcpy(sk.data(), x, sizeof(double) * n);
-    std::memcpy(yk.data(), g.data(), sizeof(double) * n);
+    sp.direction(g, p, slots.data(), coef.data(), count);
+    double *sk = sp.vec(kVecS), *yk = sp.vec(kVecY);
+    sp.copy(sk, x);
+    sp.copy(yk, g);
 
-    const double f0 = f, g0 = bfgs_dot(g.data(), p.data(), n);
-    std::memcpy(x0.data(), x, sizeof(double) * n);
+    const double f0 = f, g0 = sp.dot(g, p);
+    if (!sp.ok()) return stop(kNonFinite, k, f, cv);
+    sp.copy(x0, x);
     sur = noise_rel > 0.0 && (sur || std::fabs(g0) < 100.0 * noise_rel * std::fabs(f0));
     const double a1 = first_step(first, sur, f_prev, f, g0);
     f_prev = f;
     auto phi = [&](double a, double &fo, double &go) {
-      for (int64_t t = 0; t < n; ++t) x[t] = x0[t] + a * p[t];
-      f = eval(x, g.data());
+      sp.lin(x, x0, a, p);
+      f = eval(x, g);
       ++res.evaluations;
       fo = f;
-      go = bfgs_dot(g.data(), p.data(), n);
+      go = sp.dot(g, p);
     };
     double alpha = 0.0;
-    if (!wolfe_line_search(phi, f0, g0, a1, alpha, sur)) {
-      ++k;
-      res.status = kLineSearchFail;
-      res.iterations = k;
-      res.f = f;
-      res.check = cv;
-      return res;
-    }
+    if (!wolfe_line_search(phi, f0, g0, a1, alpha, sur))
+      return stop(sp.ok() ? kLineSearchFail : kNonFinite, k + 1, f, cv);
     first = false;
-    for (int64_t t = 0; t < n; ++t) {
-      sk[t] = x[t] - sk[t];
-      yk[t] = g[t] - yk[t];
-    }
+    sp.diff(sk, x, yk, g);
     ++k;
-    st = check(k, g.data(), cv);
+    st = check(k, g, cv);
     if (st != kIterate) break;
-    const double curv = bfgs_dot(sk.data(), yk.data(), n);
+    const double curv = sp.dot(sk, yk);
+    if (!sp.ok()) return stop(kNonFinite, k, f, cv);
     if (curv > 0.0 && history > 0) {
       newest = (newest + 1) % history;
-      if constexpr (host_pairs) {
-        S[newest].swap(sk);
-        Y[newest].swap(yk);
-        sk.resize(n);
-        yk.resize(n);
-      } else if (!hist->store(newest, sk.data(), yk.data())) {
-        res.status = kNonFinite;
-        res.iterations = k;
-        res.f = f;
-        res.check = cv;
-        return res;
-      }
+      sp.store(newest);
       rho[newest] = 1.0 / curv;
       if (count < history) ++count;
     }
@@ -270,6 +237,71 @@ BfgsResult lbfgs_solve(int64_t n, double *x, Eval &&eval, Check &&check, int his
   res.f = f;
   res.check = cv;
   return res;
+}
+
+// Host vectors of length n; the initial inverse Hessian is diag(h0), or the
+// identity when h0 == nullptr (deal.II's SolverBFGS).
+class HostSpace {
+ public:
+  HostSpace(int64_t n, int history, const double *h0)
+      : n_(n), h0_(h0), work_(kNumVecs, std::vector<double>(n)),
+        S_(history, std::vector<double>(n)), Y_(history, std::vector<double>(n)),
+        coef_(history) {}
+  double *vec(int i) { return work_[i].data(); }
+  bool ok() const { return true; }
+  void copy(double *d, const double *s) { std::memcpy(d, s, sizeof(double) * n_); }
+  double dot(const double *a, const double *b) { return bfgs_dot(a, b, n_); }
+  void lin(double *x, const double *x0, double a, const double *p) {
+    for (int64_t t = 0; t < n_; ++t) x[t] = x0[t] + a * p[t];
+  }
+  void diff(double *s, const double *x, double *y, const double *g) {
+    for (int64_t t = 0; t < n_; ++t) {
+      s[t] = x[t] - s[t];
+      y[t] = g[t] - y[t];
+    }
+  }
+  void store(int slot) {
+    S_[slot].swap(work_[kVecS]);
+    Y_[slot].swap(work_[kVecY]);
+  }
+  void direction(const double *g, double *p, const int *slots, const double *rho, int count) {
+    const int64_t n = n_;
+    std::memcpy(p, g, sizeof(double) * n);
+    for (int i = 0; i < count; ++i) {
+      const int s = slots[i];
+      coef_[i] = rho[i] * bfgs_dot(S_[s].data(), p, n);
+      const double c = -coef_[i];
+      const double *ys = Y_[s].data();
+      for (int64_t t = 0; t < n; ++t) p[t] += c * ys[t];
+    }
+    if (h0_)
+      for (int64_t t = 0; t < n; ++t) p[t] *= h0_[t];
+    for (int i = count - 1; i >= 0; --i) {
+      const int s = slots[i];
+      const double c2 = rho[i] * bfgs_dot(Y_[s].data(), p, n);
+      const double c = coef_[i] - c2;
+      const double *ss = S_[s].data();
+      for (int64_t t = 0; t < n; ++t) p[t] += c * ss[t];
+    }
+    for (int64_t t = 0; t < n; ++t) p[t] = -p[t];
+  }
+
+ private:
+  int64_t n_;
+  const double *h0_;
+  std::vector<std::vector<double>> work_, S_, Y_;
+  std::vector<double> coef_;
+};
+
+// noise_rel > 0 selects the noise-aware line search with noise floor
+// noise_rel |f|; 0 is the reference algorithm.
+// h0: diagonal of the initial inverse Hessian of the two-loop recursion
+// (nullptr: the identity, as deal.II's SolverBFGS).
+template <class Eval, class Check>
+BfgsResult lbfgs_solve(int64_t n, double *x, Eval &&eval, Check &&check, int history = 5,
+                       double noise_rel = 0.0, const double *h0 = nullptr) {
+  HostSpace sp(n, history, h0);
+  return lbfgs_core(sp, x, eval, check, history, noise_rel);
 }
 
 }  // namespace sdot

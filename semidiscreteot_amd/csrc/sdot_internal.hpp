@@ -462,14 +462,30 @@ hipError_t launch_hessian_chunk(const Plan &p, Dev &d, int64_t q_begin, int64_t 
 hipError_t launch_hessian_final(const Plan &p, const double *d_A, int64_t Npad, const double *d_g,
                                 const double *d_nu, double *d_H, hipStream_t s);
 
-// The two-loop recursion of L-BFGS on device vectors (sdot_lbfgs.hip).
-// S, Y: the stored pairs, slot k at S + k n; p: g on entry, -H g on return,
-// H the inverse-Hessian approximation of the pairs slots[0..count) (newest
-// first, with their rho) over the initial diag(h0) (h0 == nullptr: identity).
-// part: kLbParts partial sums; coef: one value per pair.
+// L-BFGS vector arithmetic on device vectors of length n (sdot_lbfgs.hip).
+// Reductions leave kLbParts partial values per quantity in `part` (a fixed
+// grid, so the host's in-order sum of them does not depend on the launch).
 constexpr int kLbParts = 512;
+constexpr int kLbMaxHist = 16;
+// part[0, kLbParts) = partial sums of a . b
+hipError_t launch_lb_dot(const double *a, const double *b, int64_t n, double *part, hipStream_t s);
+// x = x0 + a p
+hipError_t launch_lb_lin(double *x, const double *x0, double a, const double *p, int64_t n,
+                         hipStream_t s);
+// s = x - s, y = g - y
+hipError_t launch_lb_diff(double *sv, const double *x, double *y, const double *g, int64_t n,
+                          hipStream_t s);
+// kLbStats blocks of kLbParts partial values in part, for vector a (and nu):
+// sum |a|, sum a^2, max (|a| - t nu) (nu == nullptr: -inf), max a, min a
+enum { kLbL1 = 0, kLbL2, kLbCw, kLbMax, kLbMin, kLbStats };
+hipError_t launch_lb_stats(const double *a, const double *nu, double t, int64_t n, double *part,
+                           hipStream_t s);
+// Two-loop recursion: p holds g on entry and -H g on return, H the
+// inverse-Hessian approximation of the pairs (S[k], Y[k]) for k in
+// slots[0..count) (newest first, with their rho) over the initial diag(h0)
+// (h0 == nullptr: identity).  part: kLbParts partial sums; coef: count values.
 struct LbVecs {
-  const double *S, *Y;
+  const double *S[kLbMaxHist], *Y[kLbMaxHist];
   int64_t n;
   double *p;
   const double *h0;

@@ -399,22 +399,27 @@ def test_plain_2d_config_solve():
     s.close()
 
 
-@pytest.mark.parametrize("precond,precision", [("none", "fp64"), ("diagonal", "fp64"),
-                                               ("diagonal", "fp32_lse")])
-def test_device_lbfgs_pairs(precond, precision, monkeypatch):
-    """Solves of 65,536 targets and more keep the L-BFGS pairs in HBM and run the
-    two-loop recursion there (dot products over a fixed grid of partial sums).
+@pytest.mark.parametrize("precond,precision,control", [
+    ("none", "fp64", "l1norm"), ("diagonal", "fp64", "l1norm"),
+    ("diagonal", "fp64", "componentwise"), ("diagonal", "fp32_lse", "l1norm")])
+def test_device_lbfgs(precond, precision, control, monkeypatch):
+    """Solves of 65,536 targets and more keep every L-BFGS vector in HBM (iterate,
+    gradient, direction, pairs) and run the vector arithmetic there (dot
+    products and stopping statistics over a fixed grid of partial sums).
     Forced on (SDOT_LBFGS_DEVICE_MIN=0) for the plain 2D configuration
-    (1,000 targets, l1-norm tolerance 1e-8, 60 iterations from psi = 0), the
-    solve takes the decisions of the host recursion: same status and
-    evaluation count, potentials and functional to rounding (FP64); with
-    FP32 evaluations and the FP64 finish, the same status and potentials
-    within 1e-9 of their range."""
-    say(f"device L-BFGS pairs, {precond} {precision}")
+    (1,000 targets, 60 iterations from psi = 0), the solve
+    takes the decisions of the host driver: same status and evaluation count,
+    potentials and functional to rounding (FP64); with FP32 evaluations and
+    the FP64 finish, the same status and potentials within 1e-9 of their
+    range."""
+    say(f"device L-BFGS, {precond} {precision} {control}")
     src = O.structured_source(2, [64, 64], [0.0, 0.0], [1.0, 1.0], 3)
     y = np.random.default_rng(0).random((1000, 2))
     tgt = O.Target(y.ravel(), np.full(1000, 1e-3))
-    s = make(src, tgt, epsilon=1e-2, tau=0.0, tolerance=1e-8, solver_control_type="l1norm",
+    # componentwise: max_j |g_j| - 1e-5 nu_j, reached before the functional's
+    # rounding decides the reference line search
+    tol = 1e-8 if control == "l1norm" else 1e-5
+    s = make(src, tgt, epsilon=1e-2, tau=0.0, tolerance=tol, solver_control_type=control,
              use_log_sum_exp_trick=True, verbose_output=False, max_iterations=60,
              preconditioner=precond, precision=precision)
     monkeypatch.delenv("SDOT_LBFGS_DEVICE_MIN", raising=False)
