@@ -210,6 +210,14 @@ def test_covering_radius_exact():
     src, tgt = structured_case(3, 8, 2, 500, seed=4, gmm_targets=True)
     s = make(src, tgt, epsilon=1e-2)
     assert s.compute_covering_radius() == O.covering_radius(src, tgt)
+    # kept per source and target set: new targets, then a new source
+    src2, tgt2 = structured_case(3, 6, 3, 300, seed=5, gmm_targets=True)
+    s.setup_target(tgt2.y, tgt2.nu)
+    assert s.compute_covering_radius() == O.covering_radius(src, tgt2)
+    s.setup_source(src2.x, src2.w, src2.center, src2.nq)
+    assert s.compute_covering_radius() == O.covering_radius(src2, tgt2)
+    assert s.compute_covering_radius() == O.covering_radius(src2, tgt2)
+    s.close()
 
 
 def test_determinism_bitwise():
@@ -403,7 +411,7 @@ def test_plain_2d_config_solve():
     ("none", "fp64", "l1norm"), ("diagonal", "fp64", "l1norm"),
     ("diagonal", "fp64", "componentwise"), ("diagonal", "fp32_lse", "l1norm")])
 def test_device_lbfgs(precond, precision, control, monkeypatch):
-    """Solves of 65,536 targets and more keep every L-BFGS vector in HBM (iterate,
+    """Solves of 32,768 targets and more keep every L-BFGS vector in HBM (iterate,
     gradient, direction, pairs) and run the vector arithmetic there (dot
     products and stopping statistics over a fixed grid of partial sums).
     Forced on (SDOT_LBFGS_DEVICE_MIN=0) for the plain 2D configuration
