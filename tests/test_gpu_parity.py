@@ -451,6 +451,36 @@ def test_device_lbfgs(precond, precision, control, monkeypatch):
     s.close()
 
 
+@pytest.mark.parametrize("control", ["l1norm", "componentwise"])
+def test_device_lbfgs_iteration_lines(control, capfd, monkeypatch):
+    """The iteration lines of VerboseSolverControl::check (SotSolver.h:408-440)
+    from the statistics kernel of the device driver (L-2 norm, L-1 norm or the
+    componentwise residual, printed to 6 digits) match the host driver's line
+    for line on 12 FP64 iterations of the plain 2D configuration."""
+    say(f"device L-BFGS iteration lines, {control}")
+    src = O.structured_source(2, [64, 64], [0.0, 0.0], [1.0, 1.0], 3)
+    y = np.random.default_rng(0).random((1000, 2))
+    tgt = O.Target(y.ravel(), np.full(1000, 1e-3))
+    s = make(src, tgt, epsilon=1e-2, tau=0.0, tolerance=1e-8, solver_control_type=control,
+             use_log_sum_exp_trick=True, verbose_output=True, max_iterations=12,
+             preconditioner="diagonal")
+    lines = {}
+    for mode in ("host", "device"):
+        if mode == "device":
+            monkeypatch.setenv("SDOT_LBFGS_DEVICE_MIN", "0")
+        else:
+            monkeypatch.delenv("SDOT_LBFGS_DEVICE_MIN", raising=False)
+        capfd.readouterr()
+        s.solve()
+        out = capfd.readouterr().out
+        lines[mode] = [ln for ln in out.splitlines() if ln.startswith("Iteration ")]
+    monkeypatch.delenv("SDOT_LBFGS_DEVICE_MIN")
+    s.close()
+    say(f"{len(lines['host'])} lines; last: {lines['device'][-1]}")
+    assert len(lines["host"]) == 13
+    assert lines["device"] == lines["host"]
+
+
 @pytest.mark.parametrize("which", ["targets", "large_source"])
 def test_large_record_sets_match_oracle(which):
     """Record sets larger than an XCD's L2: 70,000 targets (4.5 MB of target
