@@ -268,9 +268,11 @@ int sdot_hessian(sdot_ctx *ctx, const double *psi, double *H, int64_t ld);
 /* SotSolver::solve (SotSolver.h:211-212, SotSolver.cc:105-215): L-BFGS with
  * the reference's line search and stopping test; psi in/out (size N).  In
  * FP32_LSE precision the iteration runs on FP32 evaluations until its stopping
- * value is within 10x the tolerance (or a line search fails), then continues
+ * value is within 2x the tolerance (or a line search fails), then continues
  * from there on FP64 evaluations within the same iteration budget, so that
- * the stopping decision is taken on an FP64 gradient. */
+ * the stopping decision is taken on an FP64 gradient.  From N = 32768 on, the
+ * L-BFGS vectors stay in device memory for the length of the call (18 N
+ * doubles); the environment variable SDOT_LBFGS_DEVICE_MIN moves that length. */
 int sdot_solve(sdot_ctx *ctx, double *psi, sdot_solve_info *info);
 
 /* ---- multilevel potential transfer ---------------------------------------- */
